@@ -1,0 +1,607 @@
+// amg.hip -- unsmoothed-aggregation AMG built and applied on the GPU.
+//
+// Replaces HYPRE BoomerAMG (LP/src/hypre_backend.cu:66-92, GSV1
+// main.cu:2239-2260).  Algorithm (identical to the CPU oracle, so hierarchies
+// agree bit for bit):
+//   coupling s_ij = -a_ij; strong if s_ij > 0 and s_ij >= theta*max_k s_ik
+//   6 synchronous handshake rounds: every unmatched row proposes its strongest
+//   unmatched strong neighbour (ties: symmetric edge hash); mutual proposals
+//   pair; leftovers join the pair of their strongest matched strong neighbour
+//   or stay single; aggregates numbered by their smallest member (scan)
+//   2 such passes per level (Galerkin aggregate sum in between)
+//   coarse matrix = P^T A P summed per coarse row in member, then CSR order
+//   V-cycle: one damped-Jacobi sweep from zero, residual, restriction,
+//   recursion, prolongation, one damped-Jacobi sweep; dense inverse on the
+//   coarsest level (FP64 in both precisions).
+// Setup kernels are one thread per row; every list built with atomics is
+// sorted before use, so the hierarchy is deterministic.
+#include "amg.hpp"
+
+#include <cmath>
+
+namespace an {
+
+static constexpr int kB = 256;
+static constexpr int kW = kB / 64;
+static constexpr int kCap = 768;
+static constexpr int kGalCap = 256;  // max entries of one coarse row
+
+__host__ __device__ __forceinline__ uint32_t edge_hash(uint32_t a, uint32_t b) {
+  uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  uint32_t x = lo * 0x9E3779B1u ^ (hi + 0x7F4A7C15u);
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return x;
+}
+
+// --------------------------------------------------------- matching ---
+__global__ void k_smax(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                       const double* __restrict__ va, double* __restrict__ smax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    if (ci[k] == i) continue;
+    double s = -va[k];
+    if (s > m) m = s;
+  }
+  smax[i] = m;
+}
+
+__global__ void k_fill_int(int n, int* __restrict__ a, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// mode 0: propose among unmatched neighbours (rows that are unmatched);
+// mode 1: absorption target among matched neighbours (rows unmatched).
+__global__ void k_cand(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                       const double* __restrict__ va, const double* __restrict__ smax,
+                       double theta, const int* __restrict__ match, int* __restrict__ cand,
+                       int mode) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (match[i] >= 0) {
+    cand[i] = -1;
+    return;
+  }
+  double bs = 0.0;
+  uint32_t bh = 0;
+  int best = -1;
+  const double thr = theta * smax[i];
+  for (int k = ro[i]; k < ro[i + 1]; ++k) {
+    int j = ci[k];
+    if (j == i) continue;
+    if (mode == 0 ? (match[j] >= 0) : (match[j] < 0)) continue;
+    double s = -va[k];
+    if (!(s > 0.0) || s < thr) continue;
+    uint32_t h = edge_hash((uint32_t)i, (uint32_t)j);
+    if (best < 0 || s > bs || (s == bs && h > bh)) {
+      best = j;
+      bs = s;
+      bh = h;
+    }
+  }
+  cand[i] = best;
+}
+
+__global__ void k_match(int n, const int* __restrict__ cand, int* __restrict__ match) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int j = cand[i];
+  if (j >= 0 && cand[j] == i) match[i] = j;
+}
+
+__global__ void k_rep(int n, const int* __restrict__ match, const int* __restrict__ cand2,
+                      int* __restrict__ rep) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rep[i] = match[i] >= 0 ? (i < match[i] ? 1 : 0) : (cand2[i] < 0 ? 1 : 0);
+}
+
+__global__ void k_assign(int n, const int* __restrict__ match, const int* __restrict__ cand2,
+                         const int* __restrict__ id, int* __restrict__ agg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int r;
+  if (match[i] >= 0)
+    r = i < match[i] ? i : match[i];
+  else if (cand2[i] >= 0) {
+    int j = cand2[i];
+    r = match[j] < j ? match[j] : j;
+  } else
+    r = i;
+  agg[i] = id[r];
+}
+
+__global__ void k_compose(int n, int* __restrict__ agg, const int* __restrict__ agg2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) agg[i] = agg2[agg[i]];
+}
+
+__global__ void k_count_members(int n, const int* __restrict__ agg, int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[agg[i]], 1);
+}
+
+__global__ void k_fill_members(int n, const int* __restrict__ agg, int* __restrict__ cursor,
+                               int* __restrict__ mem) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) mem[atomicAdd(&cursor[agg[i]], 1)] = i;
+}
+
+__global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __restrict__ mem) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  int b = moff[I], e = moff[I + 1];
+  for (int a = b + 1; a < e; ++a) {
+    int x = mem[a];
+    int j = a - 1;
+    while (j >= b && mem[j] > x) {
+      mem[j + 1] = mem[j];
+      --j;
+    }
+    mem[j + 1] = x;
+  }
+}
+
+// ---------------------------------------------------------- Galerkin ---
+__global__ void k_gal_count(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
+                            const int* __restrict__ ro, const int* __restrict__ ci,
+                            const int* __restrict__ agg, int* __restrict__ cnt,
+                            int* __restrict__ overflow) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  int kJ[kGalCap];
+  int c = 0;
+  for (int m = moff[I]; m < moff[I + 1]; ++m) {
+    int i = mem[m];
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      int J = agg[ci[k]];
+      bool found = false;
+      for (int t = 0; t < c; ++t)
+        if (kJ[t] == J) {
+          found = true;
+          break;
+        }
+      if (!found) {
+        if (c == kGalCap) {
+          atomicOr(overflow, 1);
+          cnt[I] = c;
+          return;
+        }
+        kJ[c++] = J;
+      }
+    }
+  }
+  cnt[I] = c;
+}
+
+__global__ void k_gal_fill(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
+                           const int* __restrict__ ro, const int* __restrict__ ci,
+                           const double* __restrict__ va, const int* __restrict__ agg,
+                           const int* __restrict__ cro, int* __restrict__ cci,
+                           double* __restrict__ cva) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  int kJ[kGalCap];
+  double kV[kGalCap];
+  int c = 0;
+  for (int m = moff[I]; m < moff[I + 1]; ++m) {
+    int i = mem[m];
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      int J = agg[ci[k]];
+      int t = 0;
+      for (; t < c; ++t)
+        if (kJ[t] == J) break;
+      if (t == c) {
+        kJ[c] = J;
+        kV[c] = va[k];
+        ++c;
+      } else {
+        kV[t] += va[k];
+      }
+    }
+  }
+  for (int a = 1; a < c; ++a) {
+    int j = kJ[a];
+    double v = kV[a];
+    int b = a - 1;
+    while (b >= 0 && kJ[b] > j) {
+      kJ[b + 1] = kJ[b];
+      kV[b + 1] = kV[b];
+      --b;
+    }
+    kJ[b + 1] = j;
+    kV[b + 1] = v;
+  }
+  const int o = cro[I];
+  for (int t = 0; t < c; ++t) {
+    cci[o + t] = kJ[t];
+    cva[o + t] = kV[t];
+  }
+}
+
+template <typename T>
+__global__ void k_omdinv(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                         const double* __restrict__ va, double omega, T* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k)
+    if (ci[k] == i) d = va[k];
+  out[i] = (T)((d != 0.0) ? omega * (1.0 / d) : 0.0);
+}
+
+template <typename T>
+__global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict__ b) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (T)a[i];
+}
+
+// --------------------------------------------------------- V-cycle -----
+// KIND 0 (down): out_i = b_i - sum_j a_ij x_j,  x_j = omdinv_j b_j
+// KIND 1 (up):   x_j = omdinv_j b_j + xc[agg_j];
+//                out_i = x_i + omdinv_i (b_i - sum_j a_ij x_j)
+// CSR-stream over 64-row wave slices (see linalg.hip).
+template <typename T, int KIND>
+__global__ __launch_bounds__(kB) void k_level_rows(int n, const int* __restrict__ ro,
+                                                   const int* __restrict__ ci,
+                                                   const T* __restrict__ va,
+                                                   const T* __restrict__ omdinv,
+                                                   const T* __restrict__ b,
+                                                   const T* __restrict__ xc,
+                                                   const int* __restrict__ agg,
+                                                   T* __restrict__ out) {
+  __shared__ double prod[kW][kCap];
+  const int lb = xcd_block(gridDim.x);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = (lb * kW + w) * 64;
+  const int r1 = min(r0 + 64, n);
+  int k0 = 0, k1 = 0;
+  bool stream = false;
+  if (r0 < n) {
+    k0 = ro[r0];
+    k1 = ro[r1];
+    stream = (k1 - k0) <= kCap;
+  }
+  if (stream)
+    for (int k = k0 + lane; k < k1; k += 64) {
+      const int j = ci[k];
+      double xj = (double)omdinv[j] * (double)b[j];
+      if (KIND == 1) xj = xj + (double)xc[agg[j]];
+      prod[w][k - k0] = (double)va[k] * xj;
+    }
+  __syncthreads();
+  const int i = r0 + lane;
+  if (i < r1) {
+    double s = 0.0;
+    const int kb = ro[i], ke = ro[i + 1];
+    if (stream) {
+      for (int k = kb; k < ke; ++k) s += prod[w][k - k0];
+    } else {
+      for (int k = kb; k < ke; ++k) {
+        const int j = ci[k];
+        double xj = (double)omdinv[j] * (double)b[j];
+        if (KIND == 1) xj = xj + (double)xc[agg[j]];
+        s += (double)va[k] * xj;
+      }
+    }
+    const double bi = (double)b[i];
+    if (KIND == 0) {
+      out[i] = (T)(bi - s);
+    } else {
+      double xi = (double)omdinv[i] * bi;
+      xi = xi + (double)xc[agg[i]];
+      out[i] = (T)(xi + (double)omdinv[i] * (bi - s));
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
+                           const T* __restrict__ r, T* __restrict__ bc) {
+  int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int m = moff[I]; m < moff[I + 1]; ++m) s += (double)r[mem[m]];
+  bc[I] = (T)s;
+}
+
+// coarsest: x_i = sum_j Ainv[i][j] b_j, Ainv stored column-major (FP64)
+template <typename T>
+__global__ void k_dense_solve(int n, const double* __restrict__ invT, const T* __restrict__ b,
+                              T* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += invT[(size_t)j * n + i] * (double)b[j];
+  x[i] = (T)s;
+}
+
+// plain damped Jacobi sweep (coarsest fallback when too large for dense)
+template <typename T>
+__global__ void k_jacobi(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                         const T* __restrict__ va, const T* __restrict__ omdinv,
+                         const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) s += (double)va[k] * (double)x[ci[k]];
+  y[i] = (T)((double)x[i] + (double)omdinv[i] * ((double)b[i] - s));
+}
+
+// ------------------------------------------------------------ host -----
+std::vector<double> dense_inverse_host(int n, const std::vector<int>& ro,
+                                       const std::vector<int>& ci, const std::vector<double>& va) {
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = ro[i]; k < ro[i + 1]; ++k) A[(size_t)i * n + ci[k]] += va[k];
+  double dmax = 0.0;
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(A[(size_t)i * n + i]) > dmax) dmax = std::fabs(A[(size_t)i * n + i]);
+  for (int k = 0; k < n; ++k) {
+    double d = A[(size_t)k * n + k];
+    for (int j = 0; j < k; ++j) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
+    if (d <= 1e-13 * dmax) {
+      for (int i = k; i < n; ++i) A[(size_t)i * n + k] = 0.0;
+      continue;
+    }
+    double l = std::sqrt(d);
+    A[(size_t)k * n + k] = l;
+    for (int i = k + 1; i < n; ++i) {
+      double s = A[(size_t)i * n + k];
+      for (int j = 0; j < k; ++j) s -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
+      A[(size_t)i * n + k] = s / l;
+    }
+  }
+  std::vector<double> inv((size_t)n * n, 0.0), y(n);
+  for (int col = 0; col < n; ++col) {
+    for (int i = 0; i < n; ++i) {
+      double s = (i == col) ? 1.0 : 0.0;
+      for (int j = 0; j < i; ++j) s -= A[(size_t)i * n + j] * y[j];
+      double l = A[(size_t)i * n + i];
+      y[i] = (l != 0.0) ? s / l : 0.0;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      for (int j = i + 1; j < n; ++j) s -= A[(size_t)j * n + i] * inv[(size_t)j * n + col];
+      double l = A[(size_t)i * n + i];
+      inv[(size_t)i * n + col] = (l != 0.0) ? s / l : 0.0;
+    }
+  }
+  return inv;
+}
+
+static int read_int(const int* d, hipStream_t s) {
+  int h = 0;
+  AN_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+// One pairwise pass on W (n rows): agg[n] and the aggregate count.
+static int pair_pass(const DevCSR<double>& W, double theta, DevBuf<int>& agg, hipStream_t s) {
+  const int n = W.n;
+  const int G = grid_for(n, kB);
+  DevBuf<double> smax((size_t)n);
+  DevBuf<int> match((size_t)n), cand((size_t)n), rep((size_t)n + 1), id((size_t)n + 1);
+  k_smax<<<G, kB, 0, s>>>(n, W.rowOff.p, W.cols.p, W.vals.p, smax.p);
+  k_fill_int<<<G, kB, 0, s>>>(n, match.p, -1);
+  for (int r = 0; r < 6; ++r) {
+    k_cand<<<G, kB, 0, s>>>(n, W.rowOff.p, W.cols.p, W.vals.p, smax.p, theta, match.p, cand.p, 0);
+    k_match<<<G, kB, 0, s>>>(n, cand.p, match.p);
+  }
+  k_cand<<<G, kB, 0, s>>>(n, W.rowOff.p, W.cols.p, W.vals.p, smax.p, theta, match.p, cand.p, 1);
+  k_rep<<<G, kB, 0, s>>>(n, match.p, cand.p, rep.p);
+  AN_CHECK_LAUNCH();
+  long long nc = exclusive_scan(rep.p, id.p, n, s);
+  agg.alloc((size_t)n);
+  k_assign<<<G, kB, 0, s>>>(n, match.p, cand.p, id.p, agg.p);
+  AN_CHECK_LAUNCH();
+  return (int)nc;
+}
+
+static void build_members(int n, int nc, const int* agg, DevBuf<int>& moff, DevBuf<int>& mem,
+                          hipStream_t s) {
+  DevBuf<int> cnt((size_t)nc + 1);
+  cnt.zero(s);
+  k_count_members<<<grid_for(n, kB), kB, 0, s>>>(n, agg, cnt.p);
+  AN_CHECK_LAUNCH();
+  moff.alloc((size_t)nc + 1);
+  exclusive_scan(cnt.p, moff.p, nc, s);
+  DevBuf<int> cursor((size_t)nc + 1);
+  AN_HIP(hipMemcpyAsync(cursor.p, moff.p, sizeof(int) * ((size_t)nc + 1), hipMemcpyDeviceToDevice,
+                        s));
+  mem.alloc((size_t)n);
+  k_fill_members<<<grid_for(n, kB), kB, 0, s>>>(n, agg, cursor.p, mem.p);
+  k_sort_members<<<grid_for(nc, kB), kB, 0, s>>>(nc, moff.p, mem.p);
+  AN_CHECK_LAUNCH();
+}
+
+static void galerkin(const DevCSR<double>& W, int nc, const int* agg, const int* moff,
+                     const int* mem, DevCSR<double>& C, hipStream_t s) {
+  DevBuf<int> cnt((size_t)nc + 1), overflow(1);
+  overflow.zero(s);
+  k_gal_count<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, agg, cnt.p,
+                                                overflow.p);
+  AN_CHECK_LAUNCH();
+  if (read_int(overflow.p, s)) throw Error(7, "AMG coarse row exceeds 256 entries");
+  C.n = nc;
+  C.rowOff.alloc((size_t)nc + 1);
+  C.nnz = exclusive_scan(cnt.p, C.rowOff.p, nc, s);
+  C.cols.alloc((size_t)C.nnz);
+  C.vals.alloc((size_t)C.nnz);
+  k_gal_fill<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, W.vals.p, agg,
+                                               C.rowOff.p, C.cols.p, C.vals.p);
+  AN_CHECK_LAUNCH();
+}
+
+void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
+  lv.clear();
+  has32 = false;
+  lv.reserve(opt.maxLevels + 1);
+  lv.emplace_back();
+  lv[0].n = fine.n;
+  lv[0].A = &fine;
+  const int maxL = opt.maxLevels > 1 ? opt.maxLevels : 2;
+  for (int l = 0;; ++l) {
+    AmgLevel& L = lv[l];
+    L.omdinv.alloc((size_t)L.n);
+    k_omdinv<double><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p,
+                                                      L.A->vals.p, opt.omega, L.omdinv.p);
+    AN_CHECK_LAUNCH();
+    L.b.alloc((size_t)L.n);
+    L.x.alloc((size_t)L.n);
+    L.r.alloc((size_t)L.n);
+    if (L.n <= opt.coarseSize || l == maxL - 1) {
+      L.isCoarsest = true;
+      break;
+    }
+    // composed pairwise passes
+    DevBuf<int> agg;
+    int cur = pair_pass(*L.A, opt.strength, agg, s);
+    DevCSR<double> W;
+    {
+      DevBuf<int> mo, me;
+      build_members(L.n, cur, agg.p, mo, me, s);
+      galerkin(*L.A, cur, agg.p, mo.p, me.p, W, s);
+    }
+    for (int p = 1; p < (opt.pairPasses > 0 ? opt.pairPasses : 1); ++p) {
+      DevBuf<int> a2;
+      int n2 = pair_pass(W, opt.strength, a2, s);
+      DevCSR<double> W2;
+      DevBuf<int> mo, me;
+      build_members(W.n, n2, a2.p, mo, me, s);
+      galerkin(W, n2, a2.p, mo.p, me.p, W2, s);
+      k_compose<<<grid_for(L.n, kB), kB, 0, s>>>(L.n, agg.p, a2.p);
+      AN_CHECK_LAUNCH();
+      W = std::move(W2);
+      cur = n2;
+    }
+    if (cur >= L.n || (double)cur > 0.9 * (double)L.n) {
+      L.isCoarsest = true;
+      break;
+    }
+    L.nc = cur;
+    L.agg = std::move(agg);
+    build_members(L.n, cur, L.agg.p, L.moff, L.mem, s);
+    lv.emplace_back();
+    AmgLevel& C = lv.back();
+    C.n = cur;
+    C.own = std::move(W);
+    C.A = &lv.back().own;
+  }
+  // lv may have reallocated while growing: re-point owned matrices
+  for (size_t l = 1; l < lv.size(); ++l) lv[l].A = &lv[l].own;
+  AmgLevel& Lc = lv.back();
+  if (Lc.n <= 4096) {
+    const DevCSR<double>& A = *Lc.A;
+    std::vector<int> ro(Lc.n + 1), ci((size_t)A.nnz);
+    std::vector<double> va((size_t)A.nnz);
+    AN_HIP(hipMemcpyAsync(ro.data(), A.rowOff.p, sizeof(int) * (Lc.n + 1), hipMemcpyDeviceToHost, s));
+    AN_HIP(hipMemcpyAsync(ci.data(), A.cols.p, sizeof(int) * A.nnz, hipMemcpyDeviceToHost, s));
+    AN_HIP(hipMemcpyAsync(va.data(), A.vals.p, sizeof(double) * A.nnz, hipMemcpyDeviceToHost, s));
+    AN_HIP(hipStreamSynchronize(s));
+    std::vector<double> inv = dense_inverse_host(Lc.n, ro, ci, va);
+    std::vector<double> invT((size_t)Lc.n * Lc.n);
+    for (int i = 0; i < Lc.n; ++i)
+      for (int j = 0; j < Lc.n; ++j) invT[(size_t)j * Lc.n + i] = inv[(size_t)i * Lc.n + j];
+    Lc.coarseInvT.upload(invT, s);
+  }
+  AN_HIP(hipStreamSynchronize(s));
+  built = true;
+}
+
+void AmgHierarchy::refresh_fine(hipStream_t s) {
+  if (!built) return;
+  AmgLevel& L = lv[0];
+  k_omdinv<double><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p, L.A->vals.p,
+                                                    opt.omega, L.omdinv.p);
+  AN_CHECK_LAUNCH();
+  if (has32) {
+    k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p, L.A->vals.p,
+                                                     opt.omega, L.omdinv32.p);
+    k_cast<float><<<grid_for(L.A->nnz, kB), kB, 0, s>>>(L.A->nnz, L.A->vals.p, L.A32.vals.p);
+    AN_CHECK_LAUNCH();
+  }
+}
+
+void AmgHierarchy::make_fp32(hipStream_t s) {
+  for (auto& L : lv) {
+    const DevCSR<double>& A = *L.A;
+    L.A32.n = A.n;
+    L.A32.nnz = A.nnz;
+    L.A32.rowOff.alloc((size_t)A.n + 1);
+    AN_HIP(hipMemcpyAsync(L.A32.rowOff.p, A.rowOff.p, sizeof(int) * ((size_t)A.n + 1),
+                          hipMemcpyDeviceToDevice, s));
+    L.A32.cols.alloc((size_t)A.nnz);
+    AN_HIP(hipMemcpyAsync(L.A32.cols.p, A.cols.p, sizeof(int) * (size_t)A.nnz,
+                          hipMemcpyDeviceToDevice, s));
+    L.A32.vals.alloc((size_t)A.nnz);
+    k_cast<float><<<grid_for(A.nnz, kB), kB, 0, s>>>(A.nnz, A.vals.p, L.A32.vals.p);
+    L.omdinv32.alloc((size_t)L.n);
+    k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, opt.omega,
+                                                     L.omdinv32.p);
+    AN_CHECK_LAUNCH();
+    L.b32.alloc((size_t)L.n);
+    L.x32.alloc((size_t)L.n);
+    L.r32.alloc((size_t)L.n);
+  }
+  has32 = true;
+}
+
+template <typename T>
+static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, T* x, hipStream_t s,
+                   const T* (*omd)(const AmgLevel&), const DevCSR<T>& (*mat)(const AmgLevel&),
+                   T* (*bbuf)(AmgLevel&), T* (*xbuf)(AmgLevel&), T* (*rbuf)(AmgLevel&)) {
+  AmgLevel& L = lv[l];
+  const DevCSR<T>& A = mat(L);
+  if (L.isCoarsest) {
+    if (L.coarseInvT.p) {
+      k_dense_solve<T><<<grid_for(L.n, 64), 64, 0, s>>>(L.n, L.coarseInvT.p, b, x);
+    } else {
+      T* t = rbuf(L);
+      fill_zero(x, sizeof(T) * (size_t)L.n, s);
+      for (int it = 0; it < 20; ++it) {
+        T* src = (it & 1) ? t : x;
+        T* dst = (it & 1) ? x : t;
+        k_jacobi<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L), b,
+                                                     src, dst);
+      }
+    }
+    AN_CHECK_LAUNCH();
+    return;
+  }
+  AmgLevel& C = lv[l + 1];
+  T* r = rbuf(L);
+  k_level_rows<T, 0><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L),
+                                                      b, nullptr, nullptr, r);
+  k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, r, bbuf(C));
+  AN_CHECK_LAUNCH();
+  vcycle<T>(lv, l + 1, bbuf(C), xbuf(C), s, omd, mat, bbuf, xbuf, rbuf);
+  k_level_rows<T, 1><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L),
+                                                      b, xbuf(C), L.agg.p, x);
+  AN_CHECK_LAUNCH();
+}
+
+void AmgHierarchy::apply(const double* r, double* z, hipStream_t s) {
+  vcycle<double>(
+      lv, 0, r, z, s, [](const AmgLevel& L) -> const double* { return L.omdinv.p; },
+      [](const AmgLevel& L) -> const DevCSR<double>& { return *L.A; },
+      [](AmgLevel& L) -> double* { return L.b.p; }, [](AmgLevel& L) -> double* { return L.x.p; },
+      [](AmgLevel& L) -> double* { return L.r.p; });
+}
+
+void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s) {
+  vcycle<float>(
+      lv, 0, r, z, s, [](const AmgLevel& L) -> const float* { return L.omdinv32.p; },
+      [](const AmgLevel& L) -> const DevCSR<float>& { return L.A32; },
+      [](AmgLevel& L) -> float* { return L.b32.p; }, [](AmgLevel& L) -> float* { return L.x32.p; },
+      [](AmgLevel& L) -> float* { return L.r32.p; });
+}
+
+}  // namespace an

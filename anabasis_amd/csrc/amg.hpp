@@ -1,0 +1,54 @@
+// amg.hpp -- aggregation AMG hierarchy (replacement for HYPRE BoomerAMG).
+#pragma once
+
+#include "linalg.hpp"
+
+namespace an {
+
+struct AmgOptions {
+  int coarseSize = 256;
+  int maxLevels = 25;
+  double strength = 0.25;
+  int pairPasses = 2;
+  double omega = 0.75;
+};
+
+struct AmgLevel {
+  int n = 0, nc = 0;
+  DevCSR<double> own;           // owned matrix (levels >= 1)
+  const DevCSR<double>* A = nullptr;
+  DevBuf<double> omdinv;        // omega / a_ii
+  DevBuf<int> agg, moff, mem;   // aggregation to level+1
+  DevBuf<double> coarseInvT;    // coarsest: dense inverse, column-major
+  DevBuf<double> b, x, r;       // work vectors (FP64)
+  // FP32 copies for the mixed-precision V-cycle
+  DevCSR<float> A32;
+  DevBuf<float> omdinv32, b32, x32, r32;
+  bool isCoarsest = false;
+};
+
+class AmgHierarchy {
+ public:
+  AmgOptions opt;
+  std::vector<AmgLevel> lv;
+  bool built = false;
+  bool has32 = false;
+
+  // Builds the hierarchy from the fine matrix (kept by reference).
+  void setup(const DevCSR<double>& fine, hipStream_t s);
+  // Refreshes level-0 smoother data after the fine values changed (stale
+  // coarse levels, as a HYPRE hierarchy reused between setups).
+  void refresh_fine(hipStream_t s);
+  void make_fp32(hipStream_t s);
+  // z = M^-1 r, one V-cycle from zero.
+  void apply(const double* r, double* z, hipStream_t s);
+  void apply32(const float* r, float* z, hipStream_t s);
+  int levels() const { return (int)lv.size(); }
+};
+
+// Host dense inverse used for the coarsest level (Cholesky; pivots below
+// 1e-13*max|diag| treated as zero), shared with the oracle's algorithm.
+std::vector<double> dense_inverse_host(int n, const std::vector<int>& ro,
+                                       const std::vector<int>& ci, const std::vector<double>& va);
+
+}  // namespace an

@@ -1,0 +1,137 @@
+// an_internal.hpp -- shared internals of the anabasis MI355X library.
+//
+// Device data layout (HBM):
+//   cells           double4 cgeo[nC] = (ccx, ccy, ccz, vol)
+//   internal faces  int2 fPN[nIF] (owner, neighbour); SoA doubles
+//                   fLam (interp. weight), fDpn (n.d), fA (area),
+//                   fN{x,y,z} (unit normal), fD{x,y,z} (cc_N - cc_P)
+//   boundary faces  int bP[nBF] (owner), int bPatch[nBF] (0-based),
+//                   SoA doubles bDpn, bA, bN{x,y,z}, bD{x,y,z} (xf - cc_P)
+//   addressing      CSR rowOff[nC+1], cols[nnz], diagPos[nC],
+//                   facePP/PN/NP/NN[nIF];
+//                   cell->internal-face incidence in face order
+//                   cfOff[nC+1], cfEnt[2nIF] = face<<1 | side (0 owner,
+//                   1 neighbour), cfSlot[2nIF] = nnzSlot<<1 | firstUse;
+//                   cell->boundary-face incidence cbOff[nC+1], cbFace[nBF]
+//                   (boundary-face index bf = f - nIF, in face order).
+// Every derived per-face quantity is computed on the host with the
+// reference's expressions (-ffp-contract=off), so kernels that read it see
+// the reference's bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host_mesh.hpp"
+
+namespace an {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define AN_HIP(stmt)                                                             \
+  do {                                                                           \
+    hipError_t _e = (stmt);                                                      \
+    if (_e != hipSuccess)                                                        \
+      throw ::an::Error(3, std::string("HIP error ") + hipGetErrorString(_e) +  \
+                               " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define AN_CHECK_LAUNCH() AN_HIP(hipGetLastError())
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) AN_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+  void ensure(size_t count) { if (count > n || !p) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count, hipStream_t s = 0) {
+    ensure(count);
+    if (count) AN_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& h, hipStream_t s = 0) { upload(h.data(), h.size(), s); }
+  void download(T* h, size_t count, hipStream_t s = 0) const {
+    if (count) AN_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  void zero(hipStream_t s = 0) {
+    if (n) AN_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+// CSR addressing + incidence built on the GPU (addressing.hip).
+struct DevAddressing {
+  int nC = 0, nF = 0, nIF = 0, nBF = 0;
+  long long nnz = 0;
+  DevBuf<int> rowOff, cols, diagPos;
+  DevBuf<int> facePP, facePN, faceNP, faceNN;
+  DevBuf<int> cfOff, cfEnt, cfSlot;
+  DevBuf<int> cbOff, cbFace;
+  DevBuf<int> owner, neigh;  // raw face addressing on device
+};
+
+// Builds addressing from owner[nF], neigh[nIF] (device pointers).
+void build_addressing(DevAddressing& A, int nC, int nF, int nIF, const int* d_owner,
+                      const int* d_neigh, hipStream_t s);
+
+struct DevMesh {
+  HostMesh host;
+  bool onDevice = false;
+  int nC = 0, nF = 0, nIF = 0, nBF = 0, nPatches = 0;
+  DevAddressing addr;
+  DevBuf<double4> cgeo;
+  DevBuf<int2> fPN;
+  DevBuf<double> fLam, fDpn, fA, fNx, fNy, fNz, fDx, fDy, fDz;
+  DevBuf<int> bP, bPatch;
+  DevBuf<double> bDpn, bA, bNx, bNy, bNz, bDx, bDy, bDz;
+  void to_device(hipStream_t s);
+};
+
+// exclusive scan of n ints into out[0..n]; returns out[n] (synchronises).
+long long exclusive_scan(const int* d_in, int* d_out, int n, hipStream_t s);
+
+inline int grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  return (int)(g > 0 ? g : 1);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
+// the hardware block id so each XCD works on one contiguous chunk of rows and
+// keeps its neighbour gathers inside its own L2.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  const int b = blockIdx.x;
+  const int per = (nblocks + 7) >> 3;
+  const int xcd = b & 7, k = b >> 3;
+  const int full = nblocks & 7 ? (nblocks & 7) : 8;  // XCDs with 'per' blocks
+  int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + k;
+}
+
+}  // namespace an

@@ -1,0 +1,219 @@
+// gradient.hip -- least-squares cell gradients, one thread per cell.
+//
+// GSV1 semantics (mode 0): build_lsq_gradient_operator + apply
+// (apps/generic_simple_v1/src/main.cu:1159-1251, 1403-1440).  The reference
+// stores a per-term operator (src, face, cx, cy, cz: 32 B/term, 429 MB at
+// 128^3); here the operator is never stored.  Each thread walks its CSR row
+// twice: pass 1 accumulates the 3x3 normal matrix from the neighbour
+// centroids, pass 2 rebuilds each term's coefficient c_t = M^-1 (w r_t) and
+// applies it, in the reference's term order (sorted neighbours, then the
+// cell's boundary faces in face order) with the reference's arithmetic, so
+// the result carries the reference's bits while reading only the CSR row,
+// the centroid table and phi.  The second walk hits L1/L2.
+//
+// LP semantics (mode 1): compute_lsq_gradient (LP/src/gradient.cu:21-83):
+// Neumann boundary faces use the normal constraint r = dn n, dphi = g dn.
+#include "kernels.hpp"
+
+namespace an {
+
+__device__ __forceinline__ void acc_M(double M[9], double r0, double r1, double r2, double w) {
+  // M[i][j] += w*r[i]*r[j] as the reference writes it: (w*r_i)*r_j is not
+  // bitwise symmetric, so all nine entries are accumulated.
+  const double a0 = w * r0, a1 = w * r1, a2 = w * r2;
+  M[0] += a0 * r0;
+  M[1] += a0 * r1;
+  M[2] += a0 * r2;
+  M[3] += a1 * r0;
+  M[4] += a1 * r1;
+  M[5] += a1 * r2;
+  M[6] += a2 * r0;
+  M[7] += a2 * r1;
+  M[8] += a2 * r2;
+}
+
+// explicit adjugate inverse (main.cu:1221-1232); returns false (zero
+// operator) when |det| <= 1e-20.
+__device__ __forceinline__ bool inv3(const double M[9], double inv[9]) {
+  const double a = M[0], b = M[1], c = M[2];
+  const double d = M[3], e = M[4], f2 = M[5];
+  const double g = M[6], h = M[7], k = M[8];
+  const double det = a * (e * k - f2 * h) - b * (d * k - f2 * g) + c * (d * h - e * g);
+  if (!(fabs(det) > 1e-20)) {
+    for (int i = 0; i < 9; ++i) inv[i] = 0.0;
+    return false;
+  }
+  inv[0] = (e * k - f2 * h) / det;
+  inv[1] = (c * h - b * k) / det;
+  inv[2] = (b * f2 - c * e) / det;
+  inv[3] = (f2 * g - d * k) / det;
+  inv[4] = (a * k - c * g) / det;
+  inv[5] = (c * d - a * f2) / det;
+  inv[6] = (d * h - e * g) / det;
+  inv[7] = (b * g - a * h) / det;
+  inv[8] = (a * e - b * d) / det;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_lsq_gsv1(MeshView m, BCView bc,
+                                                  const double* __restrict__ phi,
+                                                  double* __restrict__ gx, double* __restrict__ gy,
+                                                  double* __restrict__ gz) {
+  const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (P >= m.nC) return;
+  const double4 xP = m.cgeo[P];
+  const int rb = m.rowOff[P], re = m.rowOff[P + 1];
+  const int bb = m.cbOff[P], be = m.cbOff[P + 1];
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = rb; k < re; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+  }
+  for (int j = bb; j < be; ++j) {
+    const int b = m.cbFace[j];
+    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+  }
+  double inv[9];
+  inv3(M, inv);
+  const double phiP = phi[P];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int k = rb; k < re; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+    const double dphi = phi[N] - phiP;
+    sx += cx * dphi;
+    sy += cy * dphi;
+    sz += cz * dphi;
+  }
+  for (int j = bb; j < be; ++j) {
+    const int b = m.cbFace[j];
+    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+    const double val = (bc.type[b] == 1) ? bc.val[b] : phiP;
+    const double dphi = val - phiP;
+    sx += cx * dphi;
+    sy += cy * dphi;
+    sz += cz * dphi;
+  }
+  gx[P] = sx;
+  gy[P] = sy;
+  gz[P] = sz;
+}
+
+__global__ __launch_bounds__(256) void k_lsq_lp(MeshView m, BCView bc,
+                                                const double* __restrict__ phi,
+                                                double* __restrict__ gx, double* __restrict__ gy,
+                                                double* __restrict__ gz) {
+  const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (P >= m.nC) return;
+  const double4 xP = m.cgeo[P];
+  const double phiP = phi[P];
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double rh0 = 0.0, rh1 = 0.0, rh2 = 0.0;
+  for (int k = m.rowOff[P]; k < m.rowOff[P + 1]; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double dphi = phi[N] - phiP;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+    rh0 += w * r0 * dphi;
+    rh1 += w * r1 * dphi;
+    rh2 += w * r2 * dphi;
+  }
+  for (int j = m.cbOff[P]; j < m.cbOff[P + 1]; ++j) {
+    const int b = m.cbFace[j];
+    double r0, r1, r2, dphi;
+    if (bc.type[b] == 1) {
+      r0 = m.bDx[b];
+      r1 = m.bDy[b];
+      r2 = m.bDz[b];
+      dphi = bc.val[b] - phiP;
+    } else {
+      const double n0 = m.bNx[b], n1 = m.bNy[b], n2 = m.bNz[b];
+      double dn = m.bDx[b] * n0 + m.bDy[b] * n1 + m.bDz[b] * n2;
+      dn = dn > 1e-30 ? dn : 1e-30;
+      r0 = dn * n0;
+      r1 = dn * n1;
+      r2 = dn * n2;
+      dphi = bc.val[b] * dn;
+    }
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+    rh0 += w * r0 * dphi;
+    rh1 += w * r1 * dphi;
+    rh2 += w * r2 * dphi;
+  }
+  double inv[9];
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (inv3(M, inv)) {
+    g0 = inv[0] * rh0 + inv[1] * rh1 + inv[2] * rh2;
+    g1 = inv[3] * rh0 + inv[4] * rh1 + inv[5] * rh2;
+    g2 = inv[6] * rh0 + inv[7] * rh1 + inv[8] * rh2;
+  }
+  gx[P] = g0;
+  gy[P] = g1;
+  gz[P] = g2;
+}
+
+void launch_lsq_gradient(const MeshView& m, int mode, BCView bc, const double* phi, double* gx,
+                         double* gy, double* gz, hipStream_t s) {
+  const int B = 256;
+  if (mode == 0)
+    k_lsq_gsv1<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
+  else
+    k_lsq_lp<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
+  AN_CHECK_LAUNCH();
+}
+
+__global__ void k_aos_to_soa(int n, const double* __restrict__ a, double* __restrict__ x,
+                             double* __restrict__ y, double* __restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = a[3 * (size_t)i];
+  y[i] = a[3 * (size_t)i + 1];
+  z[i] = a[3 * (size_t)i + 2];
+}
+__global__ void k_soa_to_aos(int n, const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, double* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a[3 * (size_t)i] = x[i];
+  a[3 * (size_t)i + 1] = y[i];
+  a[3 * (size_t)i + 2] = z[i];
+}
+void launch_aos_to_soa(int n, const double* aos, double* x, double* y, double* z, hipStream_t s) {
+  k_aos_to_soa<<<grid_for(n, 256), 256, 0, s>>>(n, aos, x, y, z);
+  AN_CHECK_LAUNCH();
+}
+void launch_soa_to_aos(int n, const double* x, const double* y, const double* z, double* aos,
+                       hipStream_t s) {
+  k_soa_to_aos<<<grid_for(n, 256), 256, 0, s>>>(n, x, y, z, aos);
+  AN_CHECK_LAUNCH();
+}
+
+}  // namespace an

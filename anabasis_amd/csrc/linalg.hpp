@@ -1,0 +1,56 @@
+// linalg.hpp -- sparse/vector primitives with device-resident scalars.
+#pragma once
+
+#include "an_internal.hpp"
+
+namespace an {
+
+// Device CSR matrix (values in T).
+template <typename T>
+struct DevCSR {
+  int n = 0;
+  long long nnz = 0;
+  DevBuf<int> rowOff, cols, diag;
+  DevBuf<T> vals;
+};
+
+// Scalar slots of a solver (device memory).
+enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 5, S_TMP2 = 6,
+                  S_AL = 7, S_OM = 8, S_R0V = 9, S_TS = 10, S_TT = 11, S_SS = 12, S_COUNT = 16 };
+
+// Deterministic reduction workspace: per-block partials + completion
+// counter; the last block to finish sums the partials in a fixed order.
+struct Reducer {
+  DevBuf<double> partial;
+  DevBuf<unsigned int> counter;
+  int cap = 0;
+  void init(int maxBlocks, hipStream_t s);
+};
+
+// y = A x.  When dotWith != nullptr also computes dot(dotWith, y) into
+// scal[slot] (deterministically).
+template <typename VT, typename XT>
+void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* scal, int slot,
+          Reducer& red, hipStream_t s);
+// scal[slot] = dot(a, b)
+template <typename T>
+void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hipStream_t s);
+
+// x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den]
+template <typename T>
+void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
+                Reducer& red, hipStream_t s);
+// p = z + beta p, beta = scal[num]/scal[den]  (first: p = z)
+template <typename T>
+void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,
+                   hipStream_t s);
+// r = b - A x ; scal[S_RR] = r.r (FP64 residual, output converted to RT)
+template <typename VT, typename RT>
+void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, double* scal,
+              Reducer& red, hipStream_t s);
+
+void copy_d2f(int n, const double* a, float* b, hipStream_t s);
+void axpy_f2d(int n, double* x, const float* e, hipStream_t s);  // x += e
+void fill_zero(void* p, size_t bytes, hipStream_t s);  // kernel fill (graph-safe)
+
+}  // namespace an

@@ -1,0 +1,497 @@
+// solver.hip -- PCG + aggregation AMG and Jacobi BiCGSTAB on the GPU.
+//
+// PCG (replaces HYPRE_ParCSRPCG + BoomerAMG, LP/src/hypre_backend.cu:54-99,
+// GSV1 main.cu:2224-2273, 2617-2676): two-norm stop
+// ||r|| <= max(relTol ||b||, absTol); b == 0 gives x = 0; x0 from the caller.
+// All scalars (alpha, beta, dots) stay in device memory; the host reads
+// ||r||^2 once per iteration through pinned memory, and the wait is
+// overlapped with the next preconditioner application (which does not touch
+// x), so the host round trip is off the critical path.
+//
+// Mixed precision: FP64 iterative refinement whose inner solve is a PCG in
+// FP32 (FP32 matrix values, vectors, SpMV, smoother and AMG levels; FP64 dot
+// accumulation and FP64 dense coarsest solve); the outer residual b - A x is
+// FP64.
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace an {
+
+void default_pcg_options(an_pcg_options* o) {
+  o->maxIter = 4000;
+  o->relTol = 0.0;
+  o->absTol = 1e-12;
+  o->precision = 0;
+  o->coarseSize = 256;
+  o->maxLevels = 25;
+  o->strength = 0.25;
+  o->pairPasses = 2;
+  o->jacobiOmega = 0.75;
+  o->innerRelTol = 1e-4;
+  o->profile = 0;
+}
+
+static constexpr int kB = 256;
+
+PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
+                     const an_pcg_options& o, bool dev, hipStream_t stream)
+    : opt(o), s(stream) {
+  A.n = n;
+  A.nnz = nnz;
+  A.rowOff.alloc((size_t)n + 1);
+  A.cols.alloc((size_t)nnz);
+  A.vals.alloc((size_t)nnz);
+  hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  AN_HIP(hipMemcpyAsync(A.rowOff.p, rowOff, sizeof(int) * ((size_t)n + 1), k, s));
+  AN_HIP(hipMemcpyAsync(A.cols.p, cols, sizeof(int) * (size_t)nnz, k, s));
+  red.init(grid_for(n, kB) + 1024, s);
+  scal.alloc(S_COUNT);
+  scal.zero(s);
+  r.alloc(n);
+  z.alloc(n);
+  p.alloc(n);
+  q.alloc(n);
+  amg.opt.coarseSize = o.coarseSize;
+  amg.opt.maxLevels = o.maxLevels;
+  amg.opt.strength = o.strength;
+  amg.opt.pairPasses = o.pairPasses;
+  amg.opt.omega = o.jacobiOmega;
+  AN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  AN_HIP(hipStreamSynchronize(s));
+}
+
+PcgSolver::~PcgSolver() {
+  if (ev) (void)hipEventDestroy(ev);
+  for (auto& e : spmvEv) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto& e : vcEv) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+}
+
+PcgSolver::EvPair* PcgSolver::next_ev(std::vector<EvPair>& pool, size_t& used) {
+  if (!opt.profile) return nullptr;
+  if (used == pool.size()) {
+    EvPair e;
+    AN_HIP(hipEventCreate(&e.a));
+    AN_HIP(hipEventCreate(&e.b));
+    pool.push_back(e);
+  }
+  return &pool[used++];
+}
+
+void PcgSolver::collect_profile(an_solve_info* inf) {
+  if (!opt.profile) return;
+  AN_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (size_t i = 0; i < spmvUsed; ++i) {
+    float ms = 0.f;
+    AN_HIP(hipEventElapsedTime(&ms, spmvEv[i].a, spmvEv[i].b));
+    t += ms * 1e-3;
+  }
+  inf->spmvCalls = (int)spmvUsed;
+  inf->spmvAvgSeconds = spmvUsed ? t / spmvUsed : 0.0;
+  t = 0.0;
+  for (size_t i = 0; i < vcUsed; ++i) {
+    float ms = 0.f;
+    AN_HIP(hipEventElapsedTime(&ms, vcEv[i].a, vcEv[i].b));
+    t += ms * 1e-3;
+  }
+  inf->vcycleCalls = (int)vcUsed;
+  inf->vcycleAvgSeconds = vcUsed ? t / vcUsed : 0.0;
+}
+
+void PcgSolver::set_values_device(const double* vals, bool rebuild) {
+  AN_HIP(hipMemcpyAsync(A.vals.p, vals, sizeof(double) * (size_t)A.nnz, hipMemcpyDeviceToDevice, s));
+  valuesSet = true;
+  if (rebuild || !amg.built) {
+    auto t0 = std::chrono::steady_clock::now();
+    amg.setup(A, s);
+    if (opt.precision == 1) {
+      amg.make_fp32(s);
+      r32.alloc(A.n);
+      z32.alloc(A.n);
+      p32.alloc(A.n);
+      q32.alloc(A.n);
+      e32.alloc(A.n);
+    }
+    AN_HIP(hipStreamSynchronize(s));
+    lastSetupSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } else {
+    amg.refresh_fine(s);
+  }
+}
+
+double PcgSolver::read_scalar(int slot) {
+  AN_HIP(hipMemcpyAsync(hs.h + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  return hs.h[slot];
+}
+
+// FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.
+int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut) {
+  (void)b;
+  const int n = A.n;
+  double rn = std::sqrt(hs.h[S_RR]);
+  *rnormOut = rn;
+  if (rn <= tol) return 0;
+  amg.apply(r.p, z.p, s);
+  dot<double>(n, r.p, z.p, scal.p, S_RHO0, red, s);
+  pcg_direction<double>(n, p.p, z.p, scal.p, 0, 0, true, s);
+  int it = 1;
+  for (; it <= maxIter; ++it) {
+    const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+    const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
+    EvPair* e1 = next_ev(spmvEv, spmvUsed);
+    if (e1) AN_HIP(hipEventRecord(e1->a, s));
+    spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s);
+    if (e1) AN_HIP(hipEventRecord(e1->b, s));
+    pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s);
+    AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    AN_HIP(hipEventRecord(ev, s));
+    // next direction while the host waits for ||r||
+    EvPair* e2 = next_ev(vcEv, vcUsed);
+    if (e2) AN_HIP(hipEventRecord(e2->a, s));
+    amg.apply(r.p, z.p, s);
+    if (e2) AN_HIP(hipEventRecord(e2->b, s));
+    dot<double>(n, r.p, z.p, scal.p, rhoNew, red, s);
+    pcg_direction<double>(n, p.p, z.p, scal.p, rhoNew, rhoOld, false, s);
+    AN_HIP(hipEventSynchronize(ev));
+    rn = std::sqrt(hs.h[S_RR]);
+    *rnormOut = rn;
+    if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) return -it;
+    if (rn <= tol) break;
+  }
+  AN_HIP(hipStreamSynchronize(s));
+  return it > maxIter ? -maxIter : it;
+}
+
+// FP32 inner PCG from x = 0 on A32 with FP32 AMG.
+int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double* rnormOut) {
+  const int n = A.n;
+  const DevCSR<float>& A32 = amg.lv[0].A32;
+  fill_zero(x, sizeof(float) * (size_t)n, s);
+  AN_HIP(hipMemcpyAsync(r32.p, b, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  amg.apply32(r32.p, z32.p, s);
+  dot<float>(n, r32.p, z32.p, scal.p, S_RHO0, red, s);
+  pcg_direction<float>(n, p32.p, z32.p, scal.p, 0, 0, true, s);
+  int it = 1;
+  double rn = 0.0;
+  for (; it <= maxIter; ++it) {
+    const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+    const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
+    EvPair* e1 = next_ev(spmvEv, spmvUsed);
+    if (e1) AN_HIP(hipEventRecord(e1->a, s));
+    spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s);
+    if (e1) AN_HIP(hipEventRecord(e1->b, s));
+    pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s);
+    AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    AN_HIP(hipEventRecord(ev, s));
+    EvPair* e2 = next_ev(vcEv, vcUsed);
+    if (e2) AN_HIP(hipEventRecord(e2->a, s));
+    amg.apply32(r32.p, z32.p, s);
+    if (e2) AN_HIP(hipEventRecord(e2->b, s));
+    dot<float>(n, r32.p, z32.p, scal.p, rhoNew, red, s);
+    pcg_direction<float>(n, p32.p, z32.p, scal.p, rhoNew, rhoOld, false, s);
+    AN_HIP(hipEventSynchronize(ev));
+    rn = std::sqrt(hs.h[S_RR]);
+    *rnormOut = rn;
+    if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) break;
+    if (rn <= tol) break;
+  }
+  AN_HIP(hipStreamSynchronize(s));
+  return it > maxIter ? maxIter : it;
+}
+
+an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
+  if (!valuesSet) throw Error(1, "solver has no matrix values");
+  const int n = A.n;
+  spmvUsed = vcUsed = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  dot<double>(n, b, b, scal.p, S_BB, red, s);
+  const double bnorm = std::sqrt(read_scalar(S_BB));
+  an_solve_info inf{};
+  inf.levels = amg.levels();
+  inf.setupSeconds = lastSetupSeconds;
+  an_status st = AN_OK;
+  if (bnorm == 0.0) {
+    fill_zero(x, sizeof(double) * (size_t)n, s);
+    AN_HIP(hipStreamSynchronize(s));
+    inf.converged = 1;
+  } else {
+    const double tol = std::fmax(opt.relTol * bnorm, opt.absTol);
+    residual<double, double>(A, b, x, r.p, scal.p, red, s);
+    read_scalar(S_RR);
+    double rn = std::sqrt(hs.h[S_RR]);
+    if (opt.precision != 1) {
+      int it = pcg64(b, x, tol, opt.maxIter, &rn);
+      inf.iterations = it < 0 ? -it : it;
+      inf.converged = (it >= 0 && rn <= tol) ? 1 : 0;
+    } else {
+      int total = 0, steps = 0;
+      while (rn > tol && total < opt.maxIter) {
+        copy_d2f(n, r.p, r32.p, s);
+        // inner solve to innerRelTol relative reduction, not below the
+        // outer absolute target
+        double innerTol = std::fmax(opt.innerRelTol * rn, 0.5 * tol);
+        float* rin = r32.p;
+        double rin_n = 0.0;
+        // the FP32 PCG overwrites r32; keep the right-hand side in q32
+        AN_HIP(hipMemcpyAsync(q32.p, rin, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        int it = pcg32(q32.p, e32.p, innerTol, opt.maxIter - total, &rin_n);
+        total += it;
+        axpy_f2d(n, x, e32.p, s);
+        residual<double, double>(A, b, x, r.p, scal.p, red, s);
+        read_scalar(S_RR);
+        double rnNew = std::sqrt(hs.h[S_RR]);
+        ++steps;
+        if (!std::isfinite(rnNew) || (steps > 3 && rnNew > 0.9 * rn)) {
+          rn = rnNew;
+          break;
+        }
+        rn = rnNew;
+      }
+      inf.iterations = total;
+      inf.refinementSteps = steps;
+      inf.converged = rn <= tol ? 1 : 0;
+    }
+    inf.finalResNorm = rn;
+    inf.finalRelResNorm = rn / bnorm;
+    if (!inf.converged) st = AN_ENOCONV;
+  }
+  inf.solveSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  collect_profile(&inf);
+  if (info) *info = inf;
+  return st;
+}
+
+// ---------------------------------------------------------- BiCGSTAB ---
+void BicgWork::ensure(int nn, hipStream_t st) {
+  if (n == nn && scal.p) return;
+  n = nn;
+  const int nbSpmv = grid_for(n, kB);
+  red.init(nbSpmv > 2048 ? nbSpmv + 16 : 2048, st);
+  scal.alloc(S_COUNT);
+  dinv.alloc(n);
+  r.alloc(n);
+  r0.alloc(n);
+  p.alloc(n);
+  v.alloc(n);
+  ph.alloc(n);
+  s_.alloc(n);
+  sh.alloc(n);
+  t.alloc(n);
+}
+
+__device__ double bsum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  return t;
+}
+
+// two reductions sharing one completion ticket
+__device__ void finish2(double a, double b, int nb, double* __restrict__ partial,
+                        unsigned int* __restrict__ counter, double* __restrict__ outA,
+                        double* __restrict__ outB, double* sh) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = a;
+    partial[2 * blockIdx.x + 1] = b;
+    __threadfence();
+    last = (atomicAdd(counter, 1u) == (unsigned)nb - 1u);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* vp = partial;
+  double sa = 0.0, sb = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    sa += vp[2 * i];
+    sb += vp[2 * i + 1];
+  }
+  sa = bsum(sa, sh);
+  sb = bsum(sb, sh);
+  if (threadIdx.x == 0) {
+    if (outA) *outA = sa;
+    if (outB) *outB = sb;
+    *counter = 0u;
+  }
+}
+
+__global__ void k_dinv(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                       const double* __restrict__ va, double* __restrict__ dinv) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k)
+    if (ci[k] == i) d = va[k];
+  dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
+}
+
+// p = r + beta (p - omega v); ph = dinv p
+__global__ __launch_bounds__(kB) void k_bicg_p(int n, const double* __restrict__ r,
+                                               double* __restrict__ p, const double* __restrict__ v,
+                                               const double* __restrict__ dinv,
+                                               double* __restrict__ ph,
+                                               const double* __restrict__ sc, int rhoNew,
+                                               int rhoOld, int first) {
+  const double beta = first ? 0.0 : (sc[rhoNew] / sc[rhoOld]) * (sc[S_AL] / sc[S_OM]);
+  const double om = first ? 0.0 : sc[S_OM];
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    double pi = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
+    p[i] = pi;
+    ph[i] = dinv[i] * pi;
+  }
+}
+
+// alpha = rhoNew / r0v; s = r - alpha v; sh = dinv s; ss = s.s
+__global__ __launch_bounds__(kB) void k_bicg_s(int n, const double* __restrict__ r,
+                                               const double* __restrict__ v,
+                                               const double* __restrict__ dinv,
+                                               double* __restrict__ s_, double* __restrict__ sh,
+                                               double* __restrict__ sc, int rhoNew,
+                                               double* __restrict__ partial,
+                                               unsigned int* __restrict__ counter) {
+  __shared__ double shm[kB / 64];
+  const double alpha = sc[rhoNew] / sc[S_R0V];
+  double acc = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    double si = r[i] - alpha * v[i];
+    s_[i] = si;
+    sh[i] = dinv[i] * si;
+    acc += si * si;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_AL] = alpha;  // read by later kernels only
+  double b = bsum(acc, shm);
+  finish2(b, 0.0, gridDim.x, partial, counter, sc + S_SS, nullptr, shm);
+}
+
+__global__ __launch_bounds__(kB) void k_dot2(int n, const double* __restrict__ t,
+                                             const double* __restrict__ s_,
+                                             double* __restrict__ sc,
+                                             double* __restrict__ partial,
+                                             unsigned int* __restrict__ counter) {
+  __shared__ double shm[kB / 64];
+  double a = 0.0, b = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    a += t[i] * s_[i];
+    b += t[i] * t[i];
+  }
+  double ba = bsum(a, shm);
+  double bb = bsum(b, shm);
+  finish2(ba, bb, gridDim.x, partial, counter, sc + S_TS, sc + S_TT, shm);
+}
+
+// omega = ts/tt; if ||s|| <= tol: x += alpha ph, r = s; else x += alpha ph +
+// omega sh, r = s - omega t.  rr = r.r
+__global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
+                                               double* __restrict__ r,
+                                               const double* __restrict__ ph,
+                                               const double* __restrict__ sh,
+                                               const double* __restrict__ s_,
+                                               const double* __restrict__ t,
+                                               double* __restrict__ sc, double tol,
+                                               double* __restrict__ partial,
+                                               unsigned int* __restrict__ counter) {
+  __shared__ double shm[kB / 64];
+  const double alpha = sc[S_AL];
+  const double tt = sc[S_TT];
+  const double omega = (tt > 0.0) ? sc[S_TS] / tt : 0.0;
+  const bool convS = sqrt(sc[S_SS]) <= tol;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    double ri;
+    if (convS) {
+      x[i] += alpha * ph[i];
+      ri = s_[i];
+    } else {
+      x[i] += alpha * ph[i] + omega * sh[i];
+      ri = s_[i] - omega * t[i];
+    }
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_OM] = omega;
+  double b = bsum(acc, shm);
+  finish2(b, 0.0, gridDim.x, partial, counter, sc + S_RR, nullptr, shm);
+}
+
+an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,
+                          double relTol, double absTol, BicgWork& w, hipStream_t s,
+                          an_solve_info* info) {
+  const int n = A.n;
+  auto t0 = std::chrono::steady_clock::now();
+  w.ensure(n, s);
+  const int G = std::min(grid_for(n, kB), 1024);
+  an_solve_info inf{};
+  k_dinv<<<grid_for(n, kB), kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
+  AN_CHECK_LAUNCH();
+  dot<double>(n, b, b, w.scal.p, S_BB, w.red, s);
+  residual<double, double>(A, b, x, w.r.p, w.scal.p, w.red, s);
+  AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(w.hs.h[S_BB]);
+  double rn = std::sqrt(w.hs.h[S_RR]);
+  an_status st = AN_OK;
+  if (bnorm == 0.0) {
+    fill_zero(x, sizeof(double) * (size_t)n, s);
+    AN_HIP(hipStreamSynchronize(s));
+    inf.converged = 1;
+    if (info) *info = inf;
+    return AN_OK;
+  }
+  const double tol = std::fmax(relTol * bnorm, absTol);
+  int it = 0;
+  bool conv = rn <= tol;
+  if (!conv) {
+    AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    for (it = 1; it <= maxIter; ++it) {
+      const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
+      const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+      dot<double>(n, w.r0.p, w.r.p, w.scal.p, rhoNew, w.red, s);
+      k_bicg_p<<<G, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, w.scal.p, rhoNew, rhoOld,
+                                it == 1 ? 1 : 0);
+      AN_CHECK_LAUNCH();
+      spmv<double, double>(A, w.ph.p, w.v.p, w.r0.p, w.scal.p, S_R0V, w.red, s);
+      k_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, w.scal.p, rhoNew,
+                                w.red.partial.p, w.red.counter.p);
+      AN_CHECK_LAUNCH();
+      spmv<double, double>(A, w.sh.p, w.t.p, nullptr, w.scal.p, 0, w.red, s);
+      k_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, w.scal.p, w.red.partial.p, w.red.counter.p);
+      k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, w.scal.p, tol,
+                                w.red.partial.p, w.red.counter.p);
+      AN_CHECK_LAUNCH();
+      AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+      AN_HIP(hipStreamSynchronize(s));
+      const double* h = w.hs.h;
+      rn = std::sqrt(h[S_RR]);
+      if (std::sqrt(h[S_SS]) <= tol || rn <= tol) {
+        conv = true;
+        break;
+      }
+      if (h[rhoNew] == 0.0 || h[S_R0V] == 0.0 || h[S_OM] == 0.0 || !std::isfinite(rn)) break;
+    }
+    if (it > maxIter) it = maxIter;
+  }
+  inf.iterations = it;
+  inf.converged = conv ? 1 : 0;
+  inf.finalResNorm = rn;
+  inf.finalRelResNorm = rn / bnorm;
+  inf.solveSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!conv) st = AN_ENOCONV;
+  if (info) *info = inf;
+  return st;
+}
+
+}  // namespace an

@@ -1,0 +1,71 @@
+// solver.hpp -- PCG + AMG (FP64 and FP32-inner/FP64-refined) and BiCGSTAB.
+#pragma once
+
+#include "../../include/anabasis_hip.h"
+#include "amg.hpp"
+
+namespace an {
+
+struct PinnedScalars {
+  double* h = nullptr;
+  PinnedScalars() { AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * S_COUNT)); }
+  ~PinnedScalars() {
+    if (h) (void)hipHostFree(h);
+  }
+  PinnedScalars(const PinnedScalars&) = delete;
+  PinnedScalars& operator=(const PinnedScalars&) = delete;
+};
+
+class PcgSolver {
+ public:
+  an_pcg_options opt;
+  DevCSR<double> A;
+  AmgHierarchy amg;
+  Reducer red;
+  DevBuf<double> scal;
+  PinnedScalars hs;
+  DevBuf<double> r, z, p, q;
+  DevBuf<float> r32, z32, p32, q32, e32;
+  hipStream_t s = 0;
+  hipEvent_t ev = nullptr;
+  bool valuesSet = false;
+  double lastSetupSeconds = 0.0;
+  // profiling (opt.profile): event pairs around SpMV / V-cycle launches
+  struct EvPair {
+    hipEvent_t a, b;
+  };
+  std::vector<EvPair> spmvEv, vcEv;
+  size_t spmvUsed = 0, vcUsed = 0;
+
+  PcgSolver(int n, const int* rowOff, const int* cols, long long nnz, const an_pcg_options& o,
+            bool deviceInput, hipStream_t stream);
+  ~PcgSolver();
+  // values: device pointer; rebuild the AMG when asked or when none exists.
+  void set_values_device(const double* vals, bool rebuild);
+  // b, x device pointers; x holds x0.
+  an_status solve(const double* b, double* x, an_solve_info* info);
+  void apply_prec(const double* rr, double* zz) { amg.apply(rr, zz, s); }
+
+ private:
+  int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
+  int pcg32(const float* b, float* x, double tol, int maxIter, double* rnormOut);
+  double read_scalar(int slot);
+  EvPair* next_ev(std::vector<EvPair>& pool, size_t& used);
+  void collect_profile(an_solve_info* inf);
+};
+
+// One-shot Jacobi-preconditioned BiCGSTAB on device data (x holds x0).
+struct BicgWork {
+  Reducer red;
+  DevBuf<double> scal, dinv, r, r0, p, v, ph, s_, sh, t;
+  PinnedScalars hs;
+  int n = 0;
+  void ensure(int n, hipStream_t s);
+};
+an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,
+                          double relTol, double absTol, BicgWork& w, hipStream_t s,
+                          an_solve_info* info);
+
+void default_pcg_options(an_pcg_options* o);
+
+}  // namespace an

@@ -1,0 +1,306 @@
+/*
+ * anabasis_hip.h -- C ABI of the MI355X-native Anabasis SIMPLE hot path.
+ *
+ * Plain C: opaque handles, int32 indices, double/float values, explicit
+ * lengths.  Every entry point returns an an_status (AN_OK == 0); the message
+ * of the last failure on the calling thread is an_last_error().  No C++
+ * exception crosses this boundary.  Non-convergence of an iterative solve is
+ * reported (AN_ENOCONV) and the approximate solution is still returned, as
+ * the reference does for HYPRE error 256 (GSV1 main.cu:2656-2665).
+ *
+ * Array conventions (the reference's own, flattened):
+ *   - 3-vectors are interleaved [3*i+0..2] (std::array<double,3> layout of
+ *     LP Mesh::cc etc.) unless the name says SoA;
+ *   - "BF" arrays are per boundary face, index bf = f - nInternalFaces
+ *     (LP BoundaryFaceData holds the same entries at index f);
+ *   - patch indices are 0-based; bPatch[f] is patch+1 (0 = internal face),
+ *     exactly LP Mesh::bPatch (LP/src/mesh.cu:179-186).
+ * Pointers marked "host" are host memory; functions with an an_memspace
+ * argument accept either host or device pointers for their bulk arrays.
+ * Reference files are cited relative to the Anabasis source tree.
+ */
+#ifndef ANABASIS_HIP_H
+#define ANABASIS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AN_OK = 0,
+  AN_EINVAL = 1,    /* bad argument / mesh / size mismatch */
+  AN_ENOCONV = 2,   /* iterative solve did not reach its tolerance */
+  AN_EHIP = 3,      /* HIP runtime error */
+  AN_EIO = 4,       /* file could not be read or written */
+  AN_ENOMEM = 5,
+  AN_EDIVERGED = 6, /* SIMPLE mass residual non-finite or > 1e50 */
+  AN_EINTERNAL = 7
+} an_status;
+
+typedef enum { AN_HOST = 0, AN_DEVICE = 1 } an_memspace;
+
+/* boundary condition kinds (LP ScalarBCType, LS ScalarBCType, GSV1
+ * bc_to_int main.cu:1058) */
+#define AN_BC_NEUMANN 0
+#define AN_BC_DIRICHLET 1
+#define AN_BC_ZERO_GRADIENT 2
+
+const char *an_last_error(void);
+int an_version(void);
+/* Selects the HIP device used by later calls on this thread. */
+an_status an_set_device(int device);
+
+/* ------------------------------------------------------------------ mesh */
+typedef struct an_mesh_s *an_mesh_t;
+
+typedef struct {
+  int nPoints, nFaces, nInternalFaces, nCells, nPatches;
+  int nBoundaryFaces;
+  int64_t nnz;          /* CSR nonzeros of the cell-to-cell pattern */
+  double maxNonOrthDeg; /* LP Mesh::maxNonOrthDeg */
+  int hasGeometry;
+} an_mesh_info;
+
+/* read_openfoam_polymesh (LP/src/mesh.cu:160-288; GSV1 main.cu:582-599). */
+an_status an_mesh_read_polymesh(const char *polyMeshDir, an_mesh_t *out);
+/* Mesh from arrays (host): faces as offsets[nFaces+1] + vertex ids.
+ * nPoints == 0 / points == NULL gives a topology-only mesh (CSR addressing
+ * and patch lists only; no geometry). */
+an_status an_mesh_from_arrays(int nPoints, const double *points, int nFaces,
+                              const int *faceOffsets, const int *faceVerts,
+                              const int *owner, int nInternalFaces,
+                              const int *neigh, int nPatches,
+                              const char *const *patchNames,
+                              const int *patchStart, const int *patchNFaces,
+                              an_mesh_t *out);
+an_status an_mesh_write_polymesh(an_mesh_t mesh, const char *dir);
+/* Generators (no reference counterpart: the reference ships no generator).
+ * hex box: nx*ny*nz hexes on [0,lx]x[0,ly]x[0,lz], interior vertices
+ * jittered uniformly by +-jitter*h (boundary vertices stay on their faces),
+ * mt19937_64(seed); patches xmin,xmax,ymin,ymax,zmin,zmax. */
+an_status an_mesh_gen_hexbox(int nx, int ny, int nz, double lx, double ly,
+                             double lz, double jitter, uint64_t seed,
+                             an_mesh_t *out);
+/* unit-cube tet mesh: n^3 blocks, 6 Kuhn tets each; same patch names. */
+an_status an_mesh_gen_tetcube(int nblocks, double length, an_mesh_t *out);
+/* 2.2 x 0.41 x 0.41 channel with a D=0.1 cylinder (axis z, centre
+ * (0.5,0.2)), O-grid + H blocks, about targetCells hexes; patches in the
+ * order of examples/cylinder: patch_1_0 inlet, patch_0_0 walls y=0/H,
+ * patch_2_0 outlet, patch_5_0 walls z=0/W, patch_3_0 cylinder. */
+an_status an_mesh_gen_cylinder(long long targetCells, an_mesh_t *out);
+an_status an_mesh_destroy(an_mesh_t mesh);
+an_status an_mesh_get_info(an_mesh_t mesh, an_mesh_info *info);
+/* Copies a host mesh array.  name: "points","faceOffsets","faceVerts",
+ * "owner","neigh","bPatch","cc","vol","xf","nf","Sf","Af".  out must hold
+ * the documented number of elements (int or double). */
+an_status an_mesh_get_array(an_mesh_t mesh, const char *name, void *out);
+/* Patch k: name (nul-terminated, truncated to cap), startFace, nFaces. */
+an_status an_mesh_get_patch(an_mesh_t mesh, int k, char *name, int cap,
+                            int *startFace, int *nFaces);
+
+/* ------------------------------------------------------------ addressing */
+/* build_scalar_pattern (LP/src/scalar_elliptic.cu:16-58, == GSV1
+ * build_momentum_pattern main.cu:1315-1340), built on the GPU.  Outputs
+ * rowOffsets[nCells+1], cols[nnz], diagPos[nCells], facePP/PN/NP/NN[nIF]
+ * (any may be NULL). */
+an_status an_mesh_csr(an_mesh_t mesh, int *rowOffsets, int *cols,
+                      int *diagPos, int *facePP, int *facePN, int *faceNP,
+                      int *faceNN, an_memspace mem);
+/* Same pattern from bare owner/neighbour lists (e.g. a polyMesh without
+ * points/faces).  nnzOut receives nnz; when the output pointers are NULL
+ * only nnz is computed. */
+an_status an_csr_from_faces(int nCells, int nFaces, int nInternalFaces,
+                            const int *owner, const int *neigh,
+                            int64_t *nnzOut, int *rowOffsets, int *cols,
+                            int *diagPos, int *facePP, int *facePN,
+                            int *faceNP, int *faceNN, an_memspace mem);
+
+/* ------------------------------------------------------------- gradients */
+#define AN_GRAD_GSV1 0 /* kernel_apply_lsq_gradient, GSV1 main.cu:1403-1440 */
+#define AN_GRAD_LP 1   /* compute_lsq_gradient, LP/src/gradient.cu:21-83 */
+/* Least-squares cell gradient of phi[nCells] -> grad[3*nCells] interleaved.
+ * bcTypeBF/bcValBF: per boundary face. */
+an_status an_lsq_gradient(an_mesh_t mesh, int mode, const int *bcTypeBF,
+                          const double *bcValBF, const double *phi,
+                          double *grad, an_memspace mem);
+
+/* -------------------------------------------------------------- assembly */
+/* assemble_scalar_elliptic_system (LP/src/scalar_elliptic.cu:60-136):
+ * gammaFace[nFaces], cellSource[nCells], grad[3*nCells] -> values[nnz]
+ * (CSR order of an_mesh_csr), rhs[nCells]. */
+an_status an_elliptic_assemble(an_mesh_t mesh, const double *gammaFace,
+                               const double *cellSource, const int *bcTypeBF,
+                               const double *bcValBF, const double *grad,
+                               int includeNonOrth, int useReferenceCell,
+                               int refCell, double refValue, double *values,
+                               double *rhs, an_memspace mem);
+/* assemble_scalar_transport_system (LS/src/scalar_transport_library.cu:
+ * 250-357); upwind != 0 selects ConvectionScheme::Upwind. */
+an_status an_transport_assemble(an_mesh_t mesh, const double *faceFlux,
+                                const double *gammaFace, const double *Su,
+                                const double *Sp, const int *bcTypeBF,
+                                const double *bcValBF, const double *grad,
+                                int includeNonOrth, int upwind,
+                                double *values, double *rhs, an_memspace mem);
+
+/* ---------------------------------------------------------- linear solve */
+typedef struct {
+  int maxIter;         /* HypreOptions::maxIter */
+  double relTol;       /* stop when ||r|| <= max(relTol*||b||, absTol) */
+  double absTol;
+  int precision;       /* 0 FP64; 1 FP32 inner PCG+AMG in FP64 refinement */
+  int coarseSize;      /* AMG: coarsest level size (dense solve) */
+  int maxLevels;
+  double strength;     /* AMG: pairwise matching strength threshold */
+  int pairPasses;      /* AMG: matchings per level */
+  double jacobiOmega;  /* AMG: damped-Jacobi smoother weight */
+  double innerRelTol;  /* mixed precision: inner FP32 solve reduction */
+  int profile;         /* 1: time every SpMV / V-cycle with HIP events */
+} an_pcg_options;
+void an_pcg_default_options(an_pcg_options *o);
+
+typedef struct {
+  int iterations;          /* HypreSolveInfo::iterations */
+  double finalRelResNorm;  /* HypreSolveInfo::finalRelResNorm */
+  double finalResNorm;
+  int converged;
+  int levels;              /* AMG levels */
+  double setupSeconds, solveSeconds;
+  int refinementSteps;     /* mixed precision outer steps */
+  /* with an_pcg_options.profile: in-loop kernel timings (HIP events on the
+   * solver stream) of the Krylov SpMV and of one preconditioner apply */
+  int spmvCalls;
+  double spmvAvgSeconds;
+  int vcycleCalls;
+  double vcycleAvgSeconds;
+} an_solve_info;
+
+typedef struct an_solver_s *an_solver_t;
+/* PCG + aggregation-AMG on a CSR pattern (the replacement for HYPRE PCG +
+ * BoomerAMG, LP/src/hypre_backend.cu:3-114 and the reusable API
+ * LP/include/hypre_backend.h:64-82).  The pattern is copied. */
+an_status an_pcg_create(int n, const int *rowOffsets, const int *cols,
+                        int64_t nnz, const an_pcg_options *opt,
+                        an_memspace mem, an_solver_t *out);
+/* New matrix values; rebuildHierarchy != 0 also rebuilds the AMG hierarchy
+ * (update_reusable_hypre_system_gpu doSetup). */
+an_status an_pcg_set_values(an_solver_t s, const double *values,
+                            int rebuildHierarchy, an_memspace mem);
+/* Solves A x = b; x holds the initial guess on entry. */
+an_status an_pcg_solve(an_solver_t s, const double *b, double *x,
+                       an_memspace mem, an_solve_info *info);
+/* y = A x with the solver's current values (test hook for the SpMV). */
+an_status an_spmv(an_solver_t s, const double *x, double *y, an_memspace mem);
+/* One preconditioner application z = M^-1 r (test hook for the V-cycle). */
+an_status an_amg_apply(an_solver_t s, const double *r, double *z,
+                       an_memspace mem);
+/* Level-0 aggregate ids (host int[n]); returns the coarse size. */
+an_status an_amg_aggregates(an_solver_t s, int *agg, int *nCoarse);
+an_status an_solver_destroy(an_solver_t s);
+
+/* BiCGSTAB + Jacobi (HYPRE BiCGSTAB + DiagScale, GSV1 main.cu:2553-2576,
+ * LS/src/scalar_transport_library.cu:359-471), one-shot. */
+an_status an_bicgstab_solve(int n, const int *rowOffsets, const int *cols,
+                            int64_t nnz, const double *values,
+                            const double *b, double *x, int maxIter,
+                            double relTol, double absTol, an_memspace mem,
+                            an_solve_info *info);
+
+/* ------------------------------------------------ libpoisson / libscalar */
+typedef struct {
+  int nNonOrthCorr;      /* EllipticOptions::nNonOrthCorr */
+  int nonOrth;           /* laplacianScheme: 1 nonorth, 0 orth */
+  int useReferenceCell;
+  int referenceCell;
+  double referenceValue;
+  an_pcg_options solver; /* EllipticOptions::hypre */
+} an_elliptic_options;
+void an_elliptic_default_options(an_elliptic_options *o);
+/* solve_scalar_elliptic (LP/src/poisson_library.cu:3-47): non-orthogonal
+ * outer loop, each pass from x0 = 0 (hypre_backend.cu:35,49).  Inputs host
+ * or device per mem; phi[nCells] out. */
+an_status an_solve_scalar_elliptic(an_mesh_t mesh, const double *gammaFace,
+                                   const double *cellSource,
+                                   const int *bcTypeBF, const double *bcValBF,
+                                   const an_elliptic_options *opt,
+                                   double *phi, an_memspace mem,
+                                   an_solve_info *info);
+
+typedef struct {
+  int upwind;          /* ConvectionScheme */
+  int nonOrth;         /* DiffusionScheme */
+  int nNonOrthCorr;
+  int maxIter;
+  double relTol, absTol;
+} an_transport_options;
+void an_transport_default_options(an_transport_options *o);
+/* libscalar::solve_steady_scalar_transport
+ * (LS/src/scalar_transport_library.cu:475-512); x0 may be NULL. */
+an_status an_solve_steady_scalar_transport(
+    an_mesh_t mesh, const double *faceFlux, const double *gammaFace,
+    const double *Su, const double *Sp, const int *bcTypeBF,
+    const double *bcValBF, const an_transport_options *opt, const double *x0,
+    double *phi, an_memspace mem, an_solve_info *info);
+
+/* ------------------------------------------------------------------ SIMPLE */
+typedef struct {
+  double rho, mu;
+  int nVelNonOrthCorr, nNonOrthCorr;
+  double uRelax, pRelax;
+  double corrTol, tolMass, tolVel;
+  int velMaxit;
+  double velTol, velRelTol;
+  int pMaxit;
+  double pTol, pRelTol;
+  int pAmgRebuildEvery;
+  int precision;        /* 0 FP64 pressure solve, 1 mixed */
+  int fuseMomentumRhs;  /* 1: u/v/w share one 3-RHS BiCGSTAB per pass */
+  an_pcg_options pressure;
+} an_simple_params;
+void an_simple_default_params(an_simple_params *p);
+
+typedef struct {
+  int step;
+  double massRes, duRel, dvRel, dwRel, dpRel;
+  int bicgstabIts[3];
+  int pcgLast, pcgTotal;
+  double seconds;
+} an_simple_stats;
+
+typedef struct an_simple_s *an_simple_t;
+/* GSV1 driver state (main.cu:3294-3347).  Per-PATCH velocity/pressure BC
+ * types (AN_BC_*), per-FACE BC values [nFaces] (the reference's DeviceBC).
+ * u,v,w start at the arithmetic mean of the inlet faces' values when
+ * inletPatch >= 0 (main.cu:3302-3322), else 0; p = 0. */
+an_status an_simple_create(an_mesh_t mesh, const an_simple_params *par,
+                           const int *patchUType, const int *patchPType,
+                           const double *uFaceBC, const double *vFaceBC,
+                           const double *wFaceBC, const double *pFaceBC,
+                           int inletPatch, an_simple_t *out);
+/* Runs n outer iterations of main.cu:3388-3634; last-iteration stats.
+ * Returns AN_EDIVERGED on divergence, AN_OK otherwise (converged flag in
+ * *converged when not NULL). */
+an_status an_simple_iterate(an_simple_t s, int n, an_simple_stats *last,
+                            int *converged);
+/* name: "u","v","w","p","pCorr","rAU","phi"(nFaces),"div" */
+an_status an_simple_get_field(an_simple_t s, const char *name, double *out,
+                              an_memspace mem);
+an_status an_simple_set_field(an_simple_t s, const char *name,
+                              const double *in, an_memspace mem);
+/* compute_patch_forces_wall_shear (GSV1 main.cu:2828-2933) on the current
+ * fields, reduced on the GPU in FP64.  out[15]: Fp[3], Fv[3], F[3], area,
+ * nFaces, minWallDistance, maxWallDistance, maxUt, maxShearMag. */
+an_status an_simple_patch_forces(an_simple_t s, int patch, int normalSign,
+                                 double *out);
+/* Accumulated seconds per phase (hipEvent timed): out[0] momentum assembly,
+ * [1] momentum solve, [2] pressure assembly+setup, [3] pressure solve,
+ * [4] other kernels, [5] total. */
+an_status an_simple_timings(an_simple_t s, double *out);
+an_status an_simple_destroy(an_simple_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

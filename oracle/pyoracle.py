@@ -1,0 +1,205 @@
+"""ctypes wrapper of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker (or the timed CPU baseline).  The
+product (anabasis_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "liboracle.so"
+BC_NEUMANN, BC_DIRICHLET, BC_ZEROGRAD = 0, 1, 2
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.or_pattern_nnz.restype = C.c_int64
+        _lib.or_pcg_amg.restype = C.c_int
+        _lib.or_bicgstab_jacobi.restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class AmgOptions(C.Structure):
+    _fields_ = [("maxIter", C.c_int), ("relTol", C.c_double), ("absTol", C.c_double),
+                ("coarseSize", C.c_int), ("maxLevels", C.c_int), ("strength", C.c_double),
+                ("pairPasses", C.c_int), ("jacobiOmega", C.c_double), ("sweeps", C.c_int)]
+
+
+def amg_options(maxIter=4000, relTol=0.0, absTol=1e-12, coarseSize=256, maxLevels=25,
+                strength=0.25, pairPasses=2, jacobiOmega=0.75, sweeps=1):
+    return AmgOptions(maxIter, relTol, absTol, coarseSize, maxLevels, strength, pairPasses,
+                      jacobiOmega, sweeps)
+
+
+class OMesh:
+    """Oracle view of a mesh: plain numpy arrays (interleaved 3-vectors)."""
+
+    def __init__(self, points, face_offsets, face_verts, owner, neigh, patches):
+        self.points = _d(points).reshape(-1, 3)
+        self.faceOff = _i(face_offsets)
+        self.faceVerts = _i(face_verts)
+        self.owner = _i(owner)
+        self.neigh = _i(neigh)
+        self.patches = list(patches)  # (name, start, n)
+        self.nFaces = self.owner.size
+        self.nInternalFaces = self.neigh.size
+        self.nCells = int(max(self.owner.max(initial=-1), self.neigh.max(initial=-1))) + 1
+        self.bPatch = np.zeros(self.nFaces, np.int32)
+        for k, (_, s, n) in enumerate(self.patches):
+            self.bPatch[s:s + n] = k + 1
+        self.geometry()
+        self.pattern()
+
+    @staticmethod
+    def from_product(mesh):
+        """Copies the host arrays of an anabasis_amd Mesh."""
+        pts = mesh.array("points").copy()
+        off = mesh.array("faceOffsets").copy()
+        fv = mesh.array("faceVerts").copy()
+        patches = list(zip(mesh.patchNames, mesh.patchStartFace, mesh.patchNFaces))
+        return OMesh(pts, off, fv, mesh.owner.copy(), mesh.neigh.copy(), patches)
+
+    def geometry(self):
+        nC, nF = self.nCells, self.nFaces
+        self.cc = np.empty((nC, 3))
+        self.vol = np.empty(nC)
+        self.xf = np.empty((nF, 3))
+        self.nf = np.empty((nF, 3))
+        self.Sf = np.empty((nF, 3))
+        self.Af = np.empty(nF)
+        mx = C.c_double()
+        rc = lib().or_mesh_geometry(self.points.shape[0], _p(self.points), nF, _p(self.faceOff),
+                                    _p(self.faceVerts), _p(self.owner), self.nInternalFaces,
+                                    _p(self.neigh), nC, _p(self.cc), _p(self.vol), _p(self.xf),
+                                    _p(self.nf), _p(self.Sf), _p(self.Af), C.byref(mx))
+        if rc != 0:
+            raise RuntimeError(f"oracle geometry failed ({rc})")
+        self.maxNonOrthDeg = mx.value
+
+    def pattern(self):
+        self.rowOffsets, self.cols, self.diagPos, self.PP, self.PN, self.NP, self.NN = pattern(
+            self.nCells, self.owner, self.neigh)
+        self.nnz = self.cols.size
+
+
+def pattern(nCells, owner, neigh):
+    owner, neigh = _i(owner), _i(neigh)
+    nF, nIF = owner.size, neigh.size
+    nnz = lib().or_pattern_nnz(nCells, nF, nIF, _p(owner), _p(neigh))
+    ro = np.empty(nCells + 1, np.int32)
+    ci = np.empty(nnz, np.int32)
+    dp = np.empty(nCells, np.int32)
+    f = [np.empty(nIF, np.int32) for _ in range(4)]
+    rc = lib().or_pattern_fill(nCells, nF, nIF, _p(owner), _p(neigh), _p(ro), _p(ci), _p(dp),
+                               *[_p(a) for a in f])
+    if rc != 0:
+        raise RuntimeError("oracle pattern failed")
+    return (ro, ci, dp, *f)
+
+
+def lsq_gradient(m: OMesh, phi, bc_types, bc_vals, mode="lp"):
+    g = np.empty((m.nCells, 3))
+    phi, bt, bv = _d(phi), _i(bc_types), _d(bc_vals)
+    if mode == "gsv1":
+        lib().or_lsq_gradient_gsv1(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner),
+                                   _p(m.rowOffsets), _p(m.cols), _p(m.cc), _p(m.xf), _p(bt),
+                                   _p(bv), _p(phi), _p(g))
+    else:
+        lib().or_lsq_gradient_lp(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner),
+                                 _p(m.rowOffsets), _p(m.cols), _p(m.cc), _p(m.xf), _p(m.nf),
+                                 _p(bt), _p(bv), _p(phi), _p(g))
+    return g
+
+
+def elliptic_assemble(m: OMesh, gamma, src, bc_types, bc_vals, grad, nonorth=True, use_ref=False,
+                      ref_cell=0, ref_value=0.0):
+    vals = np.empty(m.nnz)
+    rhs = np.empty(m.nCells)
+    rc = lib().or_elliptic_assemble(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner),
+                                    _p(m.neigh), _p(m.rowOffsets), _p(m.diagPos), _p(m.PP),
+                                    _p(m.PN), _p(m.NP), _p(m.NN), _p(m.cc), _p(m.vol), _p(m.xf),
+                                    _p(m.Sf), _p(m.Af), _p(_d(gamma)), _p(_d(src)),
+                                    _p(_i(bc_types)), _p(_d(bc_vals)),
+                                    _p(_d(np.asarray(grad).reshape(-1))), int(nonorth),
+                                    int(use_ref), int(ref_cell), C.c_double(ref_value), _p(vals),
+                                    _p(rhs))
+    if rc != 0:
+        raise RuntimeError("Invalid reference cell")
+    return vals, rhs
+
+
+def transport_assemble(m: OMesh, flux, gamma, Su, Sp, bc_types, bc_vals, grad, nonorth=True,
+                       upwind=False):
+    vals = np.empty(m.nnz)
+    rhs = np.empty(m.nCells)
+    lib().or_transport_assemble(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh),
+                                _p(m.diagPos), _p(m.PP), _p(m.PN), _p(m.NP), _p(m.NN), _p(m.cc),
+                                _p(m.vol), _p(m.xf), _p(m.Sf), _p(m.Af), _p(_d(flux)),
+                                _p(_d(gamma)), _p(_d(Su)), _p(_d(Sp)), _p(_i(bc_types)),
+                                _p(_d(bc_vals)), _p(_d(np.asarray(grad).reshape(-1))),
+                                int(nonorth), int(upwind), _p(vals), _p(rhs))
+    return vals, rhs
+
+
+def spmv(ro, ci, va, x):
+    y = np.empty(len(ro) - 1)
+    lib().or_spmv(len(ro) - 1, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), _p(_d(x)), _p(y))
+    return y
+
+
+def pcg_amg(ro, ci, va, b, x0=None, hist_cap=0, **opts):
+    n = len(ro) - 1
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    o = amg_options(**opts)
+    rr = C.c_double()
+    hist = np.zeros(max(hist_cap, 1))
+    it = lib().or_pcg_amg(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), _p(_d(b)), _p(x), C.byref(o),
+                          C.byref(rr), _p(hist) if hist_cap else None, hist_cap)
+    return x, it, rr.value, hist
+
+
+def amg_aggregate(ro, ci, va, **opts):
+    n = len(ro) - 1
+    agg = np.empty(n, np.int32)
+    o = amg_options(**opts)
+    nc = lib().or_amg_aggregate(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), C.byref(o), _p(agg))
+    return agg, nc
+
+
+def amg_vcycle(ro, ci, va, r, **opts):
+    n = len(ro) - 1
+    z = np.empty(n)
+    o = amg_options(**opts)
+    nl = lib().or_amg_vcycle(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), C.byref(o), _p(_d(r)), _p(z))
+    return z, nl
+
+
+def bicgstab(ro, ci, va, b, x0=None, maxIter=1000, relTol=0.0, absTol=1e-10):
+    n = len(ro) - 1
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    rr = C.c_double()
+    it = lib().or_bicgstab_jacobi(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), _p(_d(b)), _p(x),
+                                  maxIter, C.c_double(relTol), C.c_double(absTol), C.byref(rr))
+    return x, it, rr.value
