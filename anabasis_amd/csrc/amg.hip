@@ -17,13 +17,14 @@
 // sorted before use, so the hierarchy is deterministic.
 #include "amg.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace an {
 
 static constexpr int kB = 256;
 static constexpr int kW = kB / 64;
-static constexpr int kCap = 768;
+static constexpr int kCap = kStreamCap;
 static constexpr int kGalCap = 256;  // max entries of one coarse row
 
 __host__ __device__ __forceinline__ uint32_t edge_hash(uint32_t a, uint32_t b) {
@@ -248,7 +249,8 @@ __global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict_
 //                out_i = x_i + omdinv_i (b_i - sum_j a_ij x_j)
 // CSR-stream over 64-row wave slices (see linalg.hip).
 template <typename T, int KIND>
-__global__ __launch_bounds__(kB) void k_level_rows(int n, const int* __restrict__ ro,
+__global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int* __restrict__ rbs,
+                                                   const int* __restrict__ ro,
                                                    const int* __restrict__ ci,
                                                    const T* __restrict__ va,
                                                    const T* __restrict__ omdinv,
@@ -259,29 +261,21 @@ __global__ __launch_bounds__(kB) void k_level_rows(int n, const int* __restrict_
   __shared__ double prod[kW][kCap];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = (lb * kW + w) * 64;
-  const int r1 = min(r0 + 64, n);
-  int k0 = 0, k1 = 0;
-  bool stream = false;
-  if (r0 < n) {
-    k0 = ro[r0];
-    k1 = ro[r1];
-    stream = (k1 - k0) <= kCap;
-  }
-  if (stream)
-    for (int k = k0 + lane; k < k1; k += 64) {
+  const WaveRows q = wave_rows(nrb, rbs, ro, lb * kW + w);
+  if (q.stream)
+    for (int k = q.k0 + lane; k < q.k1; k += 64) {
       const int j = ci[k];
       double xj = (double)omdinv[j] * (double)b[j];
       if (KIND == 1) xj = xj + (double)xc[agg[j]];
-      prod[w][k - k0] = (double)va[k] * xj;
+      prod[w][k - q.k0] = (double)va[k] * xj;
     }
-  __syncthreads();
-  const int i = r0 + lane;
-  if (i < r1) {
+  wave_sync();
+  const int i = q.r0 + lane;
+  if (i < q.r1) {
     double s = 0.0;
     const int kb = ro[i], ke = ro[i + 1];
-    if (stream) {
-      for (int k = kb; k < ke; ++k) s += prod[w][k - k0];
+    if (q.stream) {
+      for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     } else {
       for (int k = kb; k < ke; ++k) {
         const int j = ci[k];
@@ -311,15 +305,35 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
   bc[I] = (T)s;
 }
 
-// coarsest: x_i = sum_j Ainv[i][j] b_j, Ainv stored column-major (FP64)
+// coarsest: x_i = sum_j Ainv[i][j] b_j, Ainv stored column-major (FP64).
+// Blocked order: lane = row, wave = 64-column chunk; each chunk's partial is
+// summed in column order, then the partials in chunk order (n <= 4096).
+static constexpr int kDenseThreads = 1024;
 template <typename T>
-__global__ void k_dense_solve(int n, const double* __restrict__ invT, const T* __restrict__ b,
-                              T* __restrict__ x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int j = 0; j < n; ++j) s += invT[(size_t)j * n + i] * (double)b[j];
-  x[i] = (T)s;
+__global__ __launch_bounds__(kDenseThreads) void k_dense_solve(int n,
+                                                               const double* __restrict__ invT,
+                                                               const T* __restrict__ b,
+                                                               T* __restrict__ x) {
+  __shared__ double part[64][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int nch = (n + 63) >> 6;
+  for (int c = w; c < nch; c += nw) {
+    double t = 0.0;
+    if (i < n) {
+      const int j0 = c * 64, j1 = min(j0 + 64, n);
+#pragma unroll 8
+      for (int j = j0; j < j1; ++j) t += invT[(size_t)j * n + i] * (double)b[j];
+    }
+    part[c][lane] = t;
+  }
+  __syncthreads();
+  if (w == 0 && i < n) {
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += part[c][lane];
+    x[i] = (T)s;
+  }
 }
 
 // plain damped Jacobi sweep (coarsest fallback when too large for dense)
@@ -493,6 +507,7 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
     AmgLevel& C = lv.back();
     C.n = cur;
     C.own = std::move(W);
+    C.own.finish_pattern(s);
     C.A = &lv.back().own;
   }
   // lv may have reallocated while growing: re-point owned matrices
@@ -541,6 +556,7 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
     L.A32.cols.alloc((size_t)A.nnz);
     AN_HIP(hipMemcpyAsync(L.A32.cols.p, A.cols.p, sizeof(int) * (size_t)A.nnz,
                           hipMemcpyDeviceToDevice, s));
+    L.A32.finish_pattern(s);
     L.A32.vals.alloc((size_t)A.nnz);
     k_cast<float><<<grid_for(A.nnz, kB), kB, 0, s>>>(A.nnz, A.vals.p, L.A32.vals.p);
     L.omdinv32.alloc((size_t)L.n);
@@ -562,7 +578,9 @@ static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, T* x, hipStream
   const DevCSR<T>& A = mat(L);
   if (L.isCoarsest) {
     if (L.coarseInvT.p) {
-      k_dense_solve<T><<<grid_for(L.n, 64), 64, 0, s>>>(L.n, L.coarseInvT.p, b, x);
+      const int nch = (L.n + 63) / 64;  // <= 64: dense level only when n <= 4096
+      k_dense_solve<T><<<nch, 64 * std::min(nch, kDenseThreads / 64), 0, s>>>(
+          L.n, L.coarseInvT.p, b, x);
     } else {
       T* t = rbuf(L);
       fill_zero(x, sizeof(T) * (size_t)L.n, s);
@@ -578,13 +596,14 @@ static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, T* x, hipStream
   }
   AmgLevel& C = lv[l + 1];
   T* r = rbuf(L);
-  k_level_rows<T, 0><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L),
-                                                      b, nullptr, nullptr, r);
+  const int G = grid_for(A.rb.count, kW);
+  k_level_rows<T, 0><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
+                                      omd(L), b, nullptr, nullptr, r);
   k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, r, bbuf(C));
   AN_CHECK_LAUNCH();
   vcycle<T>(lv, l + 1, bbuf(C), xbuf(C), s, omd, mat, bbuf, xbuf, rbuf);
-  k_level_rows<T, 1><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L),
-                                                      b, xbuf(C), L.agg.p, x);
+  k_level_rows<T, 1><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
+                                      omd(L), b, xbuf(C), L.agg.p, x);
   AN_CHECK_LAUNCH();
 }
 

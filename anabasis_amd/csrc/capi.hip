@@ -8,6 +8,7 @@
 
 #include "../../include/anabasis_hip.h"
 #include "kernels.hpp"
+#include "simple.hpp"
 #include "solver.hpp"
 
 namespace an {
@@ -22,6 +23,9 @@ struct an_mesh_s {
 };
 struct an_solver_s {
   std::unique_ptr<PcgSolver> pcg;
+};
+struct an_simple_s {
+  std::unique_ptr<SimpleSolver> sim;
 };
 
 #define AN_TRY try {
@@ -515,6 +519,7 @@ an_status an_bicgstab_solve(int n, const int* rowOffsets, const int* cols, int64
   AN_HIP(hipMemcpy(A.rowOff.p, rowOffsets, sizeof(int) * ((size_t)n + 1), k));
   AN_HIP(hipMemcpy(A.cols.p, cols, sizeof(int) * (size_t)nnz, k));
   AN_HIP(hipMemcpy(A.vals.p, values, sizeof(double) * (size_t)nnz, k));
+  A.finish_pattern(0);
   InArr<double> bb(b, (size_t)n, mem, 0);
   DevBuf<double> xh;
   double* xd = x;
@@ -632,6 +637,7 @@ an_status an_solve_steady_scalar_transport(an_mesh_t mesh, const double* faceFlu
                         hipMemcpyDeviceToDevice, s));
   AN_HIP(hipMemcpyAsync(A.cols.p, d.addr.cols.p, sizeof(int) * (size_t)A.nnz,
                         hipMemcpyDeviceToDevice, s));
+  A.finish_pattern(s);
   if (x0)
     AN_HIP(hipMemcpyAsync(x.p, x0, sizeof(double) * d.nC, kind_in(mem), s));
   else
@@ -652,6 +658,105 @@ an_status an_solve_steady_scalar_transport(an_mesh_t mesh, const double* faceFlu
   if (info) *info = last;
   if (st == AN_ENOCONV) set_last_error("BiCGSTAB did not converge");
   return st;
+  AN_CATCH
+}
+
+
+// ------------------------------------------------------------ SIMPLE ----
+void an_simple_default_params(an_simple_params* p) {
+  if (!p) return;
+  // generic_simple_v1 Params defaults (main.cu:73-90) with the validated
+  // cylinder controls of cases/cylinder.case:26-50
+  p->rho = 1.0;
+  p->mu = 1e-3;
+  p->nVelNonOrthCorr = 1;
+  p->nNonOrthCorr = 10;
+  p->uRelax = 0.5;
+  p->pRelax = 0.3;
+  p->corrTol = 1e-10;
+  p->tolMass = 1e-6;
+  p->tolVel = 1e-6;
+  p->velMaxit = 1000;
+  p->velTol = 1e-10;
+  p->velRelTol = 0.0;
+  p->pMaxit = 4000;
+  p->pTol = 1e-10;
+  p->pRelTol = 0.0;
+  p->pAmgRebuildEvery = 10;
+  p->precision = 0;
+  p->fuseMomentumRhs = 0;
+  default_pcg_options(&p->pressure);
+}
+
+an_status an_simple_create(an_mesh_t mesh, const an_simple_params* par, const int* patchUType,
+                           const int* patchPType, const double* uFaceBC, const double* vFaceBC,
+                           const double* wFaceBC, const double* pFaceBC, int inletPatch,
+                           an_simple_t* out) {
+  AN_TRY
+  if (!mesh || !par || !patchUType || !patchPType || !uFaceBC || !vFaceBC || !wFaceBC ||
+      !pFaceBC || !out)
+    throw Error(AN_EINVAL, "NULL argument");
+  if (par->precision != 0 && par->precision != 1) throw Error(AN_EINVAL, "precision must be 0/1");
+  if (!(par->uRelax > 0.0) || !(par->mu >= 0.0)) throw Error(AN_EINVAL, "invalid uRelax / mu");
+  auto* h = new an_simple_s;
+  try {
+    h->sim.reset(new SimpleSolver(mesh->dm, *par, patchUType, patchPType, uFaceBC, vFaceBC,
+                                  wFaceBC, pFaceBC, inletPatch));
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_simple_iterate(an_simple_t s, int n, an_simple_stats* last, int* converged) {
+  AN_TRY
+  if (!s || n < 0) throw Error(AN_EINVAL, "invalid argument");
+  an_status st = s->sim->iterate(n, last, converged);
+  if (st == AN_EDIVERGED) set_last_error("SIMPLE-like steady solve diverged.");
+  return st;
+  AN_CATCH
+}
+
+an_status an_simple_get_field(an_simple_t s, const char* name, double* out, an_memspace mem) {
+  AN_TRY
+  if (!s || !name || !out) throw Error(AN_EINVAL, "NULL argument");
+  s->sim->get_field(name, out, mem == AN_DEVICE);
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_simple_set_field(an_simple_t s, const char* name, const double* in,
+                              an_memspace mem) {
+  AN_TRY
+  if (!s || !name || !in) throw Error(AN_EINVAL, "NULL argument");
+  s->sim->set_field(name, in, mem == AN_DEVICE);
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_simple_patch_forces(an_simple_t s, int patch, int normalSign, double* out) {
+  AN_TRY
+  if (!s || !out) throw Error(AN_EINVAL, "NULL argument");
+  s->sim->patch_forces(patch, normalSign, out);
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_simple_timings(an_simple_t s, double* out) {
+  AN_TRY
+  if (!s || !out) throw Error(AN_EINVAL, "NULL argument");
+  s->sim->timings(out);
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_simple_destroy(an_simple_t s) {
+  AN_TRY
+  delete s;
+  return AN_OK;
   AN_CATCH
 }
 

@@ -1,28 +1,69 @@
 // linalg.hip -- CSR SpMV, fused Krylov vector updates and deterministic
 // reductions for the PCG/BiCGSTAB solvers.
 //
-// SpMV (CSR-stream, wave granularity): each 64-lane wavefront owns 64
-// consecutive rows.  The lanes first stream the rows' nonzeros with fully
-// coalesced loads of vals/cols (lane k reads nonzero k0+k), multiply by the
-// gathered x and stage the products in LDS; then lane i sums row i's
-// products from LDS in CSR order.  Rows are 4-27 entries long on polyhedral
-// meshes, so thread-per-row loads would be strided by the row length; the
-// stream form turns them into unit-stride loads.  A wave whose rows hold more
-// nonzeros than its LDS slice falls back to thread-per-row.  Sums run in CSR
-// order, so y equals the serial product bit for bit.
+// SpMV (CSR-stream, wave granularity, after Greathouse & Daga's CSR-Adaptive,
+// SC'14): each 64-lane wavefront owns one row block (at most 64 consecutive
+// rows holding at most 768 nonzeros, built once per pattern).  The lanes
+// first stream the block's nonzeros with fully coalesced loads of vals/cols
+// (lane k reads nonzero k0+k), multiply by the gathered x and stage the
+// products in the wave's LDS slice; then lane i sums row i's products from
+// LDS in CSR order.  Rows are 4-27 entries long on polyhedral meshes and
+// 20-60 on AMG coarse levels, so thread-per-row loads would be strided by
+// the row length; the stream form turns them into unit-stride loads.  A row
+// longer than the LDS slice forms its own block and is summed by one lane
+// from global memory.  Sums run in CSR order, so y equals the serial product
+// bit for bit.
 //
 // Reductions: every block writes a partial sum (fixed element->block map,
-// fixed shuffle tree), the last block to finish (atomic ticket) adds the
-// partials in index order and writes the scalar to device memory: results
-// are deterministic and scalars never leave the GPU inside an iteration.
+// fixed shuffle tree) and a one-block finalize kernel adds the partials in
+// index order into a device scalar: results are deterministic, no global
+// atomics are involved, and scalars never leave the GPU inside an iteration.
 #include "linalg.hpp"
 
 namespace an {
 
 static constexpr int kBlock = 256;
 static constexpr int kWaves = kBlock / 64;
-static constexpr int kCap = 768;  // staged products per wave (6 KB)
+static constexpr int kCap = kStreamCap;  // staged products per wave (6 KB)
 static constexpr int kRedBlocks = 1024;
+
+// ------------------------------------------------------------ row blocks
+// Fixed 64-row slices are split greedily where their nonzeros exceed the
+// cap; pass 0 counts the blocks of each slice, pass 1 writes their starts.
+template <int PASS>
+__global__ void k_row_blocks(int n, const int* __restrict__ ro, const int* __restrict__ off,
+                             int* __restrict__ out) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nsl = (n + kStreamRows - 1) / kStreamRows;
+  if (sl >= nsl) return;
+  const int a = sl * kStreamRows, b = min(a + kStreamRows, n);
+  int cnt = 0, r = a;
+  while (r < b) {
+    const int st = r;
+    ++r;
+    while (r < b && ro[r + 1] - ro[st] <= kStreamCap) ++r;
+    if (PASS == 1) out[off[sl] + cnt] = st;
+    ++cnt;
+  }
+  if (PASS == 0) out[sl] = cnt;
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s) {
+  const int nsl = (n + kStreamRows - 1) / kStreamRows;
+  DevBuf<int> cnt((size_t)nsl + 1), off((size_t)nsl + 1);
+  k_row_blocks<0><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, nullptr, cnt.p);
+  AN_CHECK_LAUNCH();
+  const long long tot = exclusive_scan(cnt.p, off.p, nsl, s);
+  rb.count = (int)tot;
+  rb.start.alloc((size_t)tot + 1);
+  k_row_blocks<1><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, off.p, rb.start.p);
+  k_set_int<<<1, 1, 0, s>>>(rb.start.p + tot, n);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipStreamSynchronize(s));
+}
+
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -42,73 +83,66 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;
 }
 
-// Writes this block's partial; the last block reduces all partials into *out.
-__device__ void finish_reduction(double blockVal, int blockId, int nblocks,
-                                 double* __restrict__ partial, unsigned int* __restrict__ counter,
-                                 double* __restrict__ out, double* sh) {
-  __shared__ int isLast;
-  if (threadIdx.x == 0) {
-    partial[blockId] = blockVal;
-    __threadfence();
-    unsigned int t = atomicAdd(counter, 1u);
-    isLast = (t == (unsigned int)nblocks - 1u);
+// Two-stage deterministic reduction: every block stores its partial, then
+// k_finalize (one block) adds the partials in index order.  No global
+// atomics: a single device-wide ticket serialises across the XCDs.
+__device__ __forceinline__ void store_partial(double blockVal, int blockId,
+                                              double* __restrict__ partial) {
+  if (threadIdx.x == 0) partial[blockId] = blockVal;
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(int nparts, int comps,
+                                                   const double* __restrict__ partial,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[16];
+  for (int c = 0; c < comps; ++c) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[(size_t)i * comps + c];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[c] = s;
+    __syncthreads();
   }
-  __syncthreads();
-  if (!isLast) return;
-  __threadfence();
-  double s = 0.0;
-  const volatile double* vp = partial;
-  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) s += vp[i];
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) {
-    *out = s;
-    *counter = 0u;
-  }
+}
+
+void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s) {
+  k_finalize<<<1, 1024, 0, s>>>(nparts, comps, red.partial.p, out);
+  AN_CHECK_LAUNCH();
 }
 
 void Reducer::init(int maxBlocks, hipStream_t s) {
   cap = maxBlocks;
-  partial.alloc((size_t)maxBlocks);
-  counter.alloc(1);
-  counter.zero(s);
+  partial.alloc(4 * (size_t)maxBlocks);
+  (void)s;
 }
 
 // ------------------------------------------------------------------ SpMV
 // MODE 0: y = A x ; MODE 1: y = b - A x.  DOT 0: none; 1: dot(dw, y);
 // 2: dot(y, y).
 template <typename VT, typename XT, typename YT, int MODE, int DOT>
-__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ rowOff,
+__global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int* __restrict__ rbs,
+                                                 const int* __restrict__ rowOff,
                                                  const int* __restrict__ cols,
                                                  const VT* __restrict__ vals,
                                                  const XT* __restrict__ x,
                                                  const double* __restrict__ b, YT* __restrict__ y,
                                                  const YT* __restrict__ dw,
-                                                 double* __restrict__ partial,
-                                                 unsigned int* __restrict__ counter,
-                                                 double* __restrict__ out) {
+                                                 double* __restrict__ partial) {
   __shared__ double prod[kWaves][kCap];
   __shared__ double sh[kWaves];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r0 = (lb * kWaves + w) * 64;
-  const int r1 = min(r0 + 64, n);
-  int k0 = 0, k1 = 0;
-  bool stream = false;
-  if (r0 < n) {
-    k0 = rowOff[r0];
-    k1 = rowOff[r1];
-    stream = (k1 - k0) <= kCap;
-  }
-  if (stream)
-    for (int k = k0 + lane; k < k1; k += 64) prod[w][k - k0] = (double)vals[k] * (double)x[cols[k]];
-  __syncthreads();
+  const WaveRows q = wave_rows(nrb, rbs, rowOff, lb * kWaves + w);
+  if (q.stream)
+    for (int k = q.k0 + lane; k < q.k1; k += 64)
+      prod[w][k - q.k0] = (double)vals[k] * (double)x[cols[k]];
+  wave_sync();
   double dv = 0.0;
-  const int row = r0 + lane;
-  if (row < r1) {
+  const int row = q.r0 + lane;
+  if (row < q.r1) {
     const int kb = rowOff[row], ke = rowOff[row + 1];
     double s = 0.0;
-    if (stream)
-      for (int k = kb; k < ke; ++k) s += prod[w][k - k0];
+    if (q.stream)
+      for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     else
       for (int k = kb; k < ke; ++k) s += (double)vals[k] * (double)x[cols[k]];
     YT v;
@@ -122,23 +156,30 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ 
   }
   if (DOT) {
     double bs = block_sum(dv, sh);
-    finish_reduction(bs, lb, gridDim.x, partial, counter, out, sh);
+    store_partial(bs, lb, partial);
   }
+}
+
+template <typename VT>
+static int spmv_grid(const DevCSR<VT>& A) {
+  if (A.n > 0 && A.rb.count == 0) throw Error(7, "CSR row blocks not built");
+  return grid_for(A.rb.count, kWaves);
 }
 
 template <typename VT, typename XT>
 void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* scal, int slot,
           Reducer& red, hipStream_t s) {
-  const int rowsPerBlock = kBlock;
-  const int nb = grid_for(A.n, rowsPerBlock);
+  const int nb = spmv_grid(A);
   if (dotWith) {
     if (nb > red.cap) throw Error(7, "reducer too small");
-    k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.n, A.rowOff.p, A.cols.p, A.vals.p, x, nullptr,
-                                                   y, dotWith, red.partial.p, red.counter.p,
-                                                   scal + slot);
+    k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
+                                                   A.vals.p, x, nullptr, y, dotWith,
+                                                   red.partial.p);
+    AN_CHECK_LAUNCH();
+    finalize(red, nb, 1, scal + slot, s);
   } else {
-    k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.n, A.rowOff.p, A.cols.p, A.vals.p, x, nullptr,
-                                                   y, nullptr, nullptr, nullptr, nullptr);
+    k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
+                                                   A.vals.p, x, nullptr, y, nullptr, nullptr);
   }
   AN_CHECK_LAUNCH();
 }
@@ -146,12 +187,13 @@ void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* sc
 template <typename VT, typename RT>
 void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, double* scal,
               Reducer& red, hipStream_t s) {
-  const int nb = grid_for(A.n, kBlock);
+  const int nb = spmv_grid(A);
   if (nb > red.cap) throw Error(7, "reducer too small");
-  k_spmv<VT, double, RT, 1, 2><<<nb, kBlock, 0, s>>>(A.n, A.rowOff.p, A.cols.p, A.vals.p, x, b, r,
-                                                     nullptr, red.partial.p, red.counter.p,
-                                                     scal + S_RR);
+  k_spmv<VT, double, RT, 1, 2><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
+                                                     A.cols.p, A.vals.p, x, b, r, nullptr,
+                                                     red.partial.p);
   AN_CHECK_LAUNCH();
+  finalize(red, nb, 1, scal + S_RR, s);
 }
 
 // ---------------------------------------------------------- vector ops
@@ -163,21 +205,21 @@ static inline int red_blocks(int n) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(int n, const T* __restrict__ a,
                                                 const T* __restrict__ b,
-                                                double* __restrict__ partial,
-                                                unsigned int* __restrict__ counter,
-                                                double* __restrict__ out) {
+                                                double* __restrict__ partial) {
   __shared__ double sh[kWaves];
   double v = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     v += (double)a[i] * (double)b[i];
   double bs = block_sum(v, sh);
-  finish_reduction(bs, blockIdx.x, gridDim.x, partial, counter, out, sh);
+  store_partial(bs, blockIdx.x, partial);
 }
 
 template <typename T>
 void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hipStream_t s) {
-  k_dot<T><<<red_blocks(n), kBlock, 0, s>>>(n, a, b, red.partial.p, red.counter.p, scal + slot);
+  const int nb = red_blocks(n);
+  k_dot<T><<<nb, kBlock, 0, s>>>(n, a, b, red.partial.p);
   AN_CHECK_LAUNCH();
+  finalize(red, nb, 1, scal + slot, s);
 }
 
 template <typename T>
@@ -186,9 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const double* __restrict__ scal, int num,
-                                                       int den, double* __restrict__ partial,
-                                                       unsigned int* __restrict__ counter,
-                                                       double* __restrict__ out) {
+                                                       int den, double* __restrict__ partial) {
   __shared__ double sh[kWaves];
   const double alpha = scal[num] / scal[den];
   double v = 0.0;
@@ -199,15 +239,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
     v += (double)ri * (double)ri;
   }
   double bs = block_sum(v, sh);
-  finish_reduction(bs, blockIdx.x, gridDim.x, partial, counter, out, sh);
+  store_partial(bs, blockIdx.x, partial);
 }
 
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s) {
-  k_pcg_update<T><<<red_blocks(n), kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.partial.p,
-                                                   red.counter.p, scal + S_RR);
+  const int nb = red_blocks(n);
+  k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.partial.p);
   AN_CHECK_LAUNCH();
+  finalize(red, nb, 1, scal + S_RR, s);
 }
 
 template <typename T>

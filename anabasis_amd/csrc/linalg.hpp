@@ -5,6 +5,19 @@
 
 namespace an {
 
+// Row blocks of the CSR-stream kernels: consecutive rows grouped so that a
+// block holds at most kStreamRows rows and at most kStreamCap nonzeros (a
+// single longer row forms a block of its own).  One 64-lane wave owns one
+// block; start[count] = n.
+static constexpr int kStreamRows = 64;
+static constexpr int kStreamCap = 768;
+struct RowBlocks {
+  DevBuf<int> start;
+  int count = 0;
+};
+// Greedy partition of rowOff[0..n] (synchronises once to read the count).
+void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s);
+
 // Device CSR matrix (values in T).
 template <typename T>
 struct DevCSR {
@@ -12,20 +25,49 @@ struct DevCSR {
   long long nnz = 0;
   DevBuf<int> rowOff, cols, diag;
   DevBuf<T> vals;
+  RowBlocks rb;
+  void finish_pattern(hipStream_t s) { build_row_blocks(n, rowOff.p, rb, s); }
 };
+
+// Waves synchronise their own LDS slice without a block barrier.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Row block of wave w of (remapped) block lb: rows [r0, r1), nonzeros
+// [k0, k1); stream = the block's products fit the LDS slice.
+struct WaveRows {
+  int r0, r1, k0, k1;
+  bool stream;
+};
+__device__ __forceinline__ WaveRows wave_rows(int nrb, const int* __restrict__ rbs,
+                                              const int* __restrict__ rowOff, int gw) {
+  WaveRows q{0, 0, 0, 0, false};
+  if (gw < nrb) {
+    q.r0 = rbs[gw];
+    q.r1 = rbs[gw + 1];
+    q.k0 = rowOff[q.r0];
+    q.k1 = rowOff[q.r1];
+    q.stream = (q.k1 - q.k0) <= kStreamCap;
+  }
+  return q;
+}
 
 // Scalar slots of a solver (device memory).
 enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 5, S_TMP2 = 6,
                   S_AL = 7, S_OM = 8, S_R0V = 9, S_TS = 10, S_TT = 11, S_SS = 12, S_COUNT = 16 };
 
-// Deterministic reduction workspace: per-block partials + completion
-// counter; the last block to finish sums the partials in a fixed order.
+// Deterministic reduction workspace: per-block partials, summed in index
+// order by a one-block finalize kernel.
 struct Reducer {
-  DevBuf<double> partial;
-  DevBuf<unsigned int> counter;
+  DevBuf<double> partial;  // 4 doubles per block
   int cap = 0;
   void init(int maxBlocks, hipStream_t s);
 };
+// out[c] = sum_i partial[i*comps + c], i < nparts (one block, fixed order)
+void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s);
 
 // y = A x.  When dotWith != nullptr also computes dot(dotWith, y) into
 // scal[slot] (deterministically).

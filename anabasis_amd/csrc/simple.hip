@@ -1,0 +1,797 @@
+// simple.hip -- SIMPLE hot path of generic_simple_v1 on the GPU.
+//
+// Outer iteration of apps/generic_simple_v1/src/main.cu:3388-3634 with every
+// field resident in HBM (the reference copies ~16 nC-sized vectors across
+// PCIe per iteration).  Kernels:
+//   k_mom<FULL>      momentum matrix + rhs (main.cu:1673-1896), relaxation
+//                    (:1686-1695 / :1787-1794) and rAU (:1796-1802) fused,
+//                    one thread per row walking its faces in face order
+//   k_p_asm          pressure matrix from rAU (:1898-1950)
+//   k_rc             Rhie-Chow predicted flux (:1474-1521), one thread/face
+//   k_nonorth_flux   non-orthogonal pressure flux (:1538-1593), per face
+//   k_div            continuity / non-orthogonal divergence (+ pressure rhs
+//                    -div* + div_no, :1595-1604), per cell in face order
+//   k_correct_flux   flux correction (:1615-1658), per face
+//   k_vel_p_update   p += a_p p', u = u* - rAU grad p' (:1447, 1660-1671)
+//   k_force          wall-force contributions (:2828-2933) + ordered reduce
+// Gathered sums run in the reference's serial face order, so a face quantity
+// or a row equals the serial reference bit for bit; reductions (massRes,
+// relchg, dots) use a fixed tree and are deterministic.
+#include "simple.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace an {
+
+static constexpr int kB = 256;
+
+// ----------------------------------------------------------- momentum ----
+struct MomArgs {
+  const int* bcT;          // velocity BC type per boundary face
+  const double* qB;        // component BC value per boundary face
+  const double *uB, *vB, *wB;
+  const double *gqx, *gqy, *gqz;  // gradient of the current iterate
+  const double *uC, *vC, *wC;     // convecting velocity (old fields)
+  const double* gpc;       // pressure-gradient component
+  const double* qOld;      // iterate used by the relaxation
+  double rho, mu, corrPsi, uRelax;
+};
+
+template <bool FULL>
+__global__ __launch_bounds__(kB) void k_mom(MeshView m, MomArgs a, double* __restrict__ vals,
+                                            double* __restrict__ rhs, double* __restrict__ rAU) {
+  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.nC) return;
+  const double V = m.cgeo[c].w;
+  double diag = 0.0;
+  double r = -a.gpc[c] * V;
+  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+    const int e = m.cfEnt[k];
+    const int f = e >> 1, side = e & 1;
+    const int slotInfo = m.cfSlot[k];
+    const double nd = m.fDpn[f];
+    if (nd <= 1e-14) {  // main.cu:1826 -- the face contributes nothing
+      if (FULL && (slotInfo & 1)) vals[slotInfo >> 1] = 0.0;
+      continue;
+    }
+    const double af = m.fA[f];
+    const double dx = m.fDx[f], dy = m.fDy[f], dz = m.fDz[f];
+    const double nx = m.fNx[f], ny = m.fNy[f], nz = m.fNz[f];
+    const double tcx = a.mu * (af * nx - (af / nd) * dx);
+    const double tcy = a.mu * (af * ny - (af / nd) * dy);
+    const double tcz = a.mu * (af * nz - (af / nd) * dz);
+    const double lam = m.fLam[f];
+    const int2 pn = m.fPN[f];
+    const double gfx = (1.0 - lam) * a.gqx[pn.x] + lam * a.gqx[pn.y];
+    const double gfy = (1.0 - lam) * a.gqy[pn.x] + lam * a.gqy[pn.y];
+    const double gfz = (1.0 - lam) * a.gqz[pn.x] + lam * a.gqz[pn.y];
+    const double corr = a.corrPsi * (gfx * tcx + gfy * tcy + gfz * tcz);
+    if (FULL) {
+      const double alpha = a.mu * af / nd;
+      const double ucf = (1.0 - lam) * a.uC[pn.x] + lam * a.uC[pn.y];
+      const double vcf = (1.0 - lam) * a.vC[pn.x] + lam * a.vC[pn.y];
+      const double wcf = (1.0 - lam) * a.wC[pn.x] + lam * a.wC[pn.y];
+      const double F = a.rho * af * (ucf * nx + vcf * ny + wcf * nz);
+      const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+      if (side == 0) {
+        diag += (alpha + F * (1.0 - lam));
+        vals[slotInfo >> 1] = old + (-alpha + F * lam);
+      } else {
+        diag += (alpha - F * lam);
+        vals[slotInfo >> 1] = old + (-alpha - F * (1.0 - lam));
+      }
+    }
+    if (side == 0)
+      r += corr;
+    else
+      r += (-corr);
+  }
+  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
+    const int b = m.cbFace[j];
+    const double nd = m.bDpn[b];
+    if (nd <= 1e-14) continue;
+    const double af = m.bA[b];
+    const double dx = m.bDx[b], dy = m.bDy[b], dz = m.bDz[b];
+    const double nx = m.bNx[b], ny = m.bNy[b], nz = m.bNz[b];
+    const double alpha = a.mu * af / nd;
+    const double tcx = a.mu * (af * nx - (af / nd) * dx);
+    const double tcy = a.mu * (af * ny - (af / nd) * dy);
+    const double tcz = a.mu * (af * nz - (af / nd) * dz);
+    const bool dir = a.bcT[b] == 1;
+    if (dir) {
+      const double g = a.qB[b];
+      const double corr = a.corrPsi * (a.gqx[c] * tcx + a.gqy[c] * tcy + a.gqz[c] * tcz);
+      if (FULL) diag += alpha;
+      r += (alpha * g + corr);
+    }
+    const double ucb = dir ? a.uB[b] : a.uC[c];
+    const double vcb = dir ? a.vB[b] : a.vC[c];
+    const double wcb = dir ? a.wB[b] : a.wC[c];
+    const double F = a.rho * af * (ucb * nx + vcb * ny + wcb * nz);
+    if (dir)
+      r += (-F * a.qB[b]);
+    else if (FULL)
+      diag += F;
+  }
+  const int dp = m.diagPos[c];
+  if (FULL) {
+    double aR = diag;
+    if (a.uRelax < 0.999999) {
+      const double invRelax = 1.0 / a.uRelax;
+      aR = diag * invRelax;
+      r += (invRelax - 1.0) * diag * a.qOld[c];
+    }
+    vals[dp] = aR;
+    rAU[c] = (fabs(aR) > 1e-30) ? V / aR : 0.0;
+  } else if (a.uRelax < 0.999999) {
+    const double aPold = vals[dp] * a.uRelax;
+    const double invRelax = 1.0 / a.uRelax;
+    r += (invRelax - 1.0) * aPold * a.qOld[c];
+  }
+  rhs[c] = r;
+}
+
+// ----------------------------------------------------------- pressure ----
+__global__ __launch_bounds__(kB) void k_p_asm(MeshView m, const int* __restrict__ bcP,
+                                              const double* __restrict__ rAU,
+                                              double* __restrict__ vals) {
+  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.nC) return;
+  double diag = 0.0;
+  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+    const int f = m.cfEnt[k] >> 1;
+    const int slotInfo = m.cfSlot[k];
+    const double dpn = m.fDpn[f];
+    if (dpn <= 1e-14) {
+      if (slotInfo & 1) vals[slotInfo >> 1] = 0.0;
+      continue;
+    }
+    const double lam = m.fLam[f];
+    const int2 pn = m.fPN[f];
+    const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
+    const double coeff = m.fA[f] * rAUf / dpn;
+    diag += coeff;
+    const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+    vals[slotInfo >> 1] = old + (-coeff);
+  }
+  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
+    const int b = m.cbFace[j];
+    if (bcP[b] != 1) continue;
+    const double dpn = m.bDpn[b];
+    if (dpn <= 1e-14) continue;
+    diag += m.bA[b] * rAU[c] / dpn;
+  }
+  vals[m.diagPos[c]] = diag;
+}
+
+// --------------------------------------------------------- face kernels ---
+struct FaceArgs {
+  const int *bcU, *bcP;
+  const double *uB, *vB, *wB, *pB;
+};
+
+__global__ __launch_bounds__(kB) void k_rc(MeshView m, FaceArgs a, const double* __restrict__ u,
+                                           const double* __restrict__ v,
+                                           const double* __restrict__ w,
+                                           const double* __restrict__ p,
+                                           const double* __restrict__ gx,
+                                           const double* __restrict__ gy,
+                                           const double* __restrict__ gz,
+                                           const double* __restrict__ rAU, double rho,
+                                           double* __restrict__ phiStar,
+                                           double* __restrict__ phi) {
+  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (f >= m.nF) return;
+  double out;
+  if (f < m.nIF) {
+    const int2 pn = m.fPN[f];
+    const int P = pn.x, N = pn.y;
+    const double dx = m.fDx[f], dy = m.fDy[f], dz = m.fDz[f];
+    const double dpn = m.fDpn[f];
+    const double lam = m.fLam[f];
+    const double Af = m.fA[f];
+    const double ubar = (1.0 - lam) * u[P] + lam * u[N];
+    const double vbar = (1.0 - lam) * v[P] + lam * v[N];
+    const double wbar = (1.0 - lam) * w[P] + lam * w[N];
+    const double gpx = (1.0 - lam) * gx[P] + lam * gx[N];
+    const double gpy = (1.0 - lam) * gy[P] + lam * gy[N];
+    const double gpz = (1.0 - lam) * gz[P] + lam * gz[N];
+    const double rAUf = (1.0 - lam) * rAU[P] + lam * rAU[N];
+    const double phiInterp = rho * Af * (ubar * m.fNx[f] + vbar * m.fNy[f] + wbar * m.fNz[f]);
+    const double rc = rho * Af * rAUf / fmax(fabs(dpn), 1.0e-30) *
+                      ((p[N] - p[P]) - (gpx * dx + gpy * dy + gpz * dz));
+    out = phiInterp - rc;
+  } else {
+    const int b = f - m.nIF;
+    const int P = m.bP[b];
+    const bool dir = a.bcU[b] == 1;
+    const double uf = dir ? a.uB[b] : u[P];
+    const double vf = dir ? a.vB[b] : v[P];
+    const double wf = dir ? a.wB[b] : w[P];
+    out = rho * m.bA[b] * (uf * m.bNx[b] + vf * m.bNy[b] + wf * m.bNz[b]);
+  }
+  phiStar[f] = out;
+  phi[f] = out;  // phi = phi* (main.cu:3439)
+}
+
+__global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
+                                                     const double* __restrict__ rAU,
+                                                     const double* __restrict__ gx,
+                                                     const double* __restrict__ gy,
+                                                     const double* __restrict__ gz,
+                                                     double* __restrict__ phiNo) {
+  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (f >= m.nF) return;
+  double flux = 0.0;
+  if (f < m.nIF) {
+    const double dpn = m.fDpn[f];
+    if (fabs(dpn) > 1.0e-30) {
+      const int2 pn = m.fPN[f];
+      const double lam = m.fLam[f];
+      const double Af = m.fA[f];
+      const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
+      const double gfx = (1.0 - lam) * gx[pn.x] + lam * gx[pn.y];
+      const double gfy = (1.0 - lam) * gy[pn.x] + lam * gy[pn.y];
+      const double gfz = (1.0 - lam) * gz[pn.x] + lam * gz[pn.y];
+      const double tx = Af * m.fNx[f] - (Af / dpn) * m.fDx[f];
+      const double ty = Af * m.fNy[f] - (Af / dpn) * m.fDy[f];
+      const double tz = Af * m.fNz[f] - (Af / dpn) * m.fDz[f];
+      flux = rAUf * (gfx * tx + gfy * ty + gfz * tz);
+    }
+  } else {
+    const int b = f - m.nIF;
+    if (a.bcP[b] == 1) {
+      const double dpn = m.bDpn[b];
+      if (fabs(dpn) > 1.0e-30) {
+        const int P = m.bP[b];
+        const double Af = m.bA[b];
+        const double tx = Af * m.bNx[b] - (Af / dpn) * m.bDx[b];
+        const double ty = Af * m.bNy[b] - (Af / dpn) * m.bDy[b];
+        const double tz = Af * m.bNz[b] - (Af / dpn) * m.bDz[b];
+        flux = rAU[P] * (gx[P] * tx + gy[P] * ty + gz[P] * tz);
+      }
+    }
+  }
+  phiNo[f] = flux;
+}
+
+// MODE 0: div = continuity of phi (all faces).
+// MODE 1: non-orthogonal divergence of phiNo (active faces only) and the
+//         pressure rhs b = -divStar + divNo.
+template <int MODE>
+__global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ bcP,
+                                            const double* __restrict__ flux,
+                                            const double* __restrict__ divStar,
+                                            double* __restrict__ out) {
+  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.nC) return;
+  double d = 0.0;
+  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+    const int e = m.cfEnt[k];
+    const int f = e >> 1;
+    if (MODE == 1 && !(fabs(m.fDpn[f]) > 1.0e-30)) continue;
+    if (e & 1)
+      d += -flux[f];
+    else
+      d += flux[f];
+  }
+  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
+    const int b = m.cbFace[j];
+    if (MODE == 1 && !(bcP[b] == 1 && fabs(m.bDpn[b]) > 1.0e-30)) continue;
+    d += flux[m.nIF + b];
+  }
+  out[c] = (MODE == 0) ? d : (-divStar[c] + d);
+}
+
+__global__ __launch_bounds__(kB) void k_correct_flux(MeshView m, FaceArgs a,
+                                                     const double* __restrict__ rAU,
+                                                     const double* __restrict__ phiStar,
+                                                     const double* __restrict__ pCorr,
+                                                     const double* __restrict__ phiNo,
+                                                     double* __restrict__ phi) {
+  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (f >= m.nF) return;
+  if (f < m.nIF) {
+    const int2 pn = m.fPN[f];
+    const double lam = m.fLam[f];
+    const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
+    const double coeff = m.fA[f] * rAUf / m.fDpn[f];
+    phi[f] = phiStar[f] - coeff * (pCorr[pn.y] - pCorr[pn.x]) - phiNo[f];
+  } else {
+    const int b = f - m.nIF;
+    if (a.bcP[b] == 1) {
+      const double dpn = m.bDpn[b];
+      if (fabs(dpn) <= 1.0e-30) {
+        phi[f] = phiStar[f];
+        return;
+      }
+      const int P = m.bP[b];
+      const double coeff = m.bA[b] * rAU[P] / dpn;
+      phi[f] = phiStar[f] - coeff * (a.pB[b] - pCorr[P]) - phiNo[f];
+    } else {
+      phi[f] = phiStar[f];
+    }
+  }
+}
+
+__global__ void k_p_update(int n, double* __restrict__ p, const double* __restrict__ pc,
+                           double pRelax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] += pRelax * pc[i];
+}
+
+__global__ void k_vel_correct(int n, double* __restrict__ u, double* __restrict__ v,
+                              double* __restrict__ w, const double* __restrict__ rAU,
+                              const double* __restrict__ gx, const double* __restrict__ gy,
+                              const double* __restrict__ gz) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  u[c] = u[c] - rAU[c] * gx[c];
+  v[c] = v[c] - rAU[c] * gy[c];
+  w[c] = w[c] - rAU[c] * gz[c];
+}
+
+// ----------------------------------------------------------- reductions --
+__device__ __forceinline__ double wsum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// per-block partials of sum (a-b)^2, sum a^2, max |a| (b may be null)
+__global__ __launch_bounds__(kB) void k_stats(int n, const double* __restrict__ a,
+                                              const double* __restrict__ b,
+                                              double* __restrict__ partial) {
+  __shared__ double sh[3][kB / 64];
+  double s0 = 0.0, s1 = 0.0, mx = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    const double ai = a[i];
+    if (b) {
+      const double d = ai - b[i];
+      s0 += d * d;
+    }
+    s1 += ai * ai;
+    mx = fmax(mx, fabs(ai));
+  }
+  s0 = wsum(s0);
+  s1 = wsum(s1);
+  mx = wmax(mx);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][w] = s0;
+    sh[1][w] = s1;
+    sh[2][w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int i = 0; i < kB / 64; ++i) {
+      t0 += sh[0][i];
+      t1 += sh[1][i];
+      t2 = fmax(t2, sh[2][i]);
+    }
+    partial[3 * blockIdx.x] = t0;
+    partial[3 * blockIdx.x + 1] = t1;
+    partial[3 * blockIdx.x + 2] = t2;
+  }
+}
+
+// out[0..2] from the block partials of k_stats, summed in block order
+__global__ void k_stats_fin(int nparts, const double* __restrict__ partial,
+                            double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    t0 += partial[3 * i];
+    t1 += partial[3 * i + 1];
+    t2 = fmax(t2, partial[3 * i + 2]);
+  }
+  out[0] = t0;
+  out[1] = t1;
+  out[2] = t2;
+}
+
+// per-face wall-force terms (main.cu:2870-2910); stat = 12 doubles per face:
+// Fp[3], Fv[3], area, used, dn, Utmag, shear, valid-dn flag
+__global__ void k_force_faces(MeshView m, int f0, int nf, const double* __restrict__ u,
+                              const double* __restrict__ v, const double* __restrict__ w,
+                              const double* __restrict__ p, double mu, int normalSign,
+                              double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int b = f0 + i - m.nIF;
+  double* o = out + 12 * (size_t)i;
+  for (int k = 0; k < 12; ++k) o[k] = 0.0;
+  const int P = m.bP[b];
+  const double A = m.bA[b];
+  if (A <= 1.0e-300) return;
+  double n0 = (double)normalSign * m.bNx[b];
+  double n1 = (double)normalSign * m.bNy[b];
+  double n2 = (double)normalSign * m.bNz[b];
+  const double nmag = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  if (nmag <= 1.0e-300) return;
+  const double sc = 1.0 / nmag;
+  n0 = sc * n0;
+  n1 = sc * n1;
+  n2 = sc * n2;
+  const double dx0 = -m.bDx[b], dx1 = -m.bDy[b], dx2 = -m.bDz[b];  // cc_P - xf
+  double dn = dx0 * n0 + dx1 * n1 + dx2 * n2;
+  if (dn <= 1.0e-14) dn = fabs(dn);
+  if (dn <= 1.0e-14) dn = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+  if (dn <= 1.0e-14) return;
+  const double U0 = u[P], U1 = v[P], U2 = w[P];
+  const double Un = U0 * n0 + U1 * n1 + U2 * n2;
+  const double Ut0 = U0 - Un * n0, Ut1 = U1 - Un * n1, Ut2 = U2 - Un * n2;
+  const double Utmag = sqrt(Ut0 * Ut0 + Ut1 * Ut1 + Ut2 * Ut2);
+  const double nn[3] = {n0, n1, n2};
+  const double ut[3] = {Ut0, Ut1, Ut2};
+  for (int d = 0; d < 3; ++d) {
+    o[d] = -p[P] * nn[d] * A;
+    o[3 + d] = mu * ut[d] / dn * A;
+  }
+  o[6] = A;
+  o[7] = 1.0;
+  o[8] = dn;
+  o[9] = Utmag;
+  o[10] = mu * Utmag / dn;
+  o[11] = 1.0;
+}
+
+// ordered serial reduction of the per-face terms (one thread, patch-sized)
+__global__ void k_force_reduce(int nf, const double* __restrict__ t, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double Fp[3] = {0, 0, 0}, Fv[3] = {0, 0, 0}, F[3] = {0, 0, 0};
+  double area = 0.0, cnt = 0.0, minD = 1.0e300, maxD = 0.0, maxUt = 0.0, maxSh = 0.0;
+  for (int i = 0; i < nf; ++i) {
+    const double* o = t + 12 * (size_t)i;
+    if (o[11] == 0.0) continue;
+    for (int d = 0; d < 3; ++d) {
+      Fp[d] += o[d];
+      Fv[d] += o[3 + d];
+      F[d] += o[d] + o[3 + d];
+    }
+    area += o[6];
+    cnt += 1.0;
+    minD = fmin(minD, o[8]);
+    maxD = fmax(maxD, o[8]);
+    maxUt = fmax(maxUt, o[9]);
+    maxSh = fmax(maxSh, o[10]);
+  }
+  if (minD == 1.0e300) minD = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    out[d] = Fp[d];
+    out[3 + d] = Fv[d];
+    out[6 + d] = F[d];
+  }
+  out[9] = area;
+  out[10] = cnt;
+  out[11] = minD;
+  out[12] = maxD;
+  out[13] = maxUt;
+  out[14] = maxSh;
+}
+
+// --------------------------------------------------------------- driver --
+SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int* patchUType,
+                           const int* patchPType, const double* uFaceBC, const double* vFaceBC,
+                           const double* wFaceBC, const double* pFaceBC, int inletPatch)
+    : M(mesh), par(p_) {
+  if (!M.host.hasGeometry) throw Error(AN_EINVAL, "SIMPLE needs a mesh with geometry");
+  M.to_device(s);
+  mv = view_of(M);
+  const int nC = M.nC, nF = M.nF, nIF = M.nIF, nBF = M.nBF;
+  const HostMesh& hm = M.host;
+  const int nP = (int)hm.patchNames.size();
+  std::vector<int> tU(nBF > 0 ? nBF : 1), tP(nBF > 0 ? nBF : 1);
+  std::vector<double> vU[3], vP(nBF > 0 ? nBF : 1, 0.0);
+  const double* src[3] = {uFaceBC, vFaceBC, wFaceBC};
+  for (int k = 0; k < 3; ++k) vU[k].assign(nBF > 0 ? nBF : 1, 0.0);
+  for (int b = 0; b < nBF; ++b) {
+    const int f = nIF + b;
+    const int pt = hm.bPatch[f] - 1;
+    if (pt < 0 || pt >= nP) throw Error(AN_EINVAL, "boundary face without patch");
+    tU[b] = patchUType[pt] == 1 ? 1 : 0;
+    tP[b] = patchPType[pt] == 1 ? 1 : 0;
+    for (int k = 0; k < 3; ++k) vU[k][b] = src[k][f];
+    vP[b] = pFaceBC[f];
+  }
+  bcU.upload(tU, s);
+  bcP.upload(tP, s);
+  for (int k = 0; k < 3; ++k) bcUv[k].upload(vU[k], s);
+  bcPv.upload(vP, s);
+  // initial fields (main.cu:3302-3322): arithmetic mean of the inlet faces
+  double init[3] = {0.0, 0.0, 0.0};
+  if (inletPatch >= 0 && inletPatch < nP) {
+    int cnt = 0;
+    for (int f = nIF; f < nF; ++f)
+      if (hm.bPatch[f] - 1 == inletPatch) {
+        for (int k = 0; k < 3; ++k) init[k] += src[k][f];
+        ++cnt;
+      }
+    if (cnt > 0)
+      for (int k = 0; k < 3; ++k) init[k] /= cnt;
+  }
+  for (int k = 0; k < 3; ++k) {
+    std::vector<double> h(nC, init[k]);
+    q[k].upload(h, s);
+    qOld[k].alloc(nC);
+    gp[k].alloc(nC);
+    g[k].alloc(nC);
+  }
+  qNew.alloc(nC);
+  p.alloc(nC);
+  p.zero(s);
+  pOld.alloc(nC);
+  pCorr.alloc(nC);
+  pCorr.zero(s);
+  rAU.alloc(nC);
+  rAU.zero(s);
+  phiStar.alloc(nF);
+  phi.alloc(nF);
+  phiStar.zero(s);
+  phi.zero(s);
+  phiNo.alloc(nF);
+  divStar.alloc(nC);
+  divCorr.alloc(nC);
+  divCorr.zero(s);
+  rhs.alloc(nC);
+  Amom.n = nC;
+  Amom.nnz = M.addr.nnz;
+  Amom.rowOff.alloc((size_t)nC + 1);
+  Amom.cols.alloc((size_t)Amom.nnz);
+  Amom.vals.alloc((size_t)Amom.nnz);
+  Amom.vals.zero(s);
+  AN_HIP(hipMemcpyAsync(Amom.rowOff.p, M.addr.rowOff.p, sizeof(int) * ((size_t)nC + 1),
+                        hipMemcpyDeviceToDevice, s));
+  AN_HIP(hipMemcpyAsync(Amom.cols.p, M.addr.cols.p, sizeof(int) * (size_t)Amom.nnz,
+                        hipMemcpyDeviceToDevice, s));
+  Amom.finish_pattern(s);
+  pvals.alloc((size_t)Amom.nnz);
+  an_pcg_options po = par.pressure;
+  po.maxIter = par.pMaxit;
+  po.absTol = par.pTol;
+  po.relTol = par.pRelTol;
+  po.precision = par.precision;
+  psolver.reset(new PcgSolver(nC, M.addr.rowOff.p, M.addr.cols.p, M.addr.nnz, po, true, s));
+  red.init(1024, s);
+  scal.alloc(8);
+  for (auto& e : ev) AN_HIP(hipEventCreate(&e));
+  AN_HIP(hipStreamSynchronize(s));
+}
+
+SimpleSolver::~SimpleSolver() {
+  for (auto& e : ev) (void)hipEventDestroy(e);
+}
+
+double SimpleSolver::since(int a, int b) {
+  float ms = 0.f;
+  AN_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
+  return ms * 1e-3;
+}
+
+void SimpleSolver::grad(int bcKind, const double* f, DevBuf<double>* out) {
+  BCView bc = bcKind == 0 ? BCView{bcU.p, nullptr} : BCView{bcP.p, bcPv.p};
+  launch_lsq_gradient(mv, 0, bc, f, out[0].p, out[1].p, out[2].p, s);
+}
+
+double SimpleSolver::relchg(const double* a, const double* b) {
+  const int G = std::min(grid_for(M.nC, kB), 340);
+  k_stats<<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  k_stats_fin<<<1, 64, 0, s>>>(G, red.partial.p, scal.p);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  return std::sqrt(hs.h[0] / (1.0 + hs.h[1]));
+}
+
+double SimpleSolver::maxabs(const double* a) {
+  const int G = std::min(grid_for(M.nC, kB), 340);
+  k_stats<<<G, kB, 0, s>>>(M.nC, a, nullptr, red.partial.p);
+  k_stats_fin<<<1, 64, 0, s>>>(G, red.partial.p, scal.p);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  return hs.h[2];
+}
+
+// solve_scalar_component_with_nonorth (main.cu:3350-3384)
+void SimpleSolver::momentum_component(int comp, an_simple_stats* st) {
+  const int nC = M.nC;
+  const int nVel = std::max(par.nVelNonOrthCorr, 0) + 1;
+  // qIter = qOld ; the iterate lives in q[comp]
+  AN_HIP(hipMemcpyAsync(q[comp].p, qOld[comp].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+  for (int it = 1; it <= nVel; ++it) {
+    // velocity BC values of this component (types identical for u, v, w)
+    BCView bc{bcU.p, bcUv[comp].p};
+    launch_lsq_gradient(mv, 0, bc, q[comp].p, g[0].p, g[1].p, g[2].p, s);
+    MomArgs a;
+    a.bcT = bcU.p;
+    a.qB = bcUv[comp].p;
+    a.uB = bcUv[0].p;
+    a.vB = bcUv[1].p;
+    a.wB = bcUv[2].p;
+    a.gqx = g[0].p;
+    a.gqy = g[1].p;
+    a.gqz = g[2].p;
+    a.uC = qOld[0].p;
+    a.vC = qOld[1].p;
+    a.wC = qOld[2].p;
+    a.gpc = gp[comp].p;
+    a.qOld = q[comp].p;
+    a.rho = par.rho;
+    a.mu = par.mu;
+    a.corrPsi = 1.0;
+    a.uRelax = par.uRelax;
+    mark(0);
+    if (comp == 0 && it == 1)
+      k_mom<true><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs.p, rAU.p);
+    else
+      k_mom<false><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs.p, rAU.p);
+    AN_CHECK_LAUNCH();
+    mark(1);
+    AN_HIP(hipMemcpyAsync(qNew.p, q[comp].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    an_solve_info info{};
+    an_status stt = bicgstab_device(Amom, rhs.p, qNew.p, par.velMaxit, par.velRelTol, par.velTol,
+                                    bw, s, &info);
+    (void)stt;  // HYPRE BiCGSTAB non-convergence is not fatal in the reference
+    mark(2);
+    AN_HIP(hipStreamSynchronize(s));
+    tAcc[0] += since(0, 1);
+    tAcc[1] += since(1, 2);
+    st->bicgstabIts[comp] = info.iterations;
+    double rc = relchg(qNew.p, q[comp].p);
+    AN_HIP(hipMemcpyAsync(q[comp].p, qNew.p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    if (rc < par.corrTol) break;
+  }
+}
+
+an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut) {
+  const int nC = M.nC, nF = M.nF;
+  an_status status = AN_OK;
+  an_simple_stats st{};
+  for (int k = 0; k < n; ++k) {
+    auto tw0 = std::chrono::steady_clock::now();
+    ++step;
+    st = an_simple_stats{};
+    st.step = step;
+    for (int c = 0; c < 3; ++c)
+      AN_HIP(hipMemcpyAsync(qOld[c].p, q[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    AN_HIP(hipMemcpyAsync(pOld.p, p.p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    mark(3);
+    grad(1, pOld.p, gp);  // grad(pOld) [bcP] (main.cu:3393-3397)
+    mark(4);
+    for (int c = 0; c < 3; ++c) momentum_component(c, &st);
+    // pressure matrix + AMG (re)setup (main.cu:3407-3411)
+    mark(5);
+    k_p_asm<<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, rAU.p, pvals.p);
+    AN_CHECK_LAUNCH();
+    const int every = std::max(par.pAmgRebuildEvery, 1);
+    const bool setup = (step == 1) || (((step - 1) % every) == 0) || !psolver->amg.built;
+    psolver->set_values_device(pvals.p, setup);
+    mark(6);
+    AN_HIP(hipStreamSynchronize(s));
+    tAcc[2] += since(5, 6);
+    // predictor flux (grad(p) equals grad(pOld) here: p == pOld)
+    mark(0);
+    FaceArgs fa{bcU.p, bcP.p, bcUv[0].p, bcUv[1].p, bcUv[2].p, bcPv.p};
+    k_rc<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, q[0].p, q[1].p, q[2].p, p.p, gp[0].p, gp[1].p,
+                                         gp[2].p, rAU.p, par.rho, phiStar.p, phi.p);
+    k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phi.p, nullptr, divStar.p);
+    AN_CHECK_LAUNCH();
+    fill_zero(pCorr.p, sizeof(double) * nC, s);
+    const int nPass = std::max(par.nNonOrthCorr, 0) + 1;
+    int pcgTotal = 0, pcgLast = 0;
+    double tP = 0.0;
+    for (int pass = 1; pass <= nPass; ++pass) {
+      grad(1, pCorr.p, g);
+      k_nonorth_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p, phiNo.p);
+      k_div<1><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phiNo.p, divStar.p, rhs.p);
+      AN_CHECK_LAUNCH();
+      psolver->opt.relTol = (pass == nPass) ? 0.0 : par.pRelTol;
+      an_solve_info info{};
+      an_status ps = psolver->solve(rhs.p, pCorr.p, &info);
+      if (ps == AN_ENOCONV)
+        std::fprintf(stderr,
+                     "WARNING: pressure PCG non-convergence. its=%d finalRel=%.6e. "
+                     "Continuing with approximate pCorr.\n",
+                     info.iterations, info.finalRelResNorm);
+      tP += info.solveSeconds;
+      pcgLast = info.iterations;
+      pcgTotal += info.iterations;
+      if (pass == nPass) {
+        grad(1, pCorr.p, g);
+        k_nonorth_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p,
+                                                       phiNo.p);
+        k_correct_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, phiStar.p, pCorr.p, phiNo.p,
+                                                       phi.p);
+        k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phi.p, nullptr, divCorr.p);
+        AN_CHECK_LAUNCH();
+      }
+    }
+    tAcc[3] += tP;
+    st.massRes = maxabs(divCorr.p);
+    k_p_update<<<grid_for(nC, kB), kB, 0, s>>>(nC, p.p, pCorr.p, par.pRelax);
+    grad(1, pCorr.p, g);
+    k_vel_correct<<<grid_for(nC, kB), kB, 0, s>>>(nC, q[0].p, q[1].p, q[2].p, rAU.p, g[0].p,
+                                                  g[1].p, g[2].p);
+    AN_CHECK_LAUNCH();
+    st.duRel = relchg(q[0].p, qOld[0].p);
+    st.dvRel = relchg(q[1].p, qOld[1].p);
+    st.dwRel = relchg(q[2].p, qOld[2].p);
+    st.dpRel = relchg(p.p, pOld.p);
+    st.pcgLast = pcgLast;
+    st.pcgTotal = pcgTotal;
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw0).count();
+    tAcc[5] += st.seconds;
+    if (!std::isfinite(st.massRes) || st.massRes > 1e50) {
+      status = AN_EDIVERGED;
+      break;
+    }
+    if (st.massRes < par.tolMass && std::max({st.duRel, st.dvRel, st.dwRel}) < par.tolVel) {
+      converged = true;
+      break;
+    }
+  }
+  tAcc[4] = tAcc[5] - tAcc[0] - tAcc[1] - tAcc[2] - tAcc[3];
+  if (last) *last = st;
+  if (convergedOut) *convergedOut = converged ? 1 : 0;
+  return status;
+}
+
+void SimpleSolver::get_field(const std::string& name, double* out, bool device) {
+  const DevBuf<double>* b = nullptr;
+  size_t n = (size_t)M.nC;
+  if (name == "u") b = &q[0];
+  else if (name == "v") b = &q[1];
+  else if (name == "w") b = &q[2];
+  else if (name == "p") b = &p;
+  else if (name == "pCorr") b = &pCorr;
+  else if (name == "rAU") b = &rAU;
+  else if (name == "div") b = &divCorr;
+  else if (name == "phi") { b = &phi; n = (size_t)M.nF; }
+  else throw Error(AN_EINVAL, "unknown field '" + name + "'");
+  AN_HIP(hipMemcpy(out, b->p, sizeof(double) * n,
+                   device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+}
+
+void SimpleSolver::set_field(const std::string& name, const double* in, bool device) {
+  DevBuf<double>* b = nullptr;
+  if (name == "u") b = &q[0];
+  else if (name == "v") b = &q[1];
+  else if (name == "w") b = &q[2];
+  else if (name == "p") b = &p;
+  else throw Error(AN_EINVAL, "field '" + name + "' cannot be set");
+  AN_HIP(hipMemcpy(b->p, in, sizeof(double) * M.nC,
+                   device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+}
+
+void SimpleSolver::patch_forces(int patch, int normalSign, double* out) {
+  const HostMesh& hm = M.host;
+  if (patch < 0 || patch >= (int)hm.patchNames.size()) {
+    for (int i = 0; i < 15; ++i) out[i] = 0.0;
+    throw Error(AN_EINVAL, "patch index out of range");
+  }
+  const int f0 = hm.patchStart[patch], nf = hm.patchNFaces[patch];
+  DevBuf<double> t(12 * (size_t)(nf > 0 ? nf : 1)), o(15);
+  if (nf > 0) {
+    k_force_faces<<<grid_for(nf, kB), kB, 0, s>>>(mv, f0, nf, q[0].p, q[1].p, q[2].p, p.p,
+                                                  par.mu, normalSign, t.p);
+    AN_CHECK_LAUNCH();
+  }
+  k_force_reduce<<<1, 64, 0, s>>>(nf, t.p, o.p);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipMemcpyAsync(out, o.p, 15 * sizeof(double), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+}
+
+void SimpleSolver::timings(double* out) {
+  for (int i = 0; i < 6; ++i) out[i] = tAcc[i];
+}
+
+}  // namespace an

@@ -1,0 +1,58 @@
+// simple.hpp -- device-resident SIMPLE outer iteration (generic_simple_v1).
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "kernels.hpp"
+#include "solver.hpp"
+
+namespace an {
+
+class SimpleSolver {
+ public:
+  SimpleSolver(DevMesh& mesh, const an_simple_params& par, const int* patchUType,
+               const int* patchPType, const double* uFaceBC, const double* vFaceBC,
+               const double* wFaceBC, const double* pFaceBC, int inletPatch);
+  ~SimpleSolver();
+  // Runs n outer iterations; returns AN_EDIVERGED on divergence.
+  an_status iterate(int n, an_simple_stats* last, int* converged);
+  void get_field(const std::string& name, double* out, bool device);
+  void set_field(const std::string& name, const double* in, bool device);
+  void patch_forces(int patch, int normalSign, double* out);
+  void timings(double* out);
+
+  DevMesh& M;
+  MeshView mv;
+  an_simple_params par;
+  hipStream_t s = 0;
+  int step = 0;
+  bool converged = false;
+
+ private:
+  // boundary data per boundary face
+  DevBuf<int> bcU, bcP;
+  DevBuf<double> bcUv[3], bcPv;
+  // fields
+  DevBuf<double> q[3], qOld[3], qNew, p, pOld, pCorr, rAU;
+  DevBuf<double> gp[3], g[3];
+  DevBuf<double> phiStar, phi, phiNo, divStar, divCorr, rhs;
+  DevCSR<double> Amom;
+  DevBuf<double> pvals;
+  std::unique_ptr<PcgSolver> psolver;
+  BicgWork bw;
+  Reducer red;
+  DevBuf<double> scal;
+  PinnedScalars hs;
+  double tAcc[6] = {0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[8];
+
+  void grad(int bcKind, const double* f, DevBuf<double>* out);
+  void momentum_component(int comp, an_simple_stats* st);
+  double relchg(const double* a, const double* b);  // device reduction
+  double maxabs(const double* a);
+  void mark(int i) { AN_HIP(hipEventRecord(ev[i], s)); }
+  double since(int a, int b);
+};
+
+}  // namespace an

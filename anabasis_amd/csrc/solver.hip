@@ -46,7 +46,8 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
   hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   AN_HIP(hipMemcpyAsync(A.rowOff.p, rowOff, sizeof(int) * ((size_t)n + 1), k, s));
   AN_HIP(hipMemcpyAsync(A.cols.p, cols, sizeof(int) * (size_t)nnz, k, s));
-  red.init(grid_for(n, kB) + 1024, s);
+  A.finish_pattern(s);
+  red.init(std::max(grid_for(A.rb.count, kB / 64), grid_for(n, kB)) + 1024, s);
   scal.alloc(S_COUNT);
   scal.zero(s);
   r.alloc(n);
@@ -271,10 +272,9 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
 }
 
 // ---------------------------------------------------------- BiCGSTAB ---
-void BicgWork::ensure(int nn, hipStream_t st) {
-  if (n == nn && scal.p) return;
+void BicgWork::ensure(int nn, int nbSpmv, hipStream_t st) {
+  if (n == nn && scal.p && red.cap >= nbSpmv + 16) return;
   n = nn;
-  const int nbSpmv = grid_for(n, kB);
   red.init(nbSpmv > 2048 ? nbSpmv + 16 : 2048, st);
   scal.alloc(S_COUNT);
   dinv.alloc(n);
@@ -300,32 +300,14 @@ __device__ double bsum(double v, double* sh) {
   return t;
 }
 
-// two reductions sharing one completion ticket
-__device__ void finish2(double a, double b, int nb, double* __restrict__ partial,
-                        unsigned int* __restrict__ counter, double* __restrict__ outA,
-                        double* __restrict__ outB, double* sh) {
-  __shared__ int last;
+// block partials of one or two sums; finalize() reduces them in index order
+__device__ __forceinline__ void store1(double a, double* __restrict__ partial) {
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+__device__ __forceinline__ void store2(double a, double b, double* __restrict__ partial) {
   if (threadIdx.x == 0) {
     partial[2 * blockIdx.x] = a;
     partial[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    last = (atomicAdd(counter, 1u) == (unsigned)nb - 1u);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double* vp = partial;
-  double sa = 0.0, sb = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    sa += vp[2 * i];
-    sb += vp[2 * i + 1];
-  }
-  sa = bsum(sa, sh);
-  sb = bsum(sb, sh);
-  if (threadIdx.x == 0) {
-    if (outA) *outA = sa;
-    if (outB) *outB = sb;
-    *counter = 0u;
   }
 }
 
@@ -361,8 +343,7 @@ __global__ __launch_bounds__(kB) void k_bicg_s(int n, const double* __restrict__
                                                const double* __restrict__ dinv,
                                                double* __restrict__ s_, double* __restrict__ sh,
                                                double* __restrict__ sc, int rhoNew,
-                                               double* __restrict__ partial,
-                                               unsigned int* __restrict__ counter) {
+                                               double* __restrict__ partial) {
   __shared__ double shm[kB / 64];
   const double alpha = sc[rhoNew] / sc[S_R0V];
   double acc = 0.0;
@@ -374,14 +355,12 @@ __global__ __launch_bounds__(kB) void k_bicg_s(int n, const double* __restrict__
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_AL] = alpha;  // read by later kernels only
   double b = bsum(acc, shm);
-  finish2(b, 0.0, gridDim.x, partial, counter, sc + S_SS, nullptr, shm);
+  store1(b, partial);
 }
 
 __global__ __launch_bounds__(kB) void k_dot2(int n, const double* __restrict__ t,
                                              const double* __restrict__ s_,
-                                             double* __restrict__ sc,
-                                             double* __restrict__ partial,
-                                             unsigned int* __restrict__ counter) {
+                                             double* __restrict__ partial) {
   __shared__ double shm[kB / 64];
   double a = 0.0, b = 0.0;
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
@@ -390,7 +369,7 @@ __global__ __launch_bounds__(kB) void k_dot2(int n, const double* __restrict__ t
   }
   double ba = bsum(a, shm);
   double bb = bsum(b, shm);
-  finish2(ba, bb, gridDim.x, partial, counter, sc + S_TS, sc + S_TT, shm);
+  store2(ba, bb, partial);
 }
 
 // omega = ts/tt; if ||s|| <= tol: x += alpha ph, r = s; else x += alpha ph +
@@ -402,8 +381,7 @@ __global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
                                                const double* __restrict__ s_,
                                                const double* __restrict__ t,
                                                double* __restrict__ sc, double tol,
-                                               double* __restrict__ partial,
-                                               unsigned int* __restrict__ counter) {
+                                               double* __restrict__ partial) {
   __shared__ double shm[kB / 64];
   const double alpha = sc[S_AL];
   const double tt = sc[S_TT];
@@ -424,7 +402,7 @@ __global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_OM] = omega;
   double b = bsum(acc, shm);
-  finish2(b, 0.0, gridDim.x, partial, counter, sc + S_RR, nullptr, shm);
+  store1(b, partial);
 }
 
 an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,
@@ -432,7 +410,7 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
                           an_solve_info* info) {
   const int n = A.n;
   auto t0 = std::chrono::steady_clock::now();
-  w.ensure(n, s);
+  w.ensure(n, grid_for(A.rb.count, kB / 64), s);
   const int G = std::min(grid_for(n, kB), 1024);
   an_solve_info inf{};
   k_dinv<<<grid_for(n, kB), kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
@@ -465,13 +443,17 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
       AN_CHECK_LAUNCH();
       spmv<double, double>(A, w.ph.p, w.v.p, w.r0.p, w.scal.p, S_R0V, w.red, s);
       k_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, w.scal.p, rhoNew,
-                                w.red.partial.p, w.red.counter.p);
+                                w.red.partial.p);
       AN_CHECK_LAUNCH();
+      finalize(w.red, G, 1, w.scal.p + S_SS, s);
       spmv<double, double>(A, w.sh.p, w.t.p, nullptr, w.scal.p, 0, w.red, s);
-      k_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, w.scal.p, w.red.partial.p, w.red.counter.p);
-      k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, w.scal.p, tol,
-                                w.red.partial.p, w.red.counter.p);
+      k_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, w.red.partial.p);
       AN_CHECK_LAUNCH();
+      finalize(w.red, G, 2, w.scal.p + S_TS, s);  // S_TS, S_TT
+      k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, w.scal.p, tol,
+                                w.red.partial.p);
+      AN_CHECK_LAUNCH();
+      finalize(w.red, G, 1, w.scal.p + S_RR, s);
       AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
       AN_HIP(hipStreamSynchronize(s));
       const double* h = w.hs.h;

@@ -60,7 +60,7 @@ struct BicgWork {
   DevBuf<double> scal, dinv, r, r0, p, v, ph, s_, sh, t;
   PinnedScalars hs;
   int n = 0;
-  void ensure(int n, hipStream_t s);
+  void ensure(int n, int nbSpmv, hipStream_t s);
 };
 an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,
                           double relTol, double absTol, BicgWork& w, hipStream_t s,

@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X SIMPLE hot path (one GPU).
 
-Workloads
-  psolve  pressure-equation solve to ||r||/||b|| <= 1e-10 on a 128^3 hex box
-          with interior vertices jittered by +-20% h (seed 20261203),
-          homogeneous Neumann everywhere except fixed_value 0 on x = 0,
-          b ~ N(0,1) projected to zero mean, x0 = 0; FP64 PCG+AMG and
-          FP32-inner / FP64-refined.  One step = one solve.
+Headline ("value"): SIMPLE outer iterations per second on the cylinder-channel
+geometry of cases/cylinder.case (2.2 x 0.41 x 0.41 channel, D = 0.1 cylinder
+at (0.5, 0.2) along z, O-grid + H-block hexes) refined to about 2.0 M cells,
+with that case's controls (nVelNonOrthCorr 1, nNonOrthCorr 10, uRelax 0.5,
+pRelax 0.3, velTol = pTol = 1e-10, AMG rebuild every 10), parabolic box inlet
+H = 0.41 Umax = 0.45, outlet p = 0, no-slip walls, nu = 1e-3.  One step = one
+outer iteration; warmup iterations (including the first AMG setup) are not
+timed.  FP64 throughout.
 
-The dominant kernel's roofline is measured live: every Krylov SpMV of the
-timed solves is bracketed by HIP events on the solver's stream; algorithmic
-bytes per SpMV = 12 nnz + 4 (nC+1) + 16 nC (FP64 values + int32 columns,
-row offsets, x read once, y written once).
+Also reported: the same loop with the mixed-precision pressure solve, and the
+pressure-solve metric -- 128^3 hex box with interior vertices jittered by
++-20% h (seed 20261203), Neumann everywhere except fixed_value 0 on x = 0,
+b ~ N(0,1) mean-projected, x0 = 0, solved to ||r||/||b|| <= 1e-10 with FP64
+PCG+AMG and with FP32-inner / FP64-refined PCG+AMG.
+
+Roofline: every Krylov SpMV of the timed FP64 pressure solves is bracketed by
+HIP events on the solver stream; algorithmic bytes per launch =
+12 nnz + 4 (nC+1) + 16 nC.
 """
 import argparse
 import json
@@ -28,6 +35,66 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261203
 
 
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------- SIMPLE ----
+def simple_case(cells):
+    from anabasis_amd import libpoisson as lp
+    from anabasis_amd import simple as sm
+    t0 = time.time()
+    m = lp.Mesh.cylinder(cells)
+    vel, prs = sm.cylinder_case_bcs()
+    sm.coverage_table(m, vel, prs)
+    log(f"cylinder mesh: {m.nCells} cells, {m.nFaces} faces, maxNonOrth "
+        f"{m.maxNonOrthDeg:.2f} deg ({time.time() - t0:.1f}s)")
+    return m, vel, prs
+
+
+def run_simple(m, vel, prs, precision, warmup, steps):
+    from anabasis_amd import simple as sm
+    par = sm.default_params(precision=precision)
+    S = sm.Simple(m, par, vel, prs)
+    t0 = time.time()
+    st, _ = S.iterate(warmup)
+    log(f"SIMPLE prec={precision} warmup {warmup} its {time.time() - t0:.2f}s "
+        f"massRes {st.massRes:.3e}")
+    t0 = time.time()
+    st, conv = S.iterate(steps)
+    dt = time.time() - t0
+    tm = S.timings()
+    force = S.patch_forces("patch_3_0", -1)
+    log(f"SIMPLE prec={precision}: {steps} its {dt:.2f}s -> {steps / dt:.3f} it/s; last massRes "
+        f"{st.massRes:.3e} bicgstab {list(st.bicgstabIts)} pcgTotal {st.pcgTotal}")
+    return dict(iters_per_s=steps / dt, seconds=dt, massRes=st.massRes,
+                bicgstab_its=list(st.bicgstabIts), pcg_total=st.pcgTotal, converged=conv,
+                phase_seconds_cumulative=tm, force_cylinder=force[6:9].tolist(),
+                fields={k: S.field(k) for k in "uvwp"})
+
+
+def cpu_baseline_simple(m, vel, prs, iters):
+    from anabasis_amd import simple as sm
+    from oracle import pyoracle as O
+    par = sm.default_params()
+    om = O.OMesh.from_product(m)
+    tU, tP, ub, vb, wb, pb = sm.apply_bc_specs(m, vel, prs)
+    k = m.patch_index("patch_1_0")
+    s, n = m.patchStartFace[k], m.patchNFaces[k]
+    u0 = np.full(m.nCells, ub[s:s + n].mean())
+    v0 = np.full(m.nCells, vb[s:s + n].mean())
+    w0 = np.full(m.nCells, wb[s:s + n].mean())
+    t0 = time.time()
+    ref = O.simple_run(om, O.simple_params_from(par), tU, tP, ub, vb, wb, pb, iters, u0, v0, w0,
+                       np.zeros(m.nCells))
+    dt = time.time() - t0
+    log(f"CPU oracle SIMPLE: {iters} its {dt:.1f}s")
+    return {"value": iters / dt, "unit": "iter/s", "cores": 1, "kind": "port",
+            "sample": f"first {iters} outer iteration(s) of the same {m.nCells}-cell cylinder "
+                      "case (incl. the first AMG setup) by the serial C oracle"}
+
+
+# ------------------------------------------------------------- psolve ----
 def psolve_problem(n):
     from anabasis_amd import libpoisson as lp
     m = lp.Mesh.hexbox(n, n, n, 1.0, 1.0, 1.0, jitter=0.2, seed=SEED)
@@ -43,15 +110,13 @@ def psolve_problem(n):
     return m, pat, vals, b
 
 
-def run_psolve(args):
-    from anabasis_amd import _lib as L
+def run_psolve(n, reps):
     from anabasis_amd import libpoisson as lp
-    L.check(L.lib().an_set_device(0))
-    n = args.n
+    import scipy.sparse as sp
     t0 = time.time()
     m, pat, vals, b = psolve_problem(n)
-    print(f"# mesh {n}^3: nC={m.nCells} nnz={pat.nnz} build {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    log(f"psolve mesh {n}^3: nC={m.nCells} nnz={pat.nnz} ({time.time() - t0:.1f}s)")
+    A = sp.csr_matrix((vals, pat.cols, pat.rowOffsets))
     res = {}
     for prec, name in ((0, "fp64"), (1, "mixed")):
         S = lp.PCGSolver(pat.rowOffsets, pat.cols, relTol=1e-10, absTol=0.0, precision=prec,
@@ -59,92 +124,104 @@ def run_psolve(args):
         ts = time.time()
         S.set_values(vals, rebuild=True)
         setup = time.time() - ts
-        for _ in range(args.warmup):
-            S.solve(b)
+        S.solve(b)  # warm
         times, infos = [], []
-        for _ in range(args.steps):
-            t1 = time.time()
+        for _ in range(reps):
             x, info = S.solve(b)
-            times.append(time.time() - t1)
+            times.append(info.solveSeconds)
             infos.append(info)
-        r = b - sp_mv(pat, vals, x)
-        res[name] = dict(setup_s=setup, solve_s=float(np.mean(times)),
-                         iterations=infos[-1].iterations,
-                         rel_res=float(np.linalg.norm(r) / np.linalg.norm(b)),
-                         refinement_steps=infos[-1].refinementSteps,
+        relres = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+        res[name] = dict(setup_s=setup, solve_s=float(np.median(times)),
+                         iterations=infos[-1].iterations, rel_res=relres,
+                         refinement_steps=infos[-1].refinementSteps, levels=infos[-1].levels,
                          spmv_avg_s=float(np.mean([i.spmvAvgSeconds for i in infos])),
-                         vcycle_avg_s=float(np.mean([i.vcycleAvgSeconds for i in infos])),
-                         x=x)
-        print(f"# {name}: setup {setup:.3f}s solve {res[name]['solve_s']:.4f}s "
-              f"its {res[name]['iterations']} relres {res[name]['rel_res']:.2e} "
-              f"spmv {res[name]['spmv_avg_s'] * 1e6:.1f}us vcycle "
-              f"{res[name]['vcycle_avg_s'] * 1e6:.1f}us", file=sys.stderr, flush=True)
+                         spmv_calls=int(sum(i.spmvCalls for i in infos)),
+                         vcycle_avg_s=float(np.mean([i.vcycleAvgSeconds for i in infos])), x=x)
+        r = res[name]
+        log(f"psolve {name}: setup {setup:.3f}s solve {r['solve_s']:.4f}s its {r['iterations']} "
+            f"relres {relres:.2e} spmv {r['spmv_avg_s'] * 1e6:.1f}us vcycle "
+            f"{r['vcycle_avg_s'] * 1e6:.1f}us")
         del S
-    nC, nnz = m.nCells, pat.nnz
-    alg = 12 * nnz + 4 * (nC + 1) + 16 * nC
-    t_sp = res["fp64"]["spmv_avg_s"]
-    achieved = alg / t_sp / 1e9
-    err_mixed = float(np.linalg.norm(res["mixed"]["x"] - res["fp64"]["x"]) /
-                      np.linalg.norm(res["fp64"]["x"]))
-    cpu = None if args.no_cpu_baseline else cpu_baseline_psolve(pat, vals, b, args)
-    out = {
-        "metric": "pressure solve time to 1e-10 relative residual (FP64 PCG+AMG), 128^3 "
-                  "jittered hex",
-        "value": res["fp64"]["solve_s"],
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": res["fp64"]["solve_s"] * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"pressure solve, {n}^3 hex box jitter 0.2, Neumann + "
-                               f"fixed_value x=0, b~N(0,1) mean-projected, rel tol 1e-10",
-                   "cells": nC, "nnz": nnz},
-        "psolve": {k: {kk: vv for kk, vv in v.items() if kk != "x"} for k, v in res.items()},
-        "mixed_vs_fp64_rel_diff": err_mixed,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<double,double,double,0,1> (Krylov SpMV + fused p.q)",
-                     "alg_bytes_per_launch": alg},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
-
-
-def sp_mv(pat, vals, x):
-    import scipy.sparse as sp
-    return sp.csr_matrix((vals, pat.cols, pat.rowOffsets)) @ x
-
-
-def cpu_baseline_psolve(pat, vals, b, args):
-    from oracle import pyoracle as O
-    t0 = time.time()
-    x, it, rr, _ = O.pcg_amg(pat.rowOffsets, pat.cols, vals, b, relTol=1e-10, absTol=0.0)
-    dt = time.time() - t0
-    print(f"# cpu oracle: {dt:.1f}s its {it}", file=sys.stderr, flush=True)
-    return {"value": dt, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"full solve (setup+solve) of the same {pat.nRows}-cell system by the "
-                      "serial C oracle (same PCG + aggregation AMG)"}
+    res["mixed_vs_fp64_rel_diff"] = float(np.linalg.norm(res["mixed"]["x"] - res["fp64"]["x"]) /
+                                          np.linalg.norm(res["fp64"]["x"]))
+    for k in ("fp64", "mixed"):
+        res[k].pop("x")
+    return m, pat, res
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="psolve", choices=["psolve"])
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=2_000_000)
+    ap.add_argument("--pn", type=int, default=128)
+    ap.add_argument("--psolve-reps", type=int, default=3)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-psolve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
-    run_psolve(args)
+    from anabasis_amd import _lib as L
+    L.check(L.lib().an_set_device(0))
+
+    m, vel, prs = simple_case(args.cells)
+    r64 = run_simple(m, vel, prs, 0, args.warmup, args.steps)
+    mixed = None
+    if not args.no_mixed:
+        rmx = run_simple(m, vel, prs, 1, args.warmup, args.steps)
+        f64s = r64.pop("fields")
+        fmx = rmx.pop("fields")
+        mixed = dict(iters_per_s=rmx["iters_per_s"], massRes=rmx["massRes"],
+                     pcg_total_last=rmx["pcg_total"],
+                     rel_diff_vs_fp64={k: float(np.linalg.norm(fmx[k] - f64s[k]) /
+                                                max(np.linalg.norm(f64s[k]), 1e-300))
+                                       for k in "uvwp"},
+                     force_cylinder=rmx["force_cylinder"])
+    else:
+        r64.pop("fields")
+    ps = None
+    roof = None
+    if not args.no_psolve:
+        pm, pat, ps = run_psolve(args.pn, args.psolve_reps)
+        nC, nnz = pm.nCells, pat.nnz
+        alg = 12 * nnz + 4 * (nC + 1) + 16 * nC
+        t = ps["fp64"]["spmv_avg_s"]
+        achieved = alg / t / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_spmv<double,double,double,0,1> (PCG SpMV + fused p.q), "
+                          f"{args.pn}^3 pressure matrix",
+                "alg_bytes_per_launch": alg, "avg_launch_s": t,
+                "launches_timed": ps["fp64"]["spmv_calls"]}
+    cpu = None if args.no_cpu_baseline else cpu_baseline_simple(m, vel, prs, args.cpu_iters)
+    out = {
+        "metric": "SIMPLE outer iterations/s (FP64), cylinder channel ~2M cells",
+        "value": r64["iters_per_s"],
+        "unit": "iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / r64["iters_per_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated mesh, cases/cylinder.case controls and BCs)",
+        "config": {"workload": f"generic_simple_v1 cylinder channel, {m.nCells} hex cells, "
+                               "nVelNonOrthCorr 1, nNonOrthCorr 10, AMG rebuild every 10",
+                   "cells": m.nCells, "faces": m.nFaces},
+        "simple_fp64": r64,
+        "simple_mixed": mixed,
+        "psolve": ps,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

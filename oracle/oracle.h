@@ -238,6 +238,15 @@ int or_amg_vcycle(int n, const int *rowOffsets, const int *cols,
                   const double *vals, const or_amg_options *opt,
                   const double *r, double *z);
 
+/* Reusable PCG + AMG: hierarchy kept between solves; set_values with
+ * rebuild == 0 refreshes only the level-0 smoother (stale coarse levels). */
+void *or_pcg_create(int n, const int *rowOffsets, const int *cols,
+                    const or_amg_options *opt);
+void or_pcg_set_values(void *h, const double *vals, int rebuild);
+int or_pcg_solve(void *h, const double *b, double *x, double relTol,
+                 double absTol, int maxIter, double *finalRelRes);
+void or_pcg_destroy(void *h);
+
 /* Jacobi (DiagScale) preconditioned BiCGSTAB, HYPRE stop rule
  * ||r|| <= max(relTol*||b||, absTol).  Returns iterations (negative if not
  * converged). */

@@ -203,3 +203,52 @@ def bicgstab(ro, ci, va, b, x0=None, maxIter=1000, relTol=0.0, absTol=1e-10):
     it = lib().or_bicgstab_jacobi(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), _p(_d(b)), _p(x),
                                   maxIter, C.c_double(relTol), C.c_double(absTol), C.byref(rr))
     return x, it, rr.value
+
+
+class SimpleParams(C.Structure):
+    _fields_ = [("rho", C.c_double), ("mu", C.c_double), ("nVelNonOrthCorr", C.c_int),
+                ("nNonOrthCorr", C.c_int), ("uRelax", C.c_double), ("pRelax", C.c_double),
+                ("corrTol", C.c_double), ("tolMass", C.c_double), ("tolVel", C.c_double),
+                ("velMaxit", C.c_int), ("velTol", C.c_double), ("velRelTol", C.c_double),
+                ("pMaxit", C.c_int), ("pTol", C.c_double), ("pRelTol", C.c_double),
+                ("pAmgRebuildEvery", C.c_int), ("amg", AmgOptions)]
+
+
+class SimpleStats(C.Structure):
+    _fields_ = [("massRes", C.c_double), ("duRel", C.c_double), ("dvRel", C.c_double),
+                ("dwRel", C.c_double), ("dpRel", C.c_double), ("its", C.c_int * 3),
+                ("pcgLast", C.c_int), ("pcgTotal", C.c_int)]
+
+
+def simple_params_from(gp):
+    """Copies an anabasis_amd SimpleParams (ctypes) into the oracle layout."""
+    pr = gp.pressure
+    amg = amg_options(maxIter=gp.pMaxit, relTol=gp.pRelTol, absTol=gp.pTol,
+                      coarseSize=pr.coarseSize, maxLevels=pr.maxLevels, strength=pr.strength,
+                      pairPasses=pr.pairPasses, jacobiOmega=pr.jacobiOmega)
+    return SimpleParams(gp.rho, gp.mu, gp.nVelNonOrthCorr, gp.nNonOrthCorr, gp.uRelax,
+                        gp.pRelax, gp.corrTol, gp.tolMass, gp.tolVel, gp.velMaxit, gp.velTol,
+                        gp.velRelTol, gp.pMaxit, gp.pTol, gp.pRelTol, gp.pAmgRebuildEvery, amg)
+
+
+def simple_run(m: OMesh, par: SimpleParams, tU, tP, ub, vb, wb, pb, n_iter, u, v, w, p):
+    lib().or_simple_run.restype = C.c_int
+    u, v, w, p = (_d(a).copy() for a in (u, v, w, p))
+    phi = np.empty(m.nFaces)
+    rau = np.empty(m.nCells)
+    stats = (SimpleStats * max(n_iter, 1))()
+    done = lib().or_simple_run(
+        m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh), _p(m.bPatch),
+        _p(m.rowOffsets), _p(m.cols), _p(m.diagPos), _p(m.PP), _p(m.PN), _p(m.NP), _p(m.NN),
+        _p(m.cc), _p(m.vol), _p(m.xf), _p(m.nf), _p(m.Sf), _p(m.Af), C.byref(par),
+        _p(_i(tU)), _p(_i(tP)), _p(_d(ub)), _p(_d(vb)), _p(_d(wb)), _p(_d(pb)), n_iter,
+        _p(u), _p(v), _p(w), _p(p), _p(phi), _p(rau), stats)
+    return dict(u=u, v=v, w=w, p=p, phi=phi, rAU=rau, stats=list(stats)[:abs(done)], done=done)
+
+
+def patch_forces(m: OMesh, f0, f1, u, v, w, p, mu, sign):
+    out = np.empty(15)
+    lib().or_patch_forces(_p(m.owner), _p(m.cc), _p(m.xf), _p(m.nf), _p(m.Af), f0, f1,
+                          _p(_d(u)), _p(_d(v)), _p(_d(w)), _p(_d(p)), C.c_double(mu), sign,
+                          _p(out))
+    return out

@@ -450,10 +450,18 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
   double *r = H->rb[l];
   if (L->isCoarsest) {
     if (L->coarseInv) {
+      /* x = Ainv b summed in 64-column chunks: each chunk's partial sum in
+         column order, then the chunk partials in chunk order (the blocked
+         order the GPU coarse solve uses) */
       for (int i = 0; i < L->n; ++i) {
         double s = 0.0;
         const double *row = L->coarseInv + (size_t)i * L->n;
-        for (int j = 0; j < L->n; ++j) s += row[j] * b[j];
+        for (int c = 0; c < L->n; c += 64) {
+          double t = 0.0;
+          const int e = c + 64 < L->n ? c + 64 : L->n;
+          for (int j = c; j < e; ++j) t += row[j] * b[j];
+          s += t;
+        }
         x[i] = s;
       }
     } else {
@@ -666,4 +674,110 @@ int or_amg_vcycle(int n, const int *ro, const int *ci, const double *va,
   int nl = H.nLevels;
   amg_free(&H);
   return nl;
+}
+
+/* ---------------------------------------------- reusable PCG + AMG ---- */
+/* Hierarchy kept between solves, as the reference keeps its BoomerAMG setup
+ * between pAmgRebuildEvery rebuilds (GSV1 main.cu:3407-3411); new fine
+ * values refresh the level-0 smoother only. */
+typedef struct {
+  or_amg H;
+  or_amg_options opt;
+  int n;
+  const int *ro, *ci;
+  const double *va;
+  int built;
+} or_pcg_handle;
+
+void *or_pcg_create(int n, const int *ro, const int *ci,
+                    const or_amg_options *opt) {
+  or_pcg_handle *h = (or_pcg_handle *)calloc(1, sizeof(or_pcg_handle));
+  h->n = n;
+  h->ro = ro;
+  h->ci = ci;
+  h->opt = *opt;
+  return h;
+}
+
+void or_pcg_set_values(void *hv, const double *va, int rebuild) {
+  or_pcg_handle *h = (or_pcg_handle *)hv;
+  h->va = va;
+  if (rebuild || !h->built) {
+    if (h->built) amg_free(&h->H);
+    amg_setup(&h->H, h->n, h->ro, h->ci, va, &h->opt);
+    h->built = 1;
+  } else {
+    or_level *L = &h->H.lv[0];
+    L->va = (double *)va;
+    free(L->diag);
+    free(L->omdinv);
+    level_diag(L, h->opt.jacobiOmega);
+  }
+}
+
+int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
+                 double absTol, int maxIter, double *finalRelRes) {
+  or_pcg_handle *h = (or_pcg_handle *)hv;
+  int n = h->n;
+  const int *ro = h->ro, *ci = h->ci;
+  const double *va = h->va;
+  double *r = (double *)malloc(sizeof(double) * (size_t)n);
+  double *z = (double *)malloc(sizeof(double) * (size_t)n);
+  double *p = (double *)malloc(sizeof(double) * (size_t)n);
+  double *q = (double *)malloc(sizeof(double) * (size_t)n);
+  double bnorm = sqrt(dotv(n, b, b));
+  int it = 0, conv = 0;
+  double rnorm = 0.0, tol = 0.0, rho = 0.0;
+  if (bnorm == 0.0) {
+    memset(x, 0, sizeof(double) * (size_t)n);
+    if (finalRelRes) *finalRelRes = 0.0;
+    conv = 1;
+    goto done;
+  }
+  or_spmv(n, ro, ci, va, x, q);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  rnorm = sqrt(dotv(n, r, r));
+  tol = fmax(relTol * bnorm, absTol);
+  if (rnorm <= tol) {
+    conv = 1;
+    if (finalRelRes) *finalRelRes = rnorm / bnorm;
+    goto done;
+  }
+  vcycle(&h->H, 0, r, z);
+  memcpy(p, z, sizeof(double) * (size_t)n);
+  rho = dotv(n, r, z);
+  for (it = 1; it <= maxIter; ++it) {
+    or_spmv(n, ro, ci, va, p, q);
+    double pq = dotv(n, p, q);
+    if (!(pq > 0.0)) break;
+    double alpha = rho / pq;
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    rnorm = sqrt(dotv(n, r, r));
+    if (rnorm <= tol) {
+      conv = 1;
+      break;
+    }
+    vcycle(&h->H, 0, r, z);
+    double rhoNew = dotv(n, r, z);
+    double beta = rhoNew / rho;
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    rho = rhoNew;
+  }
+  if (it > maxIter) it = maxIter;
+  if (finalRelRes) *finalRelRes = rnorm / bnorm;
+done:
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  return conv ? it : -it - 1;
+}
+
+void or_pcg_destroy(void *hv) {
+  or_pcg_handle *h = (or_pcg_handle *)hv;
+  if (h->built) amg_free(&h->H);
+  free(h);
 }
