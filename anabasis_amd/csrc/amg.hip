@@ -244,72 +244,94 @@ __global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict_
 }
 
 // --------------------------------------------------------- V-cycle -----
-// KIND 0 (down): out_i = b_i - sum_j a_ij x_j,  x_j = omdinv_j b_j
-// KIND 1 (up):   x_j = omdinv_j b_j + xc[agg_j];
-//                out_i = x_i + omdinv_i (b_i - sum_j a_ij x_j)
-// CSR-stream over 64-row wave slices (see linalg.hip).
+// One level of the V-cycle (omega-damped Jacobi, one sweep each way, from
+// x = 0), with y = omdinv o b supplied by the producer of b:
+//   down:     r_i = b_i - sum_j a_ij y_j               (KIND 0, v = y)
+//   restrict: bc_I = sum_{i in I} r_i, yc_I = omdinv_c,I bc_I
+//   prolong:  e_i = y_i + xc[agg_i]
+//   up:       x_i = e_i + omdinv_i (b_i - sum_j a_ij e_j)   (KIND 1, v = e)
+// so the two CSR-stream kernels gather a single vector, like an SpMV.
 template <typename T, int KIND>
-__global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int* __restrict__ rbs,
+__global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restrict__ rbs,
                                                    const int* __restrict__ ro,
                                                    const int* __restrict__ ci,
                                                    const T* __restrict__ va,
                                                    const T* __restrict__ omdinv,
                                                    const T* __restrict__ b,
-                                                   const T* __restrict__ xc,
-                                                   const int* __restrict__ agg,
+                                                   const T* __restrict__ v,
                                                    T* __restrict__ out) {
   __shared__ double prod[kW][kCap];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const WaveRows q = wave_rows(nrb, rbs, ro, lb * kW + w);
-  if (q.stream)
-    for (int k = q.k0 + lane; k < q.k1; k += 64) {
-      const int j = ci[k];
-      double xj = (double)omdinv[j] * (double)b[j];
-      if (KIND == 1) xj = xj + (double)xc[agg[j]];
-      prod[w][k - q.k0] = (double)va[k] * xj;
-    }
-  wave_sync();
+  const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
   const int i = q.r0 + lane;
+  int kb = 0, ke = 0;
+  double bi = 0.0, vi = 0.0, oi = 0.0;
+  if (i < q.r1) {
+    kb = ro[i];
+    ke = ro[i + 1];
+    bi = (double)b[i];
+    if (KIND == 1) {
+      vi = (double)v[i];
+      oi = (double)omdinv[i];
+    }
+  }
+  if (q.stream && q.k1 > q.k0)
+    stage_products(q, lane, ci, va, [&](int j) { return (double)v[j]; }, prod[w]);
+  wave_sync();
   if (i < q.r1) {
     double s = 0.0;
-    const int kb = ro[i], ke = ro[i + 1];
     if (q.stream) {
       for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     } else {
-      for (int k = kb; k < ke; ++k) {
-        const int j = ci[k];
-        double xj = (double)omdinv[j] * (double)b[j];
-        if (KIND == 1) xj = xj + (double)xc[agg[j]];
-        s += (double)va[k] * xj;
-      }
+      for (int k = kb; k < ke; ++k) s += (double)va[k] * (double)v[ci[k]];
     }
-    const double bi = (double)b[i];
-    if (KIND == 0) {
+    if (KIND == 0)
       out[i] = (T)(bi - s);
-    } else {
-      double xi = (double)omdinv[i] * bi;
-      xi = xi + (double)xc[agg[i]];
-      out[i] = (T)(xi + (double)omdinv[i] * (bi - s));
-    }
+    else
+      out[i] = (T)(vi + oi * (bi - s));
   }
 }
 
+// y = omdinv o b (level-0 entry of the V-cycle)
+template <typename T>
+__global__ void k_scale(int n, const T* __restrict__ omdinv, const T* __restrict__ b,
+                        T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (T)((double)omdinv[i] * (double)b[i]);
+}
+
+// bc_I = sum of the members' residuals (ascending member order);
+// yc_I = omdinv_c,I * bc_I for the coarse level's own down kernel
 template <typename T>
 __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
-                           const T* __restrict__ r, T* __restrict__ bc) {
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
+                           const T* __restrict__ r, const T* __restrict__ omdc,
+                           T* __restrict__ bc, T* __restrict__ yc) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
   double s = 0.0;
   for (int m = moff[I]; m < moff[I + 1]; ++m) s += (double)r[mem[m]];
-  bc[I] = (T)s;
+  const T bI = (T)s;
+  bc[I] = bI;
+  yc[I] = (T)((double)omdc[I] * (double)bI);
+}
+
+// e_i = y_i + xc[agg_i]
+template <typename T>
+__global__ void k_prolong(int n, const T* __restrict__ y, const T* __restrict__ xc,
+                          const int* __restrict__ agg, T* __restrict__ e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = (T)((double)y[i] + (double)xc[agg[i]]);
 }
 
 // coarsest: x_i = sum_j Ainv[i][j] b_j, Ainv stored column-major (FP64).
-// Blocked order: lane = row, wave = 64-column chunk; each chunk's partial is
-// summed in column order, then the partials in chunk order (n <= 4096).
+// Blocked order: lane = row, wave = column chunk of CH = 16 (n <= 1024) or
+// 64 (n <= 4096) columns; each chunk's partial is summed in column order,
+// then the partials in chunk order.  A chunk's loads are all issued before
+// the sum, so a wave waits on one memory round trip per chunk.
 static constexpr int kDenseThreads = 1024;
-template <typename T>
+static inline int dense_chunk(int n) { return n <= 1024 ? 16 : 64; }
+template <typename T, int CH>
 __global__ __launch_bounds__(kDenseThreads) void k_dense_solve(int n,
                                                                const double* __restrict__ invT,
                                                                const T* __restrict__ b,
@@ -318,13 +340,21 @@ __global__ __launch_bounds__(kDenseThreads) void k_dense_solve(int n,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
   const int i = blockIdx.x * 64 + lane;
-  const int nch = (n + 63) >> 6;
+  const int ic = min(i, n - 1);
+  const int nch = (n + CH - 1) / CH;
   for (int c = w; c < nch; c += nw) {
     double t = 0.0;
-    if (i < n) {
-      const int j0 = c * 64, j1 = min(j0 + 64, n);
-#pragma unroll 8
-      for (int j = j0; j < j1; ++j) t += invT[(size_t)j * n + i] * (double)b[j];
+    for (int j0 = c * CH; j0 < c * CH + CH; j0 += 16) {  // batches of 16 loads, summed in order
+      double a[16], bb[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = min(j0 + u, n - 1);
+        a[u] = invT[(size_t)j * n + ic];
+        bb[u] = (double)b[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (j0 + u < n) t += a[u] * bb[u];
     }
     part[c][lane] = t;
   }
@@ -471,6 +501,7 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
     L.b.alloc((size_t)L.n);
     L.x.alloc((size_t)L.n);
     L.r.alloc((size_t)L.n);
+    L.y.alloc((size_t)L.n);
     if (L.n <= opt.coarseSize || l == maxL - 1) {
       L.isCoarsest = true;
       break;
@@ -566,61 +597,91 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
     L.b32.alloc((size_t)L.n);
     L.x32.alloc((size_t)L.n);
     L.r32.alloc((size_t)L.n);
+    L.y32.alloc((size_t)L.n);
   }
   has32 = true;
 }
 
+// Per-precision views of a level's matrix and work vectors.
 template <typename T>
-static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, T* x, hipStream_t s,
-                   const T* (*omd)(const AmgLevel&), const DevCSR<T>& (*mat)(const AmgLevel&),
-                   T* (*bbuf)(AmgLevel&), T* (*xbuf)(AmgLevel&), T* (*rbuf)(AmgLevel&)) {
+struct LevelData;
+template <>
+struct LevelData<double> {
+  static const DevCSR<double>& A(const AmgLevel& L) { return *L.A; }
+  static const double* omd(const AmgLevel& L) { return L.omdinv.p; }
+  static double* b(AmgLevel& L) { return L.b.p; }
+  static double* x(AmgLevel& L) { return L.x.p; }
+  static double* r(AmgLevel& L) { return L.r.p; }
+  static double* y(AmgLevel& L) { return L.y.p; }
+};
+template <>
+struct LevelData<float> {
+  static const DevCSR<float>& A(const AmgLevel& L) { return L.A32; }
+  static const float* omd(const AmgLevel& L) { return L.omdinv32.p; }
+  static float* b(AmgLevel& L) { return L.b32.p; }
+  static float* x(AmgLevel& L) { return L.x32.p; }
+  static float* r(AmgLevel& L) { return L.r32.p; }
+  static float* y(AmgLevel& L) { return L.y32.p; }
+};
+
+// x = V-cycle(b) on level l; y = omdinv o b (computed here when null).
+template <typename T>
+static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* x,
+                   hipStream_t s) {
+  using D = LevelData<T>;
   AmgLevel& L = lv[l];
-  const DevCSR<T>& A = mat(L);
+  const DevCSR<T>& A = D::A(L);
   if (L.isCoarsest) {
     if (L.coarseInvT.p) {
-      const int nch = (L.n + 63) / 64;  // <= 64: dense level only when n <= 4096
-      k_dense_solve<T><<<nch, 64 * std::min(nch, kDenseThreads / 64), 0, s>>>(
-          L.n, L.coarseInvT.p, b, x);
+      // dense level only when n <= 4096: at most 64 chunks of either width
+      const int ch = dense_chunk(L.n);
+      const int nch = (L.n + ch - 1) / ch;
+      const int rowBlocks = (L.n + 63) / 64;
+      const int thr = 64 * std::min(nch, kDenseThreads / 64);
+      if (ch == 16)
+        k_dense_solve<T, 16><<<rowBlocks, thr, 0, s>>>(L.n, L.coarseInvT.p, b, x);
+      else
+        k_dense_solve<T, 64><<<rowBlocks, thr, 0, s>>>(L.n, L.coarseInvT.p, b, x);
     } else {
-      T* t = rbuf(L);
+      T* t = D::r(L);
       fill_zero(x, sizeof(T) * (size_t)L.n, s);
       for (int it = 0; it < 20; ++it) {
         T* src = (it & 1) ? t : x;
         T* dst = (it & 1) ? x : t;
-        k_jacobi<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, omd(L), b,
-                                                     src, dst);
+        k_jacobi<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p,
+                                                     D::omd(L), b, src, dst);
       }
     }
     AN_CHECK_LAUNCH();
     return;
   }
+  if (!y) {
+    T* yy = D::y(L);
+    k_scale<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, D::omd(L), b, yy);
+    y = yy;
+  }
   AmgLevel& C = lv[l + 1];
-  T* r = rbuf(L);
+  T* r = D::r(L);
   const int G = grid_for(A.rb.count, kW);
   k_level_rows<T, 0><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      omd(L), b, nullptr, nullptr, r);
-  k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, r, bbuf(C));
+                                      D::omd(L), b, y, r);
+  k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, r, D::omd(C),
+                                                  D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
-  vcycle<T>(lv, l + 1, bbuf(C), xbuf(C), s, omd, mat, bbuf, xbuf, rbuf);
+  vcycle<T>(lv, l + 1, D::b(C), D::y(C), D::x(C), s);
+  T* e = r;  // the residual is dead once restricted
+  k_prolong<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, y, D::x(C), L.agg.p, e);
   k_level_rows<T, 1><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      omd(L), b, xbuf(C), L.agg.p, x);
+                                      D::omd(L), b, e, x);
   AN_CHECK_LAUNCH();
 }
 
 void AmgHierarchy::apply(const double* r, double* z, hipStream_t s) {
-  vcycle<double>(
-      lv, 0, r, z, s, [](const AmgLevel& L) -> const double* { return L.omdinv.p; },
-      [](const AmgLevel& L) -> const DevCSR<double>& { return *L.A; },
-      [](AmgLevel& L) -> double* { return L.b.p; }, [](AmgLevel& L) -> double* { return L.x.p; },
-      [](AmgLevel& L) -> double* { return L.r.p; });
+  vcycle<double>(lv, 0, r, nullptr, z, s);
 }
 
 void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s) {
-  vcycle<float>(
-      lv, 0, r, z, s, [](const AmgLevel& L) -> const float* { return L.omdinv32.p; },
-      [](const AmgLevel& L) -> const DevCSR<float>& { return L.A32; },
-      [](AmgLevel& L) -> float* { return L.b32.p; }, [](AmgLevel& L) -> float* { return L.x32.p; },
-      [](AmgLevel& L) -> float* { return L.r32.p; });
+  vcycle<float>(lv, 0, r, nullptr, z, s);
 }
 
 }  // namespace an

@@ -20,10 +20,10 @@ struct AmgLevel {
   DevBuf<double> omdinv;        // omega / a_ii
   DevBuf<int> agg, moff, mem;   // aggregation to level+1
   DevBuf<double> coarseInvT;    // coarsest: dense inverse, column-major
-  DevBuf<double> b, x, r;       // work vectors (FP64)
+  DevBuf<double> b, x, r, y;    // work vectors (FP64); y = omdinv o b
   // FP32 copies for the mixed-precision V-cycle
   DevCSR<float> A32;
-  DevBuf<float> omdinv32, b32, x32, r32;
+  DevBuf<float> omdinv32, b32, x32, r32, y32;
   bool isCoarsest = false;
 };
 

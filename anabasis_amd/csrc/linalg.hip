@@ -20,6 +20,8 @@
 // atomics are involved, and scalars never leave the GPU inside an iteration.
 #include "linalg.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace an {
 
 static constexpr int kBlock = 256;
@@ -32,7 +34,7 @@ static constexpr int kRedBlocks = 1024;
 // cap; pass 0 counts the blocks of each slice, pass 1 writes their starts.
 template <int PASS>
 __global__ void k_row_blocks(int n, const int* __restrict__ ro, const int* __restrict__ off,
-                             int* __restrict__ out) {
+                             int* __restrict__ cnto, int2* __restrict__ out) {
   const int sl = blockIdx.x * blockDim.x + threadIdx.x;
   const int nsl = (n + kStreamRows - 1) / kStreamRows;
   if (sl >= nsl) return;
@@ -42,24 +44,24 @@ __global__ void k_row_blocks(int n, const int* __restrict__ ro, const int* __res
     const int st = r;
     ++r;
     while (r < b && ro[r + 1] - ro[st] <= kStreamCap) ++r;
-    if (PASS == 1) out[off[sl] + cnt] = st;
+    if (PASS == 1) out[off[sl] + cnt] = make_int2(st, ro[st]);
     ++cnt;
   }
-  if (PASS == 0) out[sl] = cnt;
+  if (PASS == 0) cnto[sl] = cnt;
+  if (PASS == 1 && sl == nsl - 1) out[off[sl] + cnt] = make_int2(n, ro[n]);
 }
-
-__global__ void k_set_int(int* p, int v) { *p = v; }
 
 void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s) {
   const int nsl = (n + kStreamRows - 1) / kStreamRows;
   DevBuf<int> cnt((size_t)nsl + 1), off((size_t)nsl + 1);
-  k_row_blocks<0><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, nullptr, cnt.p);
+  rb.count = 0;
+  if (nsl == 0) return;
+  k_row_blocks<0><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, nullptr, cnt.p, nullptr);
   AN_CHECK_LAUNCH();
   const long long tot = exclusive_scan(cnt.p, off.p, nsl, s);
   rb.count = (int)tot;
   rb.start.alloc((size_t)tot + 1);
-  k_row_blocks<1><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, off.p, rb.start.p);
-  k_set_int<<<1, 1, 0, s>>>(rb.start.p + tot, n);
+  k_row_blocks<1><<<grid_for(nsl, 256), 256, 0, s>>>(n, rowOff, off.p, nullptr, rb.start.p);
   AN_CHECK_LAUNCH();
   AN_HIP(hipStreamSynchronize(s));
 }
@@ -119,7 +121,7 @@ void Reducer::init(int maxBlocks, hipStream_t s) {
 // MODE 0: y = A x ; MODE 1: y = b - A x.  DOT 0: none; 1: dot(dw, y);
 // 2: dot(y, y).
 template <typename VT, typename XT, typename YT, int MODE, int DOT>
-__global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int* __restrict__ rbs,
+__global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict__ rbs,
                                                  const int* __restrict__ rowOff,
                                                  const int* __restrict__ cols,
                                                  const VT* __restrict__ vals,
@@ -131,15 +133,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int* __restrict_
   __shared__ double sh[kWaves];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const WaveRows q = wave_rows(nrb, rbs, rowOff, lb * kWaves + w);
-  if (q.stream)
-    for (int k = q.k0 + lane; k < q.k1; k += 64)
-      prod[w][k - q.k0] = (double)vals[k] * (double)x[cols[k]];
+  const WaveRows q = wave_rows(nrb, rbs, lb * kWaves + w);
+  const int row = q.r0 + lane;
+  int kb = 0, ke = 0;
+  if (row < q.r1) {
+    kb = rowOff[row];
+    ke = rowOff[row + 1];
+  }
+  if (q.stream && q.k1 > q.k0)
+    stage_products(q, lane, cols, vals, [&](int j) { return (double)x[j]; }, prod[w]);
   wave_sync();
   double dv = 0.0;
-  const int row = q.r0 + lane;
   if (row < q.r1) {
-    const int kb = rowOff[row], ke = rowOff[row + 1];
     double s = 0.0;
     if (q.stream)
       for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
@@ -168,18 +173,33 @@ static int spmv_grid(const DevCSR<VT>& A) {
 
 template <typename VT, typename XT>
 void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* scal, int slot,
-          Reducer& red, hipStream_t s) {
+          Reducer& red, hipStream_t s, hipEvent_t evStart, hipEvent_t evStop) {
   const int nb = spmv_grid(A);
+  const int2* rbs = A.rb.start.p;
+  const int* ro = A.rowOff.p;
+  const int* ci = A.cols.p;
+  const VT* va = A.vals.p;
+  const double* nob = nullptr;
   if (dotWith) {
     if (nb > red.cap) throw Error(7, "reducer too small");
-    k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
-                                                   A.vals.p, x, nullptr, y, dotWith,
-                                                   red.partial.p);
+    double* part = red.partial.p;
+    if (evStart)
+      hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 1>, dim3(nb), dim3(kBlock), 0, s, evStart,
+                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, dotWith, part);
+    else
+      k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y,
+                                                     dotWith, part);
     AN_CHECK_LAUNCH();
     finalize(red, nb, 1, scal + slot, s);
   } else {
-    k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
-                                                   A.vals.p, x, nullptr, y, nullptr, nullptr);
+    const XT* nodw = nullptr;
+    double* nopart = nullptr;
+    if (evStart)
+      hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 0>, dim3(nb), dim3(kBlock), 0, s, evStart,
+                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, nodw, nopart);
+    else
+      k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y, nodw,
+                                                     nopart);
   }
   AN_CHECK_LAUNCH();
 }
@@ -297,9 +317,9 @@ void fill_zero(void* p, size_t bytes, hipStream_t s) {
 
 // explicit instantiations
 template void spmv<double, double>(const DevCSR<double>&, const double*, double*, const double*,
-                                   double*, int, Reducer&, hipStream_t);
+                                   double*, int, Reducer&, hipStream_t, hipEvent_t, hipEvent_t);
 template void spmv<float, float>(const DevCSR<float>&, const float*, float*, const float*, double*,
-                                 int, Reducer&, hipStream_t);
+                                 int, Reducer&, hipStream_t, hipEvent_t, hipEvent_t);
 template void residual<double, double>(const DevCSR<double>&, const double*, const double*,
                                        double*, double*, Reducer&, hipStream_t);
 template void residual<double, float>(const DevCSR<double>&, const double*, const double*, float*,

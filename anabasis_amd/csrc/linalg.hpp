@@ -8,11 +8,12 @@ namespace an {
 // Row blocks of the CSR-stream kernels: consecutive rows grouped so that a
 // block holds at most kStreamRows rows and at most kStreamCap nonzeros (a
 // single longer row forms a block of its own).  One 64-lane wave owns one
-// block; start[count] = n.
+// block.  start[b] = {first row, its first nonzero}; start[count] = {n, nnz}.
 static constexpr int kStreamRows = 64;
 static constexpr int kStreamCap = 768;
+static constexpr int kStreamPer = kStreamCap / 64;  // staged nonzeros per lane
 struct RowBlocks {
-  DevBuf<int> start;
+  DevBuf<int2> start;
   int count = 0;
 };
 // Greedy partition of rowOff[0..n] (synchronises once to read the count).
@@ -42,17 +43,45 @@ struct WaveRows {
   int r0, r1, k0, k1;
   bool stream;
 };
-__device__ __forceinline__ WaveRows wave_rows(int nrb, const int* __restrict__ rbs,
-                                              const int* __restrict__ rowOff, int gw) {
+__device__ __forceinline__ WaveRows wave_rows(int nrb, const int2* __restrict__ rbs, int gw) {
   WaveRows q{0, 0, 0, 0, false};
   if (gw < nrb) {
-    q.r0 = rbs[gw];
-    q.r1 = rbs[gw + 1];
-    q.k0 = rowOff[q.r0];
-    q.k1 = rowOff[q.r1];
+    const int2 a = rbs[gw], b = rbs[gw + 1];
+    q.r0 = a.x;
+    q.r1 = b.x;
+    q.k0 = a.y;
+    q.k1 = b.y;
     q.stream = (q.k1 - q.k0) <= kStreamCap;
   }
   return q;
+}
+
+// Stage prod[k - k0] = vals[k] * xval(cols[k]) for the block's nonzeros:
+// every lane first issues all of its (at most kStreamPer) index/value
+// loads, then all of its gathers, so a wave waits on two memory round trips
+// rather than one per nonzero.  Slots past the block's end re-read its last
+// nonzero (no divergent branches between the loads, so the compiler keeps
+// them all in flight) and are not stored.  Requires k1 > k0.
+template <typename VT, typename XF>
+__device__ __forceinline__ void stage_products(const WaveRows& q, int lane,
+                                               const int* __restrict__ cols,
+                                               const VT* __restrict__ vals, XF xval,
+                                               double* __restrict__ prod) {
+  int c[kStreamPer];
+  double v[kStreamPer], xv[kStreamPer];
+#pragma unroll
+  for (int u = 0; u < kStreamPer; ++u) {
+    const int k = min(q.k0 + lane + 64 * u, q.k1 - 1);
+    c[u] = cols[k];
+    v[u] = (double)vals[k];
+  }
+#pragma unroll
+  for (int u = 0; u < kStreamPer; ++u) xv[u] = xval(c[u]);
+#pragma unroll
+  for (int u = 0; u < kStreamPer; ++u) {
+    const int k = q.k0 + lane + 64 * u;
+    if (k < q.k1) prod[k - q.k0] = v[u] * xv[u];
+  }
 }
 
 // Scalar slots of a solver (device memory).
@@ -70,10 +99,12 @@ struct Reducer {
 void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s);
 
 // y = A x.  When dotWith != nullptr also computes dot(dotWith, y) into
-// scal[slot] (deterministically).
+// scal[slot] (deterministically).  evStart/evStop, when given, record the
+// start and end of the SpMV kernel itself (hipExtLaunchKernel), excluding
+// the reduction's finalize kernel.
 template <typename VT, typename XT>
 void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* scal, int slot,
-          Reducer& red, hipStream_t s);
+          Reducer& red, hipStream_t s, hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
 // scal[slot] = dot(a, b)
 template <typename T>
 void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hipStream_t s);

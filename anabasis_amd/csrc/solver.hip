@@ -149,9 +149,8 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
-    if (e1) AN_HIP(hipEventRecord(e1->a, s));
-    spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s);
-    if (e1) AN_HIP(hipEventRecord(e1->b, s));
+    spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
+                         e1 ? e1->b : nullptr);
     pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));
@@ -187,9 +186,8 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double* 
     const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
-    if (e1) AN_HIP(hipEventRecord(e1->a, s));
-    spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s);
-    if (e1) AN_HIP(hipEventRecord(e1->b, s));
+    spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
+                       e1 ? e1->b : nullptr);
     pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));

@@ -161,6 +161,11 @@ def main():
     ap.add_argument("--no-psolve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--only-psolve", action="store_true",
+                    help="run only the pressure-solve section (profiling / counter runs)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first copy a buffer of this many MiB three times (a kernel of known "
+                         "HBM traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X "
@@ -168,6 +173,20 @@ def main():
         sys.exit(2)
     from anabasis_amd import _lib as L
     L.check(L.lib().an_set_device(0))
+    if args.copy_probe_mb:
+        import torch
+        nb = args.copy_probe_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {nb} bytes read + written")
+        del src, dst
+    if args.only_psolve:
+        pm, pat, ps = run_psolve(args.pn, args.psolve_reps)
+        print(json.dumps({"psolve": ps}), flush=True)
+        return
 
     m, vel, prs = simple_case(args.cells)
     r64 = run_simple(m, vel, prs, 0, args.warmup, args.steps)

@@ -18,14 +18,20 @@
  *   or_extract_rau         GSV1 main.cu:1796-1802
  *   or_pressure_assemble   GSV1 main.cu:1898-1950 (+ zero, 2532-2551)
  *
- * Pinned by tests/test_oracle_assembly.py: on an orthogonal hex box the
- * elliptic coefficient equals gamma*A/h exactly and rows of interior cells sum
- * to zero; the pressure matrix is symmetric with rAU-weighted coefficients and
- * equals the elliptic matrix when rAU = gamma = 1 on an orthogonal mesh; the
- * momentum matrix with mu = 0 reduces to the central-convection matrix of
- * or_transport_assemble for the same face fluxes; the non-orthogonal
- * correction vanishes on orthogonal meshes; a Dirichlet MMS solve reaches
- * second-order L2 convergence (tests/test_oracle_mms.py).
+ * Pinned by tests/test_oracle.py and tests/test_oracle_ops.py: on an
+ * orthogonal hex box the elliptic coefficient equals gamma*A/h exactly and
+ * rows of interior cells sum to zero; linear fields are reproduced exactly
+ * with Dirichlet + Neumann data in both sign conventions (LP -gamma g A, LS
+ * +gamma g A) and with the reference cell of a pure-Neumann problem; the
+ * pressure matrix is symmetric positive definite, linear in rAU, and equals
+ * the elliptic matrix when rAU = gamma = 1 on an orthogonal mesh; the
+ * momentum matrix with mu = 0 equals the LS central-convection matrix of the
+ * same face fluxes (matrix and rhs, boundary rules included); upwind
+ * convection has its closed form; the non-orthogonal correction vanishes on
+ * orthogonal meshes; a Dirichlet MMS solve converges at second order in L2;
+ * relaxation keeps the solution of the unrelaxed system and rAU = V/aP;
+ * the rhs-only path plus its relaxation (aP rebuilt from the relaxed matrix,
+ * main.cu:1790) equals the full assembly's relaxed rhs to round-off.
  */
 #include "oracle.h"
 

@@ -14,7 +14,7 @@
  *   normal constraint r = dn*n, dphi = g*dn; the sums are accumulated
  *   directly: g = M^-1 sum w r dphi).
  *
- * Pinned by tests/test_oracle_gradient.py: exact reproduction of linear fields
+ * Pinned by tests/test_oracle.py (test_gradient_*): exact reproduction of linear fields
  * when every boundary face is an exact Dirichlet face (both variants), zero
  * gradient of constants, a brute-force numpy least-squares solve per cell on a
  * small jittered mesh (independent of the adjugate formula), and the LP

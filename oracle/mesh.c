@@ -7,11 +7,12 @@
  *   Pinned by: closed forms on generated meshes (hex volume h^3, tet volume
  *   h^3/6, sum of volumes = box volume, sum of outward S over each closed
  *   cell = 0, centroids of undistorted hexes = box centres) in
- *   tests/test_oracle_mesh.py.
+ *   tests/test_oracle.py (test_geometry_*, test_product_*).
  * or_pattern_*       follows LP/src/scalar_elliptic.cu:16-58
  *   (== GSV1 build_momentum_pattern main.cu:1315-1340).
- *   Pinned by: an independent numpy/set restatement on the 44,100-cell
- *   cylinder owner/neighbour fixture (nnz = 617,000) plus structural
+ *   Pinned by tests/test_oracle.py (test_pattern_*): the CRC of the CSR
+ *   built from the reference's 44,100-cell cylinder owner/neighbour fixture
+ *   (nnz = 617,000), an independent python set restatement, and structural
  *   invariants (sorted unique columns, diagonal present, PN/NP transposed,
  *   nnz = nC + 2*nIF for meshes without duplicate cell pairs).
  */

@@ -252,3 +252,97 @@ def patch_forces(m: OMesh, f0, f1, u, v, w, p, mu, sign):
                           _p(_d(u)), _p(_d(v)), _p(_d(w)), _p(_d(p)), C.c_double(mu), sign,
                           _p(out))
     return out
+
+
+# ------------------------------------------------- SIMPLE building blocks
+# Types are per PATCH (index = bPatch - 1), values per FACE (nFaces entries),
+# as in the reference DeviceBC arrays.
+def momentum_assemble(m: OMesh, gradQ, uc, vc, wc, gradPcomp, tQ, vQ, tU, vU, tV, vV, tW, vW,
+                      rho, mu, corrPsi=1.0):
+    vals = np.empty(m.nnz)
+    rhs = np.empty(m.nCells)
+    lib().or_momentum_assemble(
+        m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh), _p(m.bPatch),
+        _p(m.diagPos), _p(m.PP), _p(m.PN), _p(m.NP), _p(m.NN), _p(m.cc), _p(m.vol), _p(m.xf),
+        _p(m.nf), _p(m.Sf), _p(m.Af), _p(_d(np.asarray(gradQ).reshape(-1))), _p(_d(uc)),
+        _p(_d(vc)), _p(_d(wc)), _p(_d(gradPcomp)), _p(_i(tQ)), _p(_d(vQ)), _p(_i(tU)),
+        _p(_d(vU)), _p(_i(tV)), _p(_d(vV)), _p(_i(tW)), _p(_d(vW)), C.c_double(rho),
+        C.c_double(mu), C.c_double(corrPsi), C.c_int64(m.nnz), _p(vals), _p(rhs))
+    return vals, rhs
+
+
+def momentum_relax(m: OMesh, vals, rhs, qOld, uRelax):
+    vals, rhs = _d(vals).copy(), _d(rhs).copy()
+    lib().or_momentum_relax(m.nCells, _p(m.diagPos), _p(vals), _p(rhs), _p(_d(qOld)),
+                            C.c_double(uRelax))
+    return vals, rhs
+
+
+def extract_rau(m: OMesh, vals):
+    out = np.empty(m.nCells)
+    lib().or_extract_rau(m.nCells, _p(m.diagPos), _p(_d(vals)), _p(m.vol), _p(out))
+    return out
+
+
+def pressure_assemble(m: OMesh, rAU, tP):
+    vals = np.empty(m.nnz)
+    lib().or_pressure_assemble(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh),
+                               _p(m.bPatch), _p(m.diagPos), _p(m.PP), _p(m.PN), _p(m.NP),
+                               _p(m.NN), _p(m.cc), _p(m.xf), _p(m.nf), _p(m.Af), _p(_d(rAU)),
+                               _p(_i(tP)), C.c_int64(m.nnz), _p(vals))
+    return vals
+
+
+def rhiechow_flux(m: OMesh, u, v, w, p, gradP, rAU, tU, ub, tV, vb, tW, wb, rho):
+    out = np.empty(m.nFaces)
+    lib().or_rhiechow_flux(m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh), _p(m.bPatch),
+                           _p(m.cc), _p(m.xf), _p(m.nf), _p(m.Af), _p(_d(u)), _p(_d(v)),
+                           _p(_d(w)), _p(_d(p)), _p(_d(np.asarray(gradP).reshape(-1))),
+                           _p(_d(rAU)), _p(_i(tU)), _p(_d(ub)), _p(_i(tV)), _p(_d(vb)),
+                           _p(_i(tW)), _p(_d(wb)), C.c_double(rho), _p(out))
+    return out
+
+
+def continuity(m: OMesh, phi):
+    out = np.empty(m.nCells)
+    lib().or_continuity(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh),
+                        _p(_d(phi)), _p(out))
+    return out
+
+
+def pressure_nonorth(m: OMesh, rAU, tP, gradP):
+    phi = np.empty(m.nFaces)
+    div = np.empty(m.nCells)
+    lib().or_pressure_nonorth(m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh),
+                              _p(m.bPatch), _p(m.cc), _p(m.xf), _p(m.nf), _p(m.Sf), _p(m.Af),
+                              _p(_d(rAU)), _p(_i(tP)), _p(_d(np.asarray(gradP).reshape(-1))),
+                              _p(phi), _p(div))
+    return phi, div
+
+
+def correct_flux(m: OMesh, rAU, tP, pb, phiStar, pCorr, phiNo):
+    out = np.empty(m.nFaces)
+    lib().or_correct_flux(m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh), _p(m.bPatch),
+                          _p(m.cc), _p(m.xf), _p(m.nf), _p(m.Af), _p(_d(rAU)), _p(_i(tP)),
+                          _p(_d(pb)), _p(_d(phiStar)), _p(_d(pCorr)), _p(_d(phiNo)), _p(out))
+    return out
+
+
+def momentum_rhs_only(m: OMesh, gradQ, uc, vc, wc, gradPcomp, tQ, vQ, tU, vU, tV, vV, tW, vW,
+                      rho, mu, corrPsi=1.0):
+    rhs = np.empty(m.nCells)
+    lib().or_momentum_rhs_only(
+        m.nCells, m.nFaces, m.nInternalFaces, _p(m.owner), _p(m.neigh), _p(m.bPatch), _p(m.cc),
+        _p(m.vol), _p(m.xf), _p(m.nf), _p(m.Sf), _p(m.Af),
+        _p(_d(np.asarray(gradQ).reshape(-1))), _p(_d(uc)), _p(_d(vc)), _p(_d(wc)),
+        _p(_d(gradPcomp)), _p(_i(tQ)), _p(_d(vQ)), _p(_i(tU)), _p(_d(vU)), _p(_i(tV)),
+        _p(_d(vV)), _p(_i(tW)), _p(_d(vW)), C.c_double(rho), C.c_double(mu), C.c_double(corrPsi),
+        _p(rhs))
+    return rhs
+
+
+def momentum_relax_rhs_only(m: OMesh, relaxed_vals, rhs, qOld, uRelax):
+    rhs = _d(rhs).copy()
+    lib().or_momentum_relax_rhs_only(m.nCells, _p(m.diagPos), _p(_d(relaxed_vals)), _p(rhs),
+                                     _p(_d(qOld)), C.c_double(uRelax))
+    return rhs

@@ -9,14 +9,18 @@
  *   or_correct_flux      GSV1 main.cu:1615-1658
  *   or_patch_forces      GSV1 main.cu:2828-2933
  *
- * Pinned by tests/test_oracle_simple_ops.py: with uniform rAU and a linear
+ * Pinned by tests/test_oracle_ops.py: with uniform rAU and a linear
  * pressure field the Rhie-Chow term vanishes (main.cu:694 "linear-pressure-
- * preserving"), so phi* equals the interpolated flux; the continuity residual
- * of any face flux sums to the net boundary outflow; the non-orthogonal flux
- * vanishes on orthogonal meshes and reproduces rAU*grad.t for a linear field;
- * the corrected flux satisfies div(phi) = div(phi*) - A p' - div(phi_no) with
- * A the assembled pressure matrix; the wall force of a uniform pressure on a
- * closed patch is zero and the pressure force is linear in p.
+ * preserving"), so phi* equals rho U.S for a uniform velocity, and a
+ * non-linear pressure switches it on; the continuity residual is zero for a
+ * uniform velocity, sums to the net boundary outflow, and adds +phi to the
+ * owner and -phi to the neighbour; for a linear p' and uniform rAU the
+ * compact correction plus the non-orthogonal flux equals rAU grad(p').S on
+ * internal and Dirichlet-p faces, Neumann faces keep phi*, and the
+ * non-orthogonal flux vanishes on orthogonal meshes; the wall force of a
+ * uniform pressure over a closed surface is zero, on a flat patch it is
+ * p A n, the viscous force of a uniform tangential velocity is mu U A/dn,
+ * and the pressure force is linear in p.
  */
 #include "oracle.h"
 

@@ -28,10 +28,15 @@
  *       coarsest level.
  *   BiCGSTAB with right Jacobi preconditioning, same stop rule.
  *
- * Pinned by tests/test_oracle_solvers.py: converged solutions agree with a
- * scipy sparse direct solve (spsolve) to 1e-9 relative on meshes of up to
- * ~30k cells (SPD Laplacian, pressure matrix, non-symmetric momentum matrix),
- * residual histories are monotone for PCG, aggregates partition the rows.
+ * Pinned by tests/test_oracle.py and tests/test_oracle_ops.py: PCG+AMG and
+ * BiCGSTAB solutions agree with a scipy sparse direct solve (spsolve) to
+ * 1e-9 / 1e-10 relative (SPD Laplacian; non-symmetric transport matrix);
+ * aggregates partition the rows into connected sets with a coarsening
+ * ratio above 3; the V-cycle is a linear, symmetric, positive operator, and
+ * is the exact inverse when the coarsest level is the matrix itself (both
+ * blocked-sum widths of the dense solve).  The aggregation itself defines
+ * the algorithm (it replaces HYPRE BoomerAMG), so its exact aggregates have
+ * no outside reference: the GPU is checked against this file bit for bit.
  */
 #include "oracle.h"
 
@@ -450,15 +455,16 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
   double *r = H->rb[l];
   if (L->isCoarsest) {
     if (L->coarseInv) {
-      /* x = Ainv b summed in 64-column chunks: each chunk's partial sum in
-         column order, then the chunk partials in chunk order (the blocked
-         order the GPU coarse solve uses) */
+      /* x = Ainv b summed in column chunks of 16 (n <= 1024) or 64: each
+         chunk's partial sum in column order, then the chunk partials in
+         chunk order (the blocked order the GPU coarse solve uses) */
+      const int ch = L->n <= 1024 ? 16 : 64;
       for (int i = 0; i < L->n; ++i) {
         double s = 0.0;
         const double *row = L->coarseInv + (size_t)i * L->n;
-        for (int c = 0; c < L->n; c += 64) {
+        for (int c = 0; c < L->n; c += ch) {
           double t = 0.0;
-          const int e = c + 64 < L->n ? c + 64 : L->n;
+          const int e = c + ch < L->n ? c + ch : L->n;
           for (int j = c; j < e; ++j) t += row[j] * b[j];
           s += t;
         }

@@ -1,0 +1,422 @@
+"""Pins the oracle's SIMPLE building blocks, the pressure/momentum assembly
+and the AMG V-cycle against closed forms, invariants, cross-checks between
+independent oracle routines and direct solves (CPU only)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from anabasis_amd.libpoisson import Mesh
+from oracle import pyoracle as O
+
+from meshes import bc_arrays
+
+PATCHES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+
+
+def box(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, jitter=0.0, seed=0):
+    m = Mesh.hexbox(nx, ny, nz, lx, ly, lz, jitter=jitter, seed=seed)
+    assert tuple(m.patchNames) == PATCHES
+    return m, O.OMesh.from_product(m)
+
+
+def patch_types(dirichlet=()):
+    return np.array([1 if p in dirichlet else 0 for p in PATCHES], np.int32)
+
+
+def csr(om, vals):
+    return sp.csr_matrix((vals, om.cols, om.rowOffsets), shape=(om.nCells, om.nCells))
+
+
+@pytest.fixture(scope="module")
+def jb():
+    return box(6, 5, 4, 1.0, 0.8, 0.7, jitter=0.2, seed=11)
+
+
+# ------------------------------------------------------------- pressure
+def test_pressure_matrix_equals_elliptic_on_orthogonal_mesh():
+    """rAU = 1: the a8 operator is the a13 Laplacian with gamma = 1."""
+    m, om = box(5, 4, 3, 1.0, 0.9, 0.6)
+    tP = patch_types(("xmin",))
+    vp = O.pressure_assemble(om, np.ones(om.nCells), tP)
+    t, v = bc_arrays(m, dirichlet=("xmin",))
+    ve, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                np.zeros((om.nCells, 3)), nonorth=False)
+    assert np.allclose(vp, ve, rtol=1e-13, atol=0)
+
+
+def test_pressure_matrix_symmetric_spd_rau_weighted(jb):
+    m, om = jb
+    rng = np.random.default_rng(5)
+    rau = rng.uniform(0.5, 2.0, om.nCells)
+    A = csr(om, O.pressure_assemble(om, rau, patch_types(("xmax",))))
+    assert abs(A - A.T).max() == 0.0
+    interior = np.ones(om.nCells, bool)
+    interior[om.owner[om.nInternalFaces:]] = False
+    assert np.abs(np.asarray(A.sum(axis=1)).ravel()[interior]).max() < 1e-13
+    assert np.linalg.eigvalsh(A.toarray()).min() > 0.0
+    # doubling rAU doubles the matrix (coefficients are linear in rAU)
+    A2 = csr(om, O.pressure_assemble(om, 2.0 * rau, patch_types(("xmax",))))
+    assert abs(A2 - 2.0 * A).max() < 1e-14 * abs(A).max()
+
+
+# ------------------------------------------------------------- momentum
+def test_momentum_without_viscosity_is_central_convection(jb):
+    """mu = 0 and a uniform velocity: the momentum matrix is the LS central
+    convection matrix of the face flux rho U.S, with the same boundary rule
+    (Dirichlet rhs -F q_B, Neumann diag += F)."""
+    m, om = jb
+    nC, nF, nIF = om.nCells, om.nFaces, om.nInternalFaces
+    U = np.array([0.3, -0.2, 0.1])
+    rho = 1.3
+    tU = patch_types(("xmin", "ymin"))
+    qB = np.zeros(nF)
+    uB = [np.zeros(nF) for _ in range(3)]
+    for k in np.nonzero(tU)[0]:
+        sel = om.bPatch == k + 1
+        qB[sel] = 0.7
+        for d in range(3):
+            uB[d][sel] = U[d]
+    uc = [np.full(nC, U[d]) for d in range(3)]
+    vm, rm = O.momentum_assemble(om, np.zeros((nC, 3)), *uc, np.zeros(nC), tU, qB, tU, uB[0],
+                                 tU, uB[1], tU, uB[2], rho, 0.0)
+    flux = rho * (om.Sf @ U)
+    tb = tU[om.bPatch[nIF:] - 1]
+    vt, rt = O.transport_assemble(om, flux, np.zeros(nF), np.zeros(nC), np.zeros(nC), tb,
+                                  qB[nIF:], np.zeros((nC, 3)), nonorth=False)
+    scale = np.abs(vt).max()
+    assert np.abs(vm - vt).max() < 1e-13 * scale
+    assert np.abs(rm - rt).max() < 1e-13 * scale
+
+
+def test_momentum_relax_and_rau(jb):
+    m, om = jb
+    nC = om.nCells
+    rng = np.random.default_rng(2)
+    vals = rng.standard_normal(om.nnz)
+    vals[om.diagPos] = rng.uniform(1.0, 3.0, nC)
+    rhs = rng.standard_normal(nC)
+    q = rng.standard_normal(nC)
+    vr, rr = O.momentum_relax(om, vals, rhs, q, 0.5)
+    off = np.ones(om.nnz, bool)
+    off[om.diagPos] = False
+    assert np.array_equal(vr[off], vals[off])
+    # relaxed system keeps the solution of A q = b when q solves it
+    A = csr(om, vals)
+    b = A @ q
+    vr, rr = O.momentum_relax(om, vals, b, q, 0.5)
+    assert np.allclose(csr(om, vr) @ q, rr, rtol=1e-12, atol=1e-12)
+    assert np.allclose(O.extract_rau(om, vr), om.vol / vr[om.diagPos], rtol=1e-15)
+
+
+def test_momentum_rhs_only_matches_full_assembly(jb):
+    """main.cu:1697-1794: the rhs-only path (other velocity components, same
+    matrix) rebuilds a_P from the relaxed matrix (a_P^relaxed * alpha_u), so
+    its relaxed rhs equals the full assembly's relaxed rhs to round-off."""
+    m, om = jb
+    nC, nF = om.nCells, om.nFaces
+    rng = np.random.default_rng(8)
+    gq = rng.standard_normal((nC, 3))
+    uc = [0.3 + 0.1 * rng.standard_normal(nC) for _ in range(3)]
+    gpc = rng.standard_normal(nC)
+    tU = patch_types(("xmin", "ymin", "zmax"))
+    qB, uB = rng.standard_normal(nF), [rng.standard_normal(nF) for _ in range(3)]
+    args = (gq, *uc, gpc, tU, qB, tU, uB[0], tU, uB[1], tU, uB[2], 1.1, 0.02)
+    vals, rhs = O.momentum_assemble(om, *args)
+    q = rng.standard_normal(nC)
+    vr, rr = O.momentum_relax(om, vals, rhs, q, 0.5)
+    r1 = O.momentum_rhs_only(om, *args)
+    r1 = O.momentum_relax_rhs_only(om, vr, r1, q, 0.5)
+    assert np.abs(r1 - rr).max() < 1e-12 * np.abs(rr).max()
+
+
+# ------------------------------------------------------- elliptic / MMS
+def test_nonorth_correction_vanishes_on_orthogonal_mesh():
+    m, om = box(4, 5, 3)
+    t, v = bc_arrays(m, dirichlet=("xmin", "zmax"), value=lambda x: x[:, 1])
+    g = np.random.default_rng(1).standard_normal((om.nCells, 3))
+    src = np.ones(om.nCells)
+    v1, r1 = O.elliptic_assemble(om, np.ones(om.nFaces), src, t, v, g, nonorth=True)
+    v0, r0 = O.elliptic_assemble(om, np.ones(om.nFaces), src, t, v, g, nonorth=False)
+    assert np.array_equal(v1, v0)
+    assert np.abs(r1 - r0).max() < 1e-14
+
+
+def mms_error(n):
+    m, om = box(n, n, n)
+    u = lambda x: np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1]) * np.sin(np.pi * x[:, 2])
+    t, v = bc_arrays(m, dirichlet=PATCHES, value=u)
+    f = 3 * np.pi ** 2 * u(om.cc)
+    vals, rhs = O.elliptic_assemble(om, np.ones(om.nFaces), f, t, v, np.zeros((om.nCells, 3)))
+    x = spla.spsolve(csr(om, vals).tocsc(), rhs)
+    return np.sqrt(np.sum(om.vol * (x - u(om.cc)) ** 2))
+
+
+def test_dirichlet_mms_second_order():
+    e = [mms_error(n) for n in (8, 16, 32)]
+    orders = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    assert e[-1] < 1e-3
+    assert np.all(orders > 1.9), orders
+
+
+# ------------------------------------------------------- SIMPLE face ops
+def test_rhiechow_vanishes_for_linear_pressure(jb):
+    """main.cu:694: with uniform rAU and linear p the RC term is zero, so
+    phi* is the interpolated flux rho U.S of a uniform velocity."""
+    m, om = jb
+    nC, nF = om.nCells, om.nFaces
+    U = np.array([0.4, 0.1, -0.25])
+    a = np.array([1.5, -0.7, 0.3])
+    p = om.cc @ a + 2.0
+    gp = np.tile(a, (nC, 1))
+    t = patch_types(("xmin",))
+    ub = [np.zeros(nF) for _ in range(3)]
+    for d in range(3):
+        ub[d][om.bPatch == 1] = U[d]
+    phi = O.rhiechow_flux(om, *(np.full(nC, U[d]) for d in range(3)), p, gp,
+                          np.full(nC, 0.37), t, ub[0], t, ub[1], t, ub[2], 1.3)
+    ref = 1.3 * (om.Sf @ U)
+    assert np.abs(phi - ref).max() < 1e-13 * np.abs(ref).max()
+    # a non-linear pressure switches the RC term on
+    phi2 = O.rhiechow_flux(om, *(np.full(nC, U[d]) for d in range(3)), p + om.cc[:, 0] ** 2,
+                           gp, np.full(nC, 0.37), t, ub[0], t, ub[1], t, ub[2], 1.3)
+    assert np.abs(phi2 - ref).max() > 1e-4
+
+
+def test_continuity_closed_cells_and_net_outflow(jb):
+    m, om = jb
+    U = np.array([0.4, 0.1, -0.25])
+    div = O.continuity(om, om.Sf @ U)
+    assert np.abs(div).max() < 1e-15
+    phi = np.random.default_rng(3).standard_normal(om.nFaces)
+    div = O.continuity(om, phi)
+    assert np.isclose(div.sum(), phi[om.nInternalFaces:].sum(), rtol=1e-12)
+    # owner gains +phi, neighbour -phi
+    f = 7
+    e = np.zeros(om.nFaces)
+    e[f] = 1.0
+    d = O.continuity(om, e)
+    assert d[om.owner[f]] == 1.0 and d[om.neigh[f]] == -1.0 and np.count_nonzero(d) == 2
+
+
+def test_nonorth_flux_and_correction_recover_exact_gradient_flux(jb):
+    """For a linear p' (exact gradient) and uniform rAU the compact term plus
+    the non-orthogonal term equals rAU grad(p').S on every face, so the
+    corrected flux is phi* - rAU a.S (internal and Dirichlet-p faces) and
+    phi* on Neumann faces; p' = a0 x vanishes on the Dirichlet plane x = 0."""
+    m, om = jb
+    nC, nF, nIF = om.nCells, om.nFaces, om.nInternalFaces
+    a = np.array([0.9, 0.0, 0.0])
+    pc = om.cc @ a
+    g = np.tile(a, (nC, 1))
+    rau = np.full(nC, 0.8)
+    tP = patch_types(("xmin",))
+    phiNo, divNo = O.pressure_nonorth(om, rau, tP, g)
+    assert np.allclose(divNo, O.continuity(om, phiNo), rtol=0, atol=1e-15)
+    phiStar = np.random.default_rng(4).standard_normal(nF)
+    phi = O.correct_flux(om, rau, tP, np.zeros(nF), phiStar, pc, phiNo)
+    exact = phiStar - 0.8 * (om.Sf @ a)
+    dirich = np.zeros(nF, bool)
+    dirich[nIF:] = om.bPatch[nIF:] == 1
+    keep = np.arange(nF) < nIF
+    assert np.abs(phi[keep] - exact[keep]).max() < 1e-13
+    assert np.abs(phi[dirich] - exact[dirich]).max() < 1e-13
+    neu = ~keep & ~dirich
+    assert np.array_equal(phi[neu], phiStar[neu])
+    # the non-orthogonal term vanishes on an orthogonal mesh
+    m2, om2 = box(4, 4, 4)
+    pn, _ = O.pressure_nonorth(om2, np.ones(om2.nCells), tP, np.tile([1.0, 2.0, 3.0],
+                                                                    (om2.nCells, 1)))
+    assert np.abs(pn).max() < 1e-15
+
+
+# ---------------------------------------------------------------- forces
+def test_wall_force_closed_forms():
+    m, om = box(4, 5, 6, 1.0, 1.0, 1.5)
+    nC = om.nCells
+    p = np.full(nC, 2.5)
+    z = np.zeros(nC)
+    total = np.zeros(3)
+    for k, (name, s, n) in enumerate(om.patches):
+        out = O.patch_forces(om, s, s + n, z, z, z, p, 1e-3, -1)
+        total += out[0:3]
+        assert np.all(out[3:6] == 0.0)
+    assert np.abs(total).max() < 1e-13  # closed surface, uniform pressure
+    name, s, n = om.patches[1]  # xmax, area 1.0 x 1.5, outward +x
+    out = O.patch_forces(om, s, s + n, z, z, z, p, 1e-3, -1)
+    assert np.allclose(out[0:3], [2.5 * 1.5, 0, 0], atol=1e-14)
+    assert np.isclose(out[9], 1.5)
+    # tangential uniform velocity V on xmax: Fv = mu V / (h/2) * area
+    V = 0.3
+    out = O.patch_forces(om, s, s + n, z, np.full(nC, V), z, np.zeros(nC), 1e-3, -1)
+    assert np.allclose(out[3:6], [0, 1e-3 * V / (0.25 / 2) * 1.5, 0], rtol=1e-12)
+    # linear in p
+    o1 = O.patch_forces(om, s, s + n, z, z, z, 3.0 * p, 1e-3, -1)
+    assert np.allclose(o1[0:3], 3.0 * out[0:3] * 0 + 3.0 * O.patch_forces(
+        om, s, s + n, z, z, z, p, 1e-3, -1)[0:3], rtol=1e-14)
+
+
+# ----------------------------------------------------------------- SIMPLE
+def simple_params(nNonOrth=2, mu=1e-3):
+    amg = O.amg_options(maxIter=1000, relTol=0.0, absTol=1e-12)
+    # tolMass = tolVel = -1: never stops early, every requested iteration runs
+    return O.SimpleParams(1.0, mu, 1, nNonOrth, 0.5, 0.3, 1e-8, -1.0, -1.0, 1000, 1e-12, 0.0,
+                          1000, 1e-12, 0.0, 10, amg)
+
+
+def channel_bcs(om, inlet):
+    nF = om.nFaces
+    tU = patch_types(("xmin",))
+    tP = patch_types(("xmax",))
+    ub = [np.zeros(nF) for _ in range(3)]
+    sel = om.bPatch == 1
+    for d in range(3):
+        ub[d][sel] = inlet(om.xf[sel])[:, d]
+    return tU, tP, ub
+
+
+def test_simple_uniform_channel_stays_uniform(jb):
+    """Uniform inflow, slip-like (zero-gradient) side walls, p = 0 outlet:
+    the uniform field is an exact discrete solution on any mesh."""
+    m, om = jb
+    nC = om.nCells
+    tU, tP, ub = channel_bcs(om, lambda x: np.tile([0.4, 0.0, 0.0], (len(x), 1)))
+    r = O.simple_run(om, simple_params(), tU, tP, *ub, np.zeros(om.nFaces), 3, np.full(nC, 0.4),
+                     np.zeros(nC), np.zeros(nC), np.zeros(nC))
+    assert r["done"] == 3
+    assert np.abs(r["u"] - 0.4).max() < 1e-10
+    assert np.abs(r["v"]).max() < 1e-10 and np.abs(r["w"]).max() < 1e-10
+    assert np.abs(r["p"]).max() < 1e-10
+
+
+def test_simple_continuity_on_orthogonal_mesh():
+    """Orthogonal mesh: the non-orthogonal flux vanishes, so max|div phi|
+    after the correction is bounded by the pressure-solve residual.  mu = 0.1
+    keeps the cell Reynolds number near 2 on this coarse mesh (central
+    convection at cell Re >> 2 makes the SIMPLE iteration diverge, in the
+    reference as here)."""
+    m, om = box(12, 4, 4, 3.0, 1.0, 1.0)
+    nC = om.nCells
+    tU, tP, ub = channel_bcs(om, lambda x: np.stack(
+        [16 * 0.45 * x[:, 1] * x[:, 2] * (1 - x[:, 1]) * (1 - x[:, 2]), 0 * x[:, 0],
+         0 * x[:, 0]], axis=1))
+    tU[:] = 1  # no-slip walls, Dirichlet inlet, and
+    tU[1] = 0  # zero-gradient outlet
+    u0 = ub[0][om.bPatch == 1].mean()
+    r = O.simple_run(om, simple_params(0, mu=0.1), tU, tP, *ub, np.zeros(om.nFaces), 4,
+                     np.full(nC, u0), np.zeros(nC), np.zeros(nC), np.zeros(nC))
+    assert r["done"] == 4
+    for st in r["stats"]:
+        assert st.massRes < 1e-11
+    du = [st.duRel for st in r["stats"]]
+    assert du[-1] < 0.5 * du[0]  # the outer iteration converges
+    # outflow balances inflow
+    nIF = om.nInternalFaces
+    bphi = r["phi"][nIF:]
+    bp = om.bPatch[nIF:]
+    assert np.isclose(bphi[bp == 2].sum(), -bphi[bp == 1].sum(), rtol=1e-10)
+
+
+def test_simple_inert_without_driving(jb):
+    m, om = jb
+    nC = om.nCells
+    tU, tP, ub = channel_bcs(om, lambda x: np.zeros((len(x), 3)))
+    r = O.simple_run(om, simple_params(), tU, tP, *ub, np.zeros(om.nFaces), 2, np.zeros(nC),
+                     np.zeros(nC), np.zeros(nC), np.zeros(nC))
+    for k in ("u", "v", "w", "p", "phi"):
+        assert np.all(r[k] == 0.0)
+
+
+# ------------------------------------------------------------------ AMG
+def spd_system(nx, ny, nz, seed=1):
+    m, om = box(nx, ny, nz, jitter=0.2, seed=seed)
+    t, v = bc_arrays(m, dirichlet=("xmin",))
+    vals, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                  np.zeros((om.nCells, 3)))
+    return om, vals
+
+
+@pytest.mark.parametrize("dims", [(7, 6, 5), (11, 11, 10)])
+def test_vcycle_is_exact_when_coarsest_is_the_matrix(dims):
+    """n <= coarseSize: the V-cycle is the dense Cholesky inverse (16- and
+    64-column blocked products for n <= 1024 and n > 1024)."""
+    om, vals = spd_system(*dims)
+    r = np.random.default_rng(0).standard_normal(om.nCells)
+    z, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, r, coarseSize=4096)
+    assert nl == 1
+    xd = spla.spsolve(csr(om, vals).tocsc(), r)
+    assert np.linalg.norm(z - xd) / np.linalg.norm(xd) < 1e-11
+
+
+def test_vcycle_is_linear_and_symmetric():
+    om, vals = spd_system(12, 11, 10)
+    rng = np.random.default_rng(1)
+    r1, r2 = rng.standard_normal(om.nCells), rng.standard_normal(om.nCells)
+    kw = dict(coarseSize=40)
+    z1, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, r1, **kw)
+    z2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, r2, **kw)
+    z12, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, 2.0 * r1 - 3.0 * r2, **kw)
+    assert nl >= 3
+    assert np.linalg.norm(z12 - (2.0 * z1 - 3.0 * z2)) < 1e-12 * np.linalg.norm(z12)
+    assert abs(r2 @ z1 - r1 @ z2) < 1e-12 * abs(r2 @ z1)
+    assert r1 @ z1 > 0 and r2 @ z2 > 0
+
+
+# ------------------------------------------------- LP / LS boundary rules
+@pytest.mark.parametrize("path", ["lp", "ls"])
+def test_linear_field_exact_with_neumann_data(path):
+    """FV is exact for a linear field on an orthogonal box.  LP's Neumann
+    rhs is -gamma g A (scalar_elliptic.cu:121-123) and LS's is +gamma g A
+    (scalar_transport_library.cu:351): with g = -dphi/dn (LP) and
+    g = +dphi/dn (LS) both reproduce phi = a.x; the opposite data do not."""
+    m, om = box(6, 5, 4, 1.0, 0.8, 0.7)
+    a = np.array([0.7, -0.4, 0.25])
+    sgn = -1.0 if path == "lp" else 1.0
+    gam = np.full(om.nFaces, 1.3)
+    for s, exact in ((sgn, True), (-sgn, False)):
+        t, v = bc_arrays(m, dirichlet=("xmin",), value=lambda x: x @ a,
+                         neumann_value=lambda x, n: s * (n @ a))
+        z = np.zeros(om.nCells)
+        if path == "lp":
+            vals, rhs = O.elliptic_assemble(om, gam, z, t, v, np.zeros((om.nCells, 3)))
+        else:
+            vals, rhs = O.transport_assemble(om, np.zeros(om.nFaces), gam, z, z, t, v,
+                                             np.zeros((om.nCells, 3)))
+        x = spla.spsolve(csr(om, vals).tocsc(), rhs)
+        err = np.abs(x - om.cc @ a).max()
+        assert (err < 1e-12) if exact else (err > 1e-2)
+
+
+def test_reference_cell_fixes_pure_neumann_problem():
+    """scalar_elliptic.cu:126-135: the reference row becomes phi_ref = value."""
+    m, om = box(5, 5, 4)
+    a = np.array([0.7, -0.4, 0.25])
+    t, v = bc_arrays(m, dirichlet=(), neumann_value=lambda x, n: -(n @ a))
+    ref = 13
+    vals, rhs = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                    np.zeros((om.nCells, 3)), use_ref=True, ref_cell=ref,
+                                    ref_value=om.cc[ref] @ a)
+    A = csr(om, vals)
+    row = A.getrow(ref)
+    assert row.nnz >= 1 and row[0, ref] == 1.0 and abs(row).sum() == 1.0
+    x = spla.spsolve(A.tocsc(), rhs)
+    assert np.abs(x - om.cc @ a).max() < 1e-12
+
+
+def test_upwind_convection_closed_form():
+    """Uniform +x flow on an orthogonal box: a row holds +U A_x on the
+    diagonal (outflow face) and -U A_x for the upwind (x-1) neighbour."""
+    m, om = box(6, 5, 4, 1.0, 0.8, 0.7)
+    nIF = om.nInternalFaces
+    U = 0.6
+    Ax = 0.8 / 5 * 0.7 / 4
+    vals, _ = O.transport_assemble(om, om.Sf @ np.array([U, 0.0, 0.0]), np.zeros(om.nFaces),
+                                   np.zeros(om.nCells), np.zeros(om.nCells),
+                                   np.zeros(om.nFaces - nIF, np.int32), np.zeros(om.nFaces - nIF),
+                                   np.zeros((om.nCells, 3)), nonorth=False, upwind=True)
+    A = csr(om, vals).toarray()
+    for c in (7, 20, 33):  # interior cells (i = 1..4)
+        expect = np.zeros(om.nCells)
+        expect[c] = U * Ax
+        expect[c - 1] = -U * Ax
+        assert np.allclose(A[c], expect, rtol=1e-13, atol=1e-17)
