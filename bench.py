@@ -94,6 +94,27 @@ def cpu_baseline_simple(m, vel, prs, iters):
                       "case (incl. the first AMG setup) by the serial C oracle"}
 
 
+# ------------------------------------------------------- copy probe ----
+def copy_probe(nbytes):
+    """Three device-to-device copies of nbytes (HIP's blit kernel): reads
+    nbytes and writes nbytes each, the reference point that scales the
+    hardware FETCH_SIZE / WRITE_SIZE counters to bytes.  Plain HIP runtime
+    calls, so no framework kernels run under counter collection."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr_ in (a, b):
+        if hip.hipMalloc(C.byref(ptr_), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: 3 x {nbytes} bytes read + written")
+
+
 # ------------------------------------------------------------- psolve ----
 def psolve_problem(n):
     from anabasis_amd import libpoisson as lp
@@ -174,15 +195,7 @@ def main():
     from anabasis_amd import _lib as L
     L.check(L.lib().an_set_device(0))
     if args.copy_probe_mb:
-        import torch
-        nb = args.copy_probe_mb << 20
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: 3 x {nb} bytes read + written")
-        del src, dst
+        copy_probe(args.copy_probe_mb << 20)
     if args.only_psolve:
         pm, pat, ps = run_psolve(args.pn, args.psolve_reps)
         print(json.dumps({"psolve": ps}), flush=True)
