@@ -382,15 +382,20 @@ __global__ __launch_bounds__(kB) void k_stats(int n, const double* __restrict__ 
   }
 }
 
-// out[0..2] from the block partials of k_stats, summed in block order
-__global__ void k_stats_fin(int nparts, const double* __restrict__ partial,
-                            double* __restrict__ out) {
+// out[0..2] from the block partials of k_stats, summed in block order; the
+// partials are staged in LDS by the whole block first (nparts <= kStatsMax)
+static constexpr int kStatsMax = 1024;
+__global__ __launch_bounds__(256) void k_stats_fin(int nparts, const double* __restrict__ partial,
+                                                   double* __restrict__ out) {
+  __shared__ double sh[3 * kStatsMax];
+  for (int i = threadIdx.x; i < 3 * nparts; i += blockDim.x) sh[i] = partial[i];
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0;
   for (int i = 0; i < nparts; ++i) {
-    t0 += partial[3 * i];
-    t1 += partial[3 * i + 1];
-    t2 = fmax(t2, partial[3 * i + 2]);
+    t0 += sh[3 * i];
+    t1 += sh[3 * i + 1];
+    t2 = fmax(t2, sh[3 * i + 2]);
   }
   out[0] = t0;
   out[1] = t1;
@@ -443,13 +448,23 @@ __global__ void k_force_faces(MeshView m, int f0, int nf, const double* __restri
   o[11] = 1.0;
 }
 
-// ordered serial reduction of the per-face terms (one thread, patch-sized)
-__global__ void k_force_reduce(int nf, const double* __restrict__ t, double* __restrict__ out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// ordered serial reduction of the per-face terms (the reference's serial face
+// order, main.cu:2880-2910): the block stages chunks of kForceChunk faces in
+// LDS with coalesced loads, thread 0 accumulates them in face order
+static constexpr int kForceChunk = 256;
+__global__ __launch_bounds__(256) void k_force_reduce(int nf, const double* __restrict__ t,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[12 * kForceChunk];
   double Fp[3] = {0, 0, 0}, Fv[3] = {0, 0, 0}, F[3] = {0, 0, 0};
   double area = 0.0, cnt = 0.0, minD = 1.0e300, maxD = 0.0, maxUt = 0.0, maxSh = 0.0;
-  for (int i = 0; i < nf; ++i) {
-    const double* o = t + 12 * (size_t)i;
+  for (int c0 = 0; c0 < nf; c0 += kForceChunk) {
+    const int nc = min(kForceChunk, nf - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 12 * nc; i += blockDim.x) sh[i] = t[12 * (size_t)c0 + i];
+    __syncthreads();
+    if (threadIdx.x != 0) continue;
+    for (int i = 0; i < nc; ++i) {
+    const double* o = sh + 12 * i;
     if (o[11] == 0.0) continue;
     for (int d = 0; d < 3; ++d) {
       Fp[d] += o[d];
@@ -462,7 +477,9 @@ __global__ void k_force_reduce(int nf, const double* __restrict__ t, double* __r
     maxD = fmax(maxD, o[8]);
     maxUt = fmax(maxUt, o[9]);
     maxSh = fmax(maxSh, o[10]);
+    }
   }
+  if (threadIdx.x != 0) return;
   if (minD == 1.0e300) minD = 0.0;
   for (int d = 0; d < 3; ++d) {
     out[d] = Fp[d];
@@ -583,7 +600,7 @@ void SimpleSolver::grad(int bcKind, const double* f, DevBuf<double>* out) {
 double SimpleSolver::relchg(const double* a, const double* b) {
   const int G = std::min(grid_for(M.nC, kB), 340);
   k_stats<<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
-  k_stats_fin<<<1, 64, 0, s>>>(G, red.partial.p, scal.p);
+  k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p);
   AN_CHECK_LAUNCH();
   AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));
@@ -593,7 +610,7 @@ double SimpleSolver::relchg(const double* a, const double* b) {
 double SimpleSolver::maxabs(const double* a) {
   const int G = std::min(grid_for(M.nC, kB), 340);
   k_stats<<<G, kB, 0, s>>>(M.nC, a, nullptr, red.partial.p);
-  k_stats_fin<<<1, 64, 0, s>>>(G, red.partial.p, scal.p);
+  k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p);
   AN_CHECK_LAUNCH();
   AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));
@@ -784,7 +801,7 @@ void SimpleSolver::patch_forces(int patch, int normalSign, double* out) {
                                                   par.mu, normalSign, t.p);
     AN_CHECK_LAUNCH();
   }
-  k_force_reduce<<<1, 64, 0, s>>>(nf, t.p, o.p);
+  k_force_reduce<<<1, 256, 0, s>>>(nf, t.p, o.p);
   AN_CHECK_LAUNCH();
   AN_HIP(hipMemcpyAsync(out, o.p, 15 * sizeof(double), hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));

@@ -17,8 +17,10 @@ b ~ N(0,1) mean-projected, x0 = 0, solved to ||r||/||b|| <= 1e-10 with FP64
 PCG+AMG and with FP32-inner / FP64-refined PCG+AMG.
 
 Roofline: every Krylov SpMV of the timed FP64 pressure solves is bracketed by
-HIP events on the solver stream; algorithmic bytes per launch =
-12 nnz + 4 (nC+1) + 16 nC.
+HIP events recorded around the kernel itself (hipExtLaunchKernel) on the
+solver stream; algorithmic bytes per launch = 12 nnz + 4 (nC+1) + 16 nC;
+traffic = HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+recorded in profiles/k_spmv_psolve128_traffic.json.
 """
 import argparse
 import json
@@ -170,6 +172,20 @@ def run_psolve(n, reps):
     return m, pat, res
 
 
+def spmv_traffic(alg_bytes):
+    """HBM bytes per launch of the FP64 PCG SpMV on the 128^3 matrix, measured
+    by the FETCH_SIZE / WRITE_SIZE counter runs recorded in profiles/ (scaled
+    by a copy kernel of known size); None when not recorded for this size."""
+    f = os.path.join(ROOT, "profiles", "k_spmv_psolve128_traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        d = json.load(fh)
+    if d.get("algorithmic_bytes_per_launch") != alg_bytes:
+        return None
+    return d["traffic_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,8 +240,9 @@ def main():
         alg = 12 * nnz + 4 * (nC + 1) + 16 * nC
         t = ps["fp64"]["spmv_avg_s"]
         achieved = alg / t / 1e9
+        traffic = spmv_traffic(alg)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_spmv<double,double,double,0,1> (PCG SpMV + fused p.q), "
                           f"{args.pn}^3 pressure matrix",
                 "alg_bytes_per_launch": alg, "avg_launch_s": t,

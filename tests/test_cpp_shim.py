@@ -63,7 +63,7 @@ def test_cpp_mms_driver_matches_python_api(gpu, tmp_path):
     from anabasis_amd import libpoisson as lp
     exe = build_driver(tmp_path)
     errs = []
-    for n in (4, 8, 16):
+    for n in (8, 16, 32):
         m = lp.Mesh.tetcube(n)
         d = tmp_path / f"tet{n}"
         m.write_polymesh(d)
