@@ -120,6 +120,8 @@ SIGNATURES = {
     "an_solver_destroy": (_I, [_P]),
     "an_bicgstab_solve": (_I, [_I, _P, _P, C.c_int64, _P, _P, _P, _I, _D, _D, _I,
                                C.POINTER(SolveInfo)]),
+    "an_bicgstab_solve3": (_I, [_I, _P, _P, C.c_int64, _P, _P, _P, _I, _D, _D, _I,
+                                C.POINTER(SolveInfo)]),
     "an_elliptic_default_options": (None, [C.POINTER(EllipticOptions)]),
     "an_solve_scalar_elliptic": (_I, [_P, _P, _P, _P, _P, C.POINTER(EllipticOptions), _P, _I,
                                       C.POINTER(SolveInfo)]),

@@ -1,0 +1,434 @@
+// bicg3.hip -- BiCGSTAB + Jacobi for three right-hand sides sharing one
+// matrix (the u/v/w momentum systems of generic_simple_v1, which reuses the
+// x-momentum matrix for y and z: main.cu:3401-3405).
+//
+// Vectors are interleaved, element (i, c) at 3i + c, so one SpMV streams the
+// matrix once and gathers 24 contiguous bytes per nonzero for all three
+// columns.  Every column keeps its own scalars (slot-major: sc[3*slot + c])
+// and its own stop test; a converged column is frozen by the active mask.
+//
+// Per column the arithmetic is the single-RHS bicgstab_device's, operation
+// for operation: same kernels' element->block maps (grid-stride loops over
+// rows, CSR row blocks), the same shuffle trees and the same index-order
+// finalize.  A column's iterates, scalars and iteration count are therefore
+// bit-identical to a separate solve of that column.
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "solver.hpp"
+
+namespace an {
+
+static constexpr int kB = 256;
+static constexpr int kW = kB / 64;
+static constexpr int kCap = kStreamCap;
+
+__device__ __forceinline__ double wsum3(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// three block sums with the single-RHS tree (valid in thread 0)
+__device__ __forceinline__ void block_sum3(double v[3], double (*sh)[kW]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = wsum3(v[c]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh[c][w] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[c][i];
+    v[c] = t;
+  }
+}
+
+__device__ __forceinline__ void store3(const double v[3], int blk, double* __restrict__ partial) {
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) partial[3 * blk + c] = v[c];
+}
+
+__global__ void k3_pack(int n, const double* __restrict__ a0, const double* __restrict__ a1,
+                        const double* __restrict__ a2, double* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  X[3 * i] = a0[i];
+  X[3 * i + 1] = a1[i];
+  X[3 * i + 2] = a2[i];
+}
+
+__global__ void k3_unpack(int n, const double* __restrict__ X, double* __restrict__ a0,
+                          double* __restrict__ a1, double* __restrict__ a2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a0[i] = X[3 * i];
+  a1[i] = X[3 * i + 1];
+  a2[i] = X[3 * i + 2];
+}
+
+__global__ void k3_zero_col(int n, double* __restrict__ X, int c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) X[3 * i + c] = 0.0;
+}
+
+// partial[3 blk + c] = block's share of sum_i A[3i+c] B[3i+c] (k_dot's map)
+__global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A,
+                                             const double* __restrict__ B,
+                                             double* __restrict__ partial) {
+  __shared__ double sh[3][kW];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] += A[3 * i + c] * B[3 * i + c];
+  block_sum3(v, sh);
+  store3(v, blockIdx.x, partial);
+}
+
+// MODE 0: Y = A X; MODE 1: Y = B - A X.  DOT 0: none; 1: dot(DW, Y); 2: dot(Y, Y)
+// (per column, k_spmv's row blocks and trees).  Columns outside 'mask' are
+// neither written nor reduced.
+template <int MODE, int DOT>
+__global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ rbs,
+                                              const int* __restrict__ rowOff,
+                                              const int* __restrict__ cols,
+                                              const double* __restrict__ vals,
+                                              const double* __restrict__ X,
+                                              const double* __restrict__ B,
+                                              double* __restrict__ Y,
+                                              const double* __restrict__ DW, int mask,
+                                              double* __restrict__ partial) {
+  __shared__ double prod[kW][kCap];
+  __shared__ double sh[3][kW];
+  const int lb = xcd_block(gridDim.x);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
+  const int row = q.r0 + lane;
+  int kb = 0, ke = 0;
+  if (row < q.r1) {
+    kb = rowOff[row];
+    ke = rowOff[row + 1];
+  }
+  double dv[3] = {0.0, 0.0, 0.0};
+  const bool stream = q.stream && q.k1 > q.k0;
+  int cidx[kStreamPer];
+  double v[kStreamPer], x3[kStreamPer][3];
+  if (stream) {
+#pragma unroll
+    for (int u = 0; u < kStreamPer; ++u) {
+      const int k = min(q.k0 + lane + 64 * u, q.k1 - 1);
+      cidx[u] = cols[k];
+      v[u] = vals[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamPer; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x3[u][c] = X[3 * cidx[u] + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((mask >> c) & 1)) continue;  // wave-uniform
+    if (stream)
+#pragma unroll
+      for (int u = 0; u < kStreamPer; ++u) {
+        const int k = q.k0 + lane + 64 * u;
+        if (k < q.k1) prod[w][k - q.k0] = v[u] * x3[u][c];
+      }
+    wave_sync();
+    if (row < q.r1) {
+      double s = 0.0;
+      if (stream)
+        for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
+      else
+        for (int k = kb; k < ke; ++k) s += vals[k] * X[3 * cols[k] + c];
+      const double yv = (MODE == 1) ? B[3 * row + c] - s : s;
+      Y[3 * row + c] = yv;
+      if (DOT == 1) dv[c] = DW[3 * row + c] * yv;
+      if (DOT == 2) dv[c] = yv * yv;
+    }
+    wave_sync();  // the slice is reused by the next column
+  }
+  if (DOT) {
+    block_sum3(dv, sh);
+    store3(dv, lb, partial);
+  }
+}
+
+// p = r + beta (p - omega v); ph = dinv p   (k_bicg_p per column)
+__global__ __launch_bounds__(kB) void k3_bicg_p(int n, const double* __restrict__ R,
+                                                double* __restrict__ P,
+                                                const double* __restrict__ V,
+                                                const double* __restrict__ dinv,
+                                                double* __restrict__ PH,
+                                                const double* __restrict__ sc, int rhoNew,
+                                                int rhoOld, int first, int mask) {
+  double beta[3], om[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    beta[c] = first ? 0.0
+                    : (sc[3 * rhoNew + c] / sc[3 * rhoOld + c]) * (sc[3 * S_AL + c] / sc[3 * S_OM + c]);
+    om[c] = first ? 0.0 : sc[3 * S_OM + c];
+  }
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    const double di = dinv[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      const int e = 3 * i + c;
+      const double pi = first ? R[e] : R[e] + beta[c] * (P[e] - om[c] * V[e]);
+      P[e] = pi;
+      PH[e] = di * pi;
+    }
+  }
+}
+
+// alpha = rhoNew / r0v; s = r - alpha v; sh = dinv s; ss = s.s
+__global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict__ R,
+                                                const double* __restrict__ V,
+                                                const double* __restrict__ dinv,
+                                                double* __restrict__ S, double* __restrict__ SH,
+                                                double* __restrict__ sc, int rhoNew, int mask,
+                                                double* __restrict__ partial) {
+  __shared__ double sh[3][kW];
+  double alpha[3], acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) alpha[c] = sc[3 * rhoNew + c] / sc[3 * S_R0V + c];
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    const double di = dinv[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      const int e = 3 * i + c;
+      const double si = R[e] - alpha[c] * V[e];
+      S[e] = si;
+      SH[e] = di * si;
+      acc[c] += si * si;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if ((mask >> c) & 1) sc[3 * S_AL + c] = alpha[c];
+  block_sum3(acc, sh);
+  store3(acc, blockIdx.x, partial);
+}
+
+// partial[6 blk + c] = t.s, partial[6 blk + 3 + c] = t.t   (k_dot2 per column)
+__global__ __launch_bounds__(kB) void k3_dot2(int n, const double* __restrict__ T,
+                                              const double* __restrict__ S,
+                                              double* __restrict__ partial) {
+  __shared__ double sh[3][kW];
+  double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double t = T[3 * i + c];
+      a[c] += t * S[3 * i + c];
+      b[c] += t * t;
+    }
+  block_sum3(a, sh);
+  block_sum3(b, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      partial[6 * blockIdx.x + c] = a[c];
+      partial[6 * blockIdx.x + 3 + c] = b[c];
+    }
+}
+
+struct Tol3 {
+  double t[3];
+};
+
+// omega = ts/tt; x += alpha ph (+ omega sh); r = s (- omega t); rr = r.r
+__global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
+                                                double* __restrict__ R,
+                                                const double* __restrict__ PH,
+                                                const double* __restrict__ SH,
+                                                const double* __restrict__ S,
+                                                const double* __restrict__ T,
+                                                double* __restrict__ sc, Tol3 tol, int mask,
+                                                double* __restrict__ partial) {
+  __shared__ double sh[3][kW];
+  double alpha[3], omega[3], acc[3] = {0.0, 0.0, 0.0};
+  bool convS[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    alpha[c] = sc[3 * S_AL + c];
+    const double tt = sc[3 * S_TT + c];
+    omega[c] = (tt > 0.0) ? sc[3 * S_TS + c] / tt : 0.0;
+    convS[c] = sqrt(sc[3 * S_SS + c]) <= tol.t[c];
+  }
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (!((mask >> c) & 1)) continue;
+      const int e = 3 * i + c;
+      double ri;
+      if (convS[c]) {
+        X[e] += alpha[c] * PH[e];
+        ri = S[e];
+      } else {
+        X[e] += alpha[c] * PH[e] + omega[c] * SH[e];
+        ri = S[e] - omega[c] * T[e];
+      }
+      R[e] = ri;
+      acc[c] += ri * ri;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if ((mask >> c) & 1) sc[3 * S_OM + c] = omega[c];
+  block_sum3(acc, sh);
+  store3(acc, blockIdx.x, partial);
+}
+
+__global__ void k3_dinv(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                        const double* __restrict__ va, double* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k)
+    if (ci[k] == i) d = va[k];
+  dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
+}
+
+void Bicg3Work::ensure(int nn, int nbSpmv, hipStream_t st) {
+  const int need = std::max(nbSpmv, 1024) * 2 + 16;  // 6 doubles per block fit in 4 * need
+  if (n == nn && scal.p && red.cap >= need) return;
+  n = nn;
+  red.init(need, st);
+  scal.alloc(3 * S_COUNT);
+  dinv.alloc(n);
+  for (DevBuf<double>* b : {&x, &b3, &r, &r0, &p, &v, &ph, &s_, &sh, &t}) b->alloc(3 * (size_t)n);
+  if (!hs) AN_HIP(hipHostMalloc((void**)&hs, sizeof(double) * 3 * S_COUNT));
+}
+
+Bicg3Work::~Bicg3Work() {
+  if (hs) (void)hipHostFree(hs);
+}
+
+an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], double* const x[3],
+                           int maxIter, double relTol, double absTol, Bicg3Work& w,
+                           hipStream_t s, an_solve_info info[3], int activeIn,
+                           hipEvent_t evStart, hipEvent_t evStop) {
+  const int n = A.n;
+  auto t0 = std::chrono::steady_clock::now();
+  const int nbS = grid_for(A.rb.count, kW);
+  if (A.n > 0 && A.rb.count == 0) throw Error(7, "CSR row blocks not built");
+  w.ensure(n, nbS, s);
+  const int G = std::min(grid_for(n, kB), 1024);
+  const int Gd = std::min(grid_for(n, kB), 1024);  // dot(): red_blocks
+  const int gN = grid_for(n, kB);
+  const int2* rbs = A.rb.start.p;
+  double* sc = w.scal.p;
+  double* part = w.red.partial.p;
+  an_solve_info inf[3] = {};
+  k3_dinv<<<gN, kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
+  k3_pack<<<gN, kB, 0, s>>>(n, b[0], b[1], b[2], w.b3.p);
+  k3_pack<<<gN, kB, 0, s>>>(n, x[0], x[1], x[2], w.x.p);
+  AN_CHECK_LAUNCH();
+  int mask = activeIn & 7;
+  // ||b||, r = b - A x, ||r||
+  k3_dot<<<Gd, kB, 0, s>>>(n, w.b3.p, w.b3.p, part);
+  finalize(w.red, Gd, 3, sc + 3 * S_BB, s);
+  k3_spmv<1, 2><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.x.p,
+                                   w.b3.p, w.r.p, nullptr, 7, part);
+  finalize(w.red, nbS, 3, sc + 3 * S_RR, s);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipMemcpyAsync(w.hs, sc, sizeof(double) * 3 * S_COUNT, hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  double bnorm[3], rn[3];
+  Tol3 tol;
+  bool conv[3] = {false, false, false};
+  for (int c = 0; c < 3; ++c) {
+    bnorm[c] = std::sqrt(w.hs[3 * S_BB + c]);
+    rn[c] = std::sqrt(w.hs[3 * S_RR + c]);
+    tol.t[c] = std::fmax(relTol * bnorm[c], absTol);
+    if (!((mask >> c) & 1)) continue;
+    if (bnorm[c] == 0.0) {  // HYPRE convention: b = 0 -> x = 0
+      k3_zero_col<<<gN, kB, 0, s>>>(n, w.x.p, c);
+      conv[c] = true;
+      mask &= ~(1 << c);
+    } else if (rn[c] <= tol.t[c]) {
+      conv[c] = true;
+      mask &= ~(1 << c);
+    }
+  }
+  const int solving = mask;
+  int it = 0;
+  if (mask) {
+    AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * 3 * (size_t)n,
+                          hipMemcpyDeviceToDevice, s));
+    for (it = 1; it <= maxIter && mask; ++it) {
+      const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
+      const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+      k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, part);
+      finalize(w.red, Gd, 3, sc + 3 * rhoNew, s);
+      k3_bicg_p<<<G, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
+                                 it == 1 ? 1 : 0, mask);
+      if (evStart && it == 1)
+        hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
+                              A.rb.count, rbs, (const int*)A.rowOff.p, (const int*)A.cols.p,
+                              (const double*)A.vals.p, (const double*)w.ph.p,
+                              (const double*)nullptr, w.v.p, (const double*)w.r0.p, mask, part);
+      else
+        k3_spmv<0, 1><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.ph.p,
+                                         nullptr, w.v.p, w.r0.p, mask, part);
+      finalize(w.red, nbS, 3, sc + 3 * S_R0V, s);
+      k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, sc, rhoNew, mask,
+                                 part);
+      finalize(w.red, G, 3, sc + 3 * S_SS, s);
+      k3_spmv<0, 0><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
+                                       nullptr, w.t.p, nullptr, mask, nullptr);
+      k3_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, part);
+      finalize(w.red, G, 6, sc + 3 * S_TS, s);  // S_TS[0..2], S_TT[0..2]
+      k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, sc, tol, mask,
+                                 part);
+      finalize(w.red, G, 3, sc + 3 * S_RR, s);
+      AN_CHECK_LAUNCH();
+      AN_HIP(hipMemcpyAsync(w.hs, sc, sizeof(double) * 3 * S_COUNT, hipMemcpyDeviceToHost, s));
+      AN_HIP(hipStreamSynchronize(s));
+      const double* h = w.hs;
+      for (int c = 0; c < 3; ++c) {
+        if (!((mask >> c) & 1)) continue;
+        rn[c] = std::sqrt(h[3 * S_RR + c]);
+        inf[c].iterations = it;
+        if (std::sqrt(h[3 * S_SS + c]) <= tol.t[c] || rn[c] <= tol.t[c]) {
+          conv[c] = true;
+          mask &= ~(1 << c);
+        } else if (h[3 * rhoNew + c] == 0.0 || h[3 * S_R0V + c] == 0.0 ||
+                   h[3 * S_OM + c] == 0.0 || !std::isfinite(rn[c])) {
+          mask &= ~(1 << c);
+        }
+      }
+    }
+  }
+  k3_unpack<<<gN, kB, 0, s>>>(n, w.x.p, x[0], x[1], x[2]);
+  AN_CHECK_LAUNCH();
+  AN_HIP(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  an_status st = AN_OK;
+  for (int c = 0; c < 3; ++c) {
+    if (!((activeIn >> c) & 1)) continue;
+    if (!((solving >> c) & 1)) inf[c].iterations = 0;
+    inf[c].converged = conv[c] ? 1 : 0;
+    inf[c].finalResNorm = bnorm[c] == 0.0 ? 0.0 : rn[c];
+    inf[c].finalRelResNorm = bnorm[c] == 0.0 ? 0.0 : rn[c] / bnorm[c];
+    inf[c].solveSeconds = secs;
+    if (!conv[c]) st = AN_ENOCONV;
+  }
+  if (info)
+    for (int c = 0; c < 3; ++c) info[c] = inf[c];
+  return st;
+}
+
+}  // namespace an

@@ -535,6 +535,61 @@ an_status an_bicgstab_solve(int n, const int* rowOffsets, const int* cols, int64
   AN_CATCH
 }
 
+an_status an_bicgstab_solve3(int n, const int* rowOffsets, const int* cols, int64_t nnz,
+                             const double* values, const double* const* b, double* const* x,
+                             int maxIter, double relTol, double absTol, an_memspace mem,
+                             an_solve_info* info) {
+  AN_TRY
+  if (n <= 0 || !rowOffsets || !cols || !values || !b || !x) throw Error(AN_EINVAL, "NULL argument");
+  for (int c = 0; c < 3; ++c)
+    if (!b[c] || !x[c]) throw Error(AN_EINVAL, "NULL right-hand side or solution");
+  DevCSR<double> A;
+  A.n = n;
+  A.nnz = nnz;
+  A.rowOff.alloc((size_t)n + 1);
+  A.cols.alloc((size_t)nnz);
+  A.vals.alloc((size_t)nnz);
+  hipMemcpyKind k = kind_in(mem);
+  AN_HIP(hipMemcpy(A.rowOff.p, rowOffsets, sizeof(int) * ((size_t)n + 1), k));
+  AN_HIP(hipMemcpy(A.cols.p, cols, sizeof(int) * (size_t)nnz, k));
+  AN_HIP(hipMemcpy(A.vals.p, values, sizeof(double) * (size_t)nnz, k));
+  A.finish_pattern(0);
+  DevBuf<double> bh[3], xh[3];
+  const double* bd[3];
+  double* xd[3];
+  for (int c = 0; c < 3; ++c) {
+    bd[c] = b[c];
+    xd[c] = x[c];
+    if (mem == AN_HOST) {
+      bh[c].upload(b[c], (size_t)n, 0);
+      xh[c].upload(x[c], (size_t)n, 0);
+      bd[c] = bh[c].p;
+      xd[c] = xh[c].p;
+    }
+  }
+  hipEvent_t e0, e1;
+  AN_HIP(hipEventCreate(&e0));
+  AN_HIP(hipEventCreate(&e1));
+  Bicg3Work w;
+  an_solve_info inf[3] = {};
+  an_status st = bicgstab3_device(A, bd, xd, maxIter, relTol, absTol, w, 0, inf, 7, e0, e1);
+  float ms = 0.f;
+  const bool timed = inf[0].iterations + inf[1].iterations + inf[2].iterations > 0;
+  if (timed) AN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int c = 0; c < 3; ++c) {
+    inf[c].spmvCalls = timed ? 1 : 0;
+    inf[c].spmvAvgSeconds = ms * 1e-3;
+    if (mem == AN_HOST) AN_HIP(hipMemcpy(x[c], xd[c], sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  if (info)
+    for (int c = 0; c < 3; ++c) info[c] = inf[c];
+  if (st == AN_ENOCONV) set_last_error("BiCGSTAB did not converge for every right-hand side");
+  return st;
+  AN_CATCH
+}
+
 // ------------------------------------------------- libpoisson / LS ----
 void an_elliptic_default_options(an_elliptic_options* o) {
   if (!o) return;
@@ -684,7 +739,7 @@ void an_simple_default_params(an_simple_params* p) {
   p->pRelTol = 0.0;
   p->pAmgRebuildEvery = 10;
   p->precision = 0;
-  p->fuseMomentumRhs = 0;
+  p->fuseMomentumRhs = 1;  // bit-identical to the component-by-component order
   default_pcg_options(&p->pressure);
 }
 

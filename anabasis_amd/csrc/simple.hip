@@ -29,16 +29,6 @@ namespace an {
 static constexpr int kB = 256;
 
 // ----------------------------------------------------------- momentum ----
-struct MomArgs {
-  const int* bcT;          // velocity BC type per boundary face
-  const double* qB;        // component BC value per boundary face
-  const double *uB, *vB, *wB;
-  const double *gqx, *gqy, *gqz;  // gradient of the current iterate
-  const double *uC, *vC, *wC;     // convecting velocity (old fields)
-  const double* gpc;       // pressure-gradient component
-  const double* qOld;      // iterate used by the relaxation
-  double rho, mu, corrPsi, uRelax;
-};
 
 template <bool FULL>
 __global__ __launch_bounds__(kB) void k_mom(MeshView m, MomArgs a, double* __restrict__ vals,
@@ -542,6 +532,12 @@ SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int*
     g[k].alloc(nC);
   }
   qNew.alloc(nC);
+  if (par.fuseMomentumRhs)
+    for (int c = 0; c < 3; ++c) {
+      for (int d = 0; d < 3; ++d) g3[c][d].alloc(nC);
+      rhs3[c].alloc(nC);
+      qNew3[c].alloc(nC);
+    }
   p.alloc(nC);
   p.zero(s);
   pOld.alloc(nC);
@@ -618,6 +614,28 @@ double SimpleSolver::maxabs(const double* a) {
 }
 
 // solve_scalar_component_with_nonorth (main.cu:3350-3384)
+MomArgs SimpleSolver::mom_args(int comp, const DevBuf<double>* grad) const {
+  MomArgs a;
+  a.bcT = bcU.p;
+  a.qB = bcUv[comp].p;
+  a.uB = bcUv[0].p;
+  a.vB = bcUv[1].p;
+  a.wB = bcUv[2].p;
+  a.gqx = grad[0].p;
+  a.gqy = grad[1].p;
+  a.gqz = grad[2].p;
+  a.uC = qOld[0].p;
+  a.vC = qOld[1].p;
+  a.wC = qOld[2].p;
+  a.gpc = gp[comp].p;
+  a.qOld = q[comp].p;
+  a.rho = par.rho;
+  a.mu = par.mu;
+  a.corrPsi = 1.0;
+  a.uRelax = par.uRelax;
+  return a;
+}
+
 void SimpleSolver::momentum_component(int comp, an_simple_stats* st) {
   const int nC = M.nC;
   const int nVel = std::max(par.nVelNonOrthCorr, 0) + 1;
@@ -627,24 +645,7 @@ void SimpleSolver::momentum_component(int comp, an_simple_stats* st) {
     // velocity BC values of this component (types identical for u, v, w)
     BCView bc{bcU.p, bcUv[comp].p};
     launch_lsq_gradient(mv, 0, bc, q[comp].p, g[0].p, g[1].p, g[2].p, s);
-    MomArgs a;
-    a.bcT = bcU.p;
-    a.qB = bcUv[comp].p;
-    a.uB = bcUv[0].p;
-    a.vB = bcUv[1].p;
-    a.wB = bcUv[2].p;
-    a.gqx = g[0].p;
-    a.gqy = g[1].p;
-    a.gqz = g[2].p;
-    a.uC = qOld[0].p;
-    a.vC = qOld[1].p;
-    a.wC = qOld[2].p;
-    a.gpc = gp[comp].p;
-    a.qOld = q[comp].p;
-    a.rho = par.rho;
-    a.mu = par.mu;
-    a.corrPsi = 1.0;
-    a.uRelax = par.uRelax;
+    const MomArgs a = mom_args(comp, g);
     mark(0);
     if (comp == 0 && it == 1)
       k_mom<true><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs.p, rAU.p);
@@ -668,6 +669,56 @@ void SimpleSolver::momentum_component(int comp, an_simple_stats* st) {
   }
 }
 
+// u, v and w together (main.cu:3350-3405 run component by component; every
+// component's passes read only the old fields, its own iterate and the
+// matrix built in the first x pass, so the passes of the three components
+// can advance in lock-step).  Each component keeps its own gradient, rhs,
+// BiCGSTAB scalars, stop test and corrTol exit, so its result is bit-for-bit
+// the component-by-component result; the three solves share one matrix
+// stream per SpMV.
+void SimpleSolver::momentum_fused(an_simple_stats* st) {
+  const int nC = M.nC;
+  const int nVel = std::max(par.nVelNonOrthCorr, 0) + 1;
+  for (int c = 0; c < 3; ++c)
+    AN_HIP(hipMemcpyAsync(q[c].p, qOld[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+  int active = 7;
+  for (int it = 1; it <= nVel && active; ++it) {
+    mark(0);
+    for (int c = 0; c < 3; ++c) {
+      if (!((active >> c) & 1)) continue;
+      BCView bc{bcU.p, bcUv[c].p};
+      launch_lsq_gradient(mv, 0, bc, q[c].p, g3[c][0].p, g3[c][1].p, g3[c][2].p, s);
+    }
+    for (int c = 0; c < 3; ++c) {  // x first: its first pass builds the shared matrix
+      if (!((active >> c) & 1)) continue;
+      const MomArgs a = mom_args(c, g3[c]);
+      if (c == 0 && it == 1)
+        k_mom<true><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs3[c].p, rAU.p);
+      else
+        k_mom<false><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs3[c].p, rAU.p);
+      AN_CHECK_LAUNCH();
+      AN_HIP(hipMemcpyAsync(qNew3[c].p, q[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    }
+    mark(1);
+    const double* b3[3] = {rhs3[0].p, rhs3[1].p, rhs3[2].p};
+    double* x3[3] = {qNew3[0].p, qNew3[1].p, qNew3[2].p};
+    an_solve_info info[3] = {};
+    (void)bicgstab3_device(Amom, b3, x3, par.velMaxit, par.velRelTol, par.velTol, bw3, s, info,
+                           active);
+    mark(2);
+    AN_HIP(hipStreamSynchronize(s));
+    tAcc[0] += since(0, 1);
+    tAcc[1] += since(1, 2);
+    for (int c = 0; c < 3; ++c) {
+      if (!((active >> c) & 1)) continue;
+      st->bicgstabIts[c] = info[c].iterations;
+      const double rc = relchg(qNew3[c].p, q[c].p);
+      AN_HIP(hipMemcpyAsync(q[c].p, qNew3[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+      if (rc < par.corrTol) active &= ~(1 << c);
+    }
+  }
+}
+
 an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut) {
   const int nC = M.nC, nF = M.nF;
   an_status status = AN_OK;
@@ -683,7 +734,10 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
     mark(3);
     grad(1, pOld.p, gp);  // grad(pOld) [bcP] (main.cu:3393-3397)
     mark(4);
-    for (int c = 0; c < 3; ++c) momentum_component(c, &st);
+    if (par.fuseMomentumRhs)
+      momentum_fused(&st);
+    else
+      for (int c = 0; c < 3; ++c) momentum_component(c, &st);
     // pressure matrix + AMG (re)setup (main.cu:3407-3411)
     mark(5);
     k_p_asm<<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, rAU.p, pvals.p);

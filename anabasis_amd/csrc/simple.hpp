@@ -9,6 +9,18 @@
 
 namespace an {
 
+// inputs of the momentum assembly kernel (one velocity component)
+struct MomArgs {
+  const int* bcT;          // velocity BC type per boundary face
+  const double* qB;        // component BC value per boundary face
+  const double *uB, *vB, *wB;
+  const double *gqx, *gqy, *gqz;  // gradient of the current iterate
+  const double *uC, *vC, *wC;     // convecting velocity (old fields)
+  const double* gpc;       // pressure-gradient component
+  const double* qOld;      // iterate used by the relaxation
+  double rho, mu, corrPsi, uRelax;
+};
+
 class SimpleSolver {
  public:
   SimpleSolver(DevMesh& mesh, const an_simple_params& par, const int* patchUType,
@@ -41,6 +53,10 @@ class SimpleSolver {
   DevBuf<double> pvals;
   std::unique_ptr<PcgSolver> psolver;
   BicgWork bw;
+  // fused u/v/w momentum (par.fuseMomentumRhs): per-component gradients,
+  // right-hand sides and new iterates
+  Bicg3Work bw3;
+  DevBuf<double> g3[3][3], rhs3[3], qNew3[3];
   Reducer red;
   DevBuf<double> scal;
   PinnedScalars hs;
@@ -49,6 +65,8 @@ class SimpleSolver {
 
   void grad(int bcKind, const double* f, DevBuf<double>* out);
   void momentum_component(int comp, an_simple_stats* st);
+  void momentum_fused(an_simple_stats* st);
+  MomArgs mom_args(int comp, const DevBuf<double>* grad) const;
   double relchg(const double* a, const double* b);  // device reduction
   double maxabs(const double* a);
   void mark(int i) { AN_HIP(hipEventRecord(ev[i], s)); }

@@ -66,6 +66,25 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
                           double relTol, double absTol, BicgWork& w, hipStream_t s,
                           an_solve_info* info);
 
+// Three right-hand sides sharing one matrix (interleaved work vectors); per
+// column bit-identical to bicgstab_device.  activeMask selects the columns
+// to solve; evStart/evStop (optional) time the first fused SpMV.
+struct Bicg3Work {
+  Reducer red;
+  DevBuf<double> scal, dinv, x, b3, r, r0, p, v, ph, s_, sh, t;
+  double* hs = nullptr;  // pinned copy of scal
+  int n = 0;
+  void ensure(int n, int nbSpmv, hipStream_t s);
+  Bicg3Work() = default;
+  ~Bicg3Work();
+  Bicg3Work(const Bicg3Work&) = delete;
+  Bicg3Work& operator=(const Bicg3Work&) = delete;
+};
+an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], double* const x[3],
+                           int maxIter, double relTol, double absTol, Bicg3Work& w,
+                           hipStream_t s, an_solve_info info[3], int activeMask = 7,
+                           hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
+
 void default_pcg_options(an_pcg_options* o);
 
 }  // namespace an

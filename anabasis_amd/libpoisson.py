@@ -346,6 +346,23 @@ def bicgstab(rowOffsets, cols, values, b, x0=None, maxIter=1000, relTol=0.0, abs
     return x, info
 
 
+def bicgstab3(rowOffsets, cols, values, bs, x0s=None, maxIter=1000, relTol=0.0, absTol=1e-10,
+              allow_noconv=False):
+    """Three right-hand sides sharing one matrix (an_bicgstab_solve3): returns
+    ([x0, x1, x2], [info0, info1, info2])."""
+    ro, ci = i32(rowOffsets), i32(cols)
+    n = ro.size - 1
+    bs = [f64(b) for b in bs]
+    xs = [np.zeros(n) if x0s is None else f64(x0s[c]).copy() for c in range(3)]
+    bp = (C.c_void_p * 3)(*[b.ctypes.data for b in bs])
+    xp = (C.c_void_p * 3)(*[x.ctypes.data for x in xs])
+    info = (L.SolveInfo * 3)()
+    st = lib().an_bicgstab_solve3(n, ptr(ro), ptr(ci), ci.size, ptr(f64(values)), bp, xp,
+                                  maxIter, relTol, absTol, L.AN_HOST, info)
+    check(st, allow=(L.AN_ENOCONV,) if allow_noconv else ())
+    return xs, list(info)
+
+
 @dataclass
 class EllipticOptions:
     """LP/include/poisson_library.h:5-13 (hypre -> PCG/AMG options)."""

@@ -96,6 +96,50 @@ def cpu_baseline_simple(m, vel, prs, iters):
                       "case (incl. the first AMG setup) by the serial C oracle"}
 
 
+# ------------------------------------------------------ momentum 3-RHS ----
+def run_momentum3(n):
+    """u/v/w momentum-type systems on an n^3 jittered hex box sharing one
+    matrix (central convection of a uniform flux rho U.S plus diffusion
+    mu = 1e-3, Dirichlet inflow faces), solved as three separate BiCGSTAB
+    solves and as one fused 3-RHS solve; reports both times, whether the
+    results are bitwise equal, and the fused SpMV's bandwidth
+    (12 nnz + 4(nC+1) + 48 nC algorithmic bytes)."""
+    from anabasis_amd import libpoisson as lp
+    t0 = time.time()
+    m = lp.Mesh.hexbox(n, n, n, 1.0, 1.0, 1.0, jitter=0.2, seed=SEED + 1)
+    pat = lp.build_scalar_pattern(m)
+    U = np.array([1.0, 0.5, 0.25])
+    nIF, nb = m.nInternalFaces, m.nBoundaryFaces
+    t = np.zeros(nb, np.int32)
+    t[(m.Sf[nIF:] @ U) < 0] = 1  # inflow faces Dirichlet
+    rng = np.random.default_rng(SEED + 2)
+    vals, rhs = lp.assemble_scalar_transport_system(m, m.Sf @ U, np.full(m.nFaces, 1e-3),
+                                                    np.zeros(m.nCells), np.zeros(m.nCells), t,
+                                                    rng.standard_normal(nb),
+                                                    np.zeros((m.nCells, 3)), False, False,
+                                                    nnz=pat.nnz)
+    bs = [rhs, rng.standard_normal(m.nCells), rng.standard_normal(m.nCells)]
+    log(f"momentum3 mesh {n}^3: nC={m.nCells} nnz={pat.nnz} ({time.time() - t0:.1f}s)")
+    kw = dict(absTol=1e-10, maxIter=2000)
+    lp.bicgstab3(pat.rowOffsets, pat.cols, vals, bs, **kw)  # warm
+    t0 = time.time()
+    seps = [lp.bicgstab(pat.rowOffsets, pat.cols, vals, b, **kw) for b in bs]
+    t_sep = time.time() - t0
+    t0 = time.time()
+    xs, infos = lp.bicgstab3(pat.rowOffsets, pat.cols, vals, bs, **kw)
+    t_fused = time.time() - t0
+    same = all(np.array_equal(xs[c], seps[c][0]) for c in range(3))
+    alg = 12 * pat.nnz + 4 * (m.nCells + 1) + 48 * m.nCells
+    ts = infos[0].spmvAvgSeconds
+    log(f"momentum3: separate {t_sep:.3f}s fused {t_fused:.3f}s its "
+        f"{[i.iterations for i in infos]} bitwise={same} spmv3 {ts * 1e6:.1f}us "
+        f"{alg / ts / 1e9:.0f} GB/s")
+    return dict(cells=m.nCells, nnz=pat.nnz, separate_s=t_sep, fused_s=t_fused,
+                iterations=[i.iterations for i in infos], bitwise_equal_to_separate=same,
+                spmv3_s=ts, spmv3_alg_bytes=alg, spmv3_GBps=alg / ts / 1e9,
+                note="host-array API: both timings include the PCIe upload of matrix and vectors")
+
+
 # ------------------------------------------------------- copy probe ----
 def copy_probe(nbytes):
     """Three device-to-device copies of nbytes (HIP's blit kernel): reads
@@ -196,6 +240,7 @@ def main():
     ap.add_argument("--psolve-reps", type=int, default=3)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-psolve", action="store_true")
+    ap.add_argument("--mom3-n", type=int, default=100, help="3-RHS momentum box size (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--only-psolve", action="store_true",
@@ -247,6 +292,7 @@ def main():
                           f"{args.pn}^3 pressure matrix",
                 "alg_bytes_per_launch": alg, "avg_launch_s": t,
                 "launches_timed": ps["fp64"]["spmv_calls"]}
+    mom3 = run_momentum3(args.mom3_n) if args.mom3_n > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline_simple(m, vel, prs, args.cpu_iters)
     out = {
         "metric": "SIMPLE outer iterations/s (FP64), cylinder channel ~2M cells",
@@ -267,6 +313,7 @@ def main():
         "simple_fp64": r64,
         "simple_mixed": mixed,
         "psolve": ps,
+        "momentum3": mom3,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -208,6 +208,19 @@ an_status an_bicgstab_solve(int n, const int *rowOffsets, const int *cols,
                             double relTol, double absTol, an_memspace mem,
                             an_solve_info *info);
 
+/* Three right-hand sides sharing one matrix -- the u/v/w momentum systems,
+ * which the reference solves one after the other with the x-momentum matrix
+ * (GSV1 main.cu:3401-3405, solve_momentum_gpu :2553-2576).  Each column has
+ * its own BiCGSTAB scalars and stop test and is bit-identical to
+ * an_bicgstab_solve on that column; the fused SpMV reads the matrix once for
+ * all three.  b[3], x[3] (x holds x0), info[3]; info[c].spmvAvgSeconds holds
+ * the duration of the first fused SpMV.  AN_ENOCONV if any column did not
+ * converge. */
+an_status an_bicgstab_solve3(int n, const int *rowOffsets, const int *cols,
+                             int64_t nnz, const double *values,
+                             const double *const *b, double *const *x,
+                             int maxIter, double relTol, double absTol,
+                             an_memspace mem, an_solve_info *info);
 /* ------------------------------------------------ libpoisson / libscalar */
 typedef struct {
   int nNonOrthCorr;      /* EllipticOptions::nNonOrthCorr */

@@ -208,6 +208,37 @@ def test_bicgstab_matches_oracle(meshes, omeshes):
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
 
 
+def test_bicgstab3_columns_bitwise_equal_single_solves(meshes, omeshes):
+    """Fused 3-RHS BiCGSTAB (u/v/w momentum systems sharing one matrix):
+    every column equals a separate an_bicgstab_solve bit for bit, including
+    its iteration count, when columns converge at different iterations and
+    when one right-hand side is zero (x = 0, HYPRE convention)."""
+    m, om = meshes["cyl"], omeshes["cyl"]
+    flux = m.Sf @ np.array([0.5, 0.1, 0.05])
+    t, v = bc_arrays(m, dirichlet=("patch_1_0", "patch_3_0"), value=lambda x: x[:, 1])
+    vals, rhs = O.transport_assemble(om, flux, np.full(m.nFaces, 0.01), np.ones(m.nCells),
+                                     np.zeros(m.nCells), t, v, np.zeros((m.nCells, 3)))
+    rng = np.random.default_rng(12)
+    cases = [
+        [rhs, 1e-3 * rng.standard_normal(m.nCells), rhs + 0.5 * rng.standard_normal(m.nCells)],
+        [rhs, np.zeros(m.nCells), 3.0 * rhs],
+    ]
+    for bs in cases:
+        x0s = [np.zeros(m.nCells), 0.1 * rng.standard_normal(m.nCells), np.ones(m.nCells)]
+        xs, infos = lp.bicgstab3(om.rowOffsets, om.cols, vals, bs, x0s, absTol=1e-12,
+                                 maxIter=5000)
+        its = []
+        for c in range(3):
+            x1, i1 = lp.bicgstab(om.rowOffsets, om.cols, vals, bs[c], x0s[c], absTol=1e-12,
+                                 maxIter=5000)
+            assert np.array_equal(xs[c], x1), c
+            assert infos[c].iterations == i1.iterations and infos[c].converged == i1.converged
+            assert infos[c].finalResNorm == i1.finalResNorm
+            its.append(i1.iterations)
+        assert infos[0].spmvAvgSeconds > 0.0
+    assert len(set(its)) > 1 or 0 in its  # the columns really stopped apart
+
+
 # --------------------------------------------------- libpoisson drivers
 def mms_error(n):
     m = lp.Mesh.tetcube(n)
