@@ -129,6 +129,11 @@ SIGNATURES = {
     "an_solve_steady_scalar_transport": (_I, [_P, _P, _P, _P, _P, _P, _P,
                                               C.POINTER(TransportOptions), _P, _P, _I,
                                               C.POINTER(SolveInfo)]),
+    "an_transport_adjoint": (_I, [_P, _P, _P, _P, _P, _P, _P, C.POINTER(TransportOptions), _P,
+                                  _P, _P, _P, _P, _I, C.POINTER(SolveInfo)]),
+    "an_face_gamma_from_cells": (_I, [_P, _P, _P, _I]),
+    "an_dgamma_to_cells": (_I, [_P, _P, _P, _I]),
+    "an_lsq_gradient_adjoint": (_I, [_P, _P, _P, _P, _P, _P, _I]),
     "an_simple_default_params": (None, [C.POINTER(SimpleParams)]),
     "an_simple_create": (_I, [_P, C.POINTER(SimpleParams), _P, _P, _P, _P, _P, _P, _I,
                               C.POINTER(_P)]),

@@ -53,33 +53,6 @@ MeshView view_of(const DevMesh& d) {
   return m;
 }
 
-// D = gamma |S|^2 / max(d.S, 1e-30) with S = A n (Sf = mul3(areaMag, nloc),
-// LP/src/mesh.cu:259) and d the owner->neighbour (or owner->face) vector.
-struct DiffFace {
-  double S0, S1, S2, d0, d1, d2, D;
-};
-__device__ __forceinline__ DiffFace diff_face(double A, double n0, double n1, double n2, double d0,
-                                              double d1, double d2, double gamma) {
-  DiffFace r;
-  r.S0 = A * n0;
-  r.S1 = A * n1;
-  r.S2 = A * n2;
-  r.d0 = d0;
-  r.d1 = d1;
-  r.d2 = d2;
-  const double dDotS = d0 * r.S0 + d1 * r.S1 + d2 * r.S2;
-  const double SS = r.S0 * r.S0 + r.S1 * r.S1 + r.S2 * r.S2;
-  r.D = gamma * SS / (dDotS > 1e-30 ? dDotS : 1e-30);
-  return r;
-}
-
-// non-orthogonal correction gamma * T.g with T = S - (D/max(gamma,1e-30)) d
-__device__ __forceinline__ double nonorth_corr(const DiffFace& q, double gamma, double g0, double g1,
-                                               double g2) {
-  const double s = q.D / (gamma > 1e-30 ? gamma : 1e-30);
-  const double T0 = q.S0 - s * q.d0, T1 = q.S1 - s * q.d1, T2 = q.S2 - s * q.d2;
-  return gamma * (T0 * g0 + T1 * g1 + T2 * g2);
-}
 
 __device__ __forceinline__ void add_slot(double* __restrict__ vals, int slotInfo, double v) {
   // slotInfo = slot<<1 | first: the first contribution to a slot starts

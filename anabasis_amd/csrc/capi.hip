@@ -815,4 +815,93 @@ an_status an_simple_destroy(an_simple_t s) {
   AN_CATCH
 }
 
+// ----------------------------------------------------------- adjoint ----
+an_status an_face_gamma_from_cells(an_mesh_t mesh, const double* kCell, double* gammaFace,
+                                   an_memspace mem) {
+  AN_TRY
+  if (!kCell || !gammaFace) throw Error(AN_EINVAL, "NULL argument");
+  DevMesh& d = devmesh(mesh, true);
+  hipStream_t s = 0;
+  InArr<double> k(kCell, (size_t)d.nC, mem, s);
+  DevBuf<double> g((size_t)d.nF);
+  launch_gamma_from_cells(view_of(d), k.p, g.p, s);
+  AN_HIP(hipMemcpyAsync(gammaFace, g.p, sizeof(double) * d.nF, kind_out(mem), s));
+  AN_HIP(hipStreamSynchronize(s));
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_dgamma_to_cells(an_mesh_t mesh, const double* dJdgammaFace, double* dJdkCell,
+                             an_memspace mem) {
+  AN_TRY
+  if (!dJdgammaFace || !dJdkCell) throw Error(AN_EINVAL, "NULL argument");
+  DevMesh& d = devmesh(mesh, true);
+  hipStream_t s = 0;
+  InArr<double> dg(dJdgammaFace, (size_t)d.nF, mem, s);
+  DevBuf<double> dk((size_t)d.nC);
+  launch_dk_from_dgamma(view_of(d), dg.p, dk.p, s);
+  AN_HIP(hipMemcpyAsync(dJdkCell, dk.p, sizeof(double) * d.nC, kind_out(mem), s));
+  AN_HIP(hipStreamSynchronize(s));
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_lsq_gradient_adjoint(an_mesh_t mesh, const int* bcTypeBF, const double* bcValBF,
+                                  const double* w, double* phibar, double* phibarBF,
+                                  an_memspace mem) {
+  AN_TRY
+  if (!w || !phibar) throw Error(AN_EINVAL, "NULL argument");
+  DevMesh& d = devmesh(mesh, true);
+  hipStream_t s = 0;
+  InArr<int> bt(bcTypeBF, (size_t)d.nBF, mem, s);
+  InArr<double> bv(bcValBF, (size_t)d.nBF, mem, s);
+  InArr<double> ww(w, 3 * (size_t)d.nC, mem, s);
+  DevBuf<double> z(3 * (size_t)d.nC), pb((size_t)d.nC), pbB((size_t)(d.nBF > 0 ? d.nBF : 1));
+  launch_lsq_adjoint(view_of(d), BCView{bt.p, bv.p}, ww.p, z.p, pb.p, pbB.p, s);
+  AN_HIP(hipMemcpyAsync(phibar, pb.p, sizeof(double) * d.nC, kind_out(mem), s));
+  if (phibarBF && d.nBF > 0)
+    AN_HIP(hipMemcpyAsync(phibarBF, pbB.p, sizeof(double) * d.nBF, kind_out(mem), s));
+  AN_HIP(hipStreamSynchronize(s));
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_transport_adjoint(an_mesh_t mesh, const double* faceFlux, const double* gammaFace,
+                               const double* Su, const double* Sp, const int* bcTypeBF,
+                               const double* bcValBF, const an_transport_options* opt,
+                               const double* phi, const double* dJdphi, double* dJdgammaFace,
+                               double* dJdbcBF, double* adjoint, an_memspace mem,
+                               an_solve_info* info) {
+  AN_TRY
+  if (!phi || !dJdphi || !dJdgammaFace) throw Error(AN_EINVAL, "NULL argument");
+  DevMesh& d = devmesh(mesh, true);
+  an_transport_options o;
+  an_transport_default_options(&o);
+  if (opt) o = *opt;
+  hipStream_t s = 0;
+  InArr<double> ff(faceFlux, (size_t)d.nF, mem, s);
+  InArr<double> gam(gammaFace, (size_t)d.nF, mem, s);
+  InArr<double> su(Su, (size_t)d.nC, mem, s);
+  InArr<double> sp(Sp, (size_t)d.nC, mem, s);
+  InArr<int> bt(bcTypeBF, (size_t)d.nBF, mem, s);
+  InArr<double> bv(bcValBF, (size_t)d.nBF, mem, s);
+  InArr<double> ph(phi, (size_t)d.nC, mem, s);
+  InArr<double> dj(dJdphi, (size_t)d.nC, mem, s);
+  DevBuf<double> dg((size_t)d.nF), db((size_t)(d.nBF > 0 ? d.nBF : 1)), psi((size_t)d.nC);
+  const int nOuter = o.nonOrth ? (o.nNonOrthCorr > 0 ? o.nNonOrthCorr : 0) + 1 : 1;
+  an_solve_info last{};
+  an_status st = transport_adjoint(d, view_of(d), ff.p, gam.p, su.p, sp.p, BCView{bt.p, bv.p},
+                                   o.upwind, o.nonOrth, nOuter, o.maxIter, o.relTol, o.absTol,
+                                   ph.p, dj.p, dg.p, db.p, psi.p, s, &last);
+  AN_HIP(hipMemcpyAsync(dJdgammaFace, dg.p, sizeof(double) * d.nF, kind_out(mem), s));
+  if (dJdbcBF && d.nBF > 0)
+    AN_HIP(hipMemcpyAsync(dJdbcBF, db.p, sizeof(double) * d.nBF, kind_out(mem), s));
+  if (adjoint) AN_HIP(hipMemcpyAsync(adjoint, psi.p, sizeof(double) * d.nC, kind_out(mem), s));
+  AN_HIP(hipStreamSynchronize(s));
+  if (info) *info = last;
+  if (st == AN_ENOCONV) set_last_error("adjoint BiCGSTAB did not converge");
+  return st;
+  AN_CATCH
+}
+
 }  // extern "C"

@@ -17,43 +17,6 @@
 
 namespace an {
 
-__device__ __forceinline__ void acc_M(double M[9], double r0, double r1, double r2, double w) {
-  // M[i][j] += w*r[i]*r[j] as the reference writes it: (w*r_i)*r_j is not
-  // bitwise symmetric, so all nine entries are accumulated.
-  const double a0 = w * r0, a1 = w * r1, a2 = w * r2;
-  M[0] += a0 * r0;
-  M[1] += a0 * r1;
-  M[2] += a0 * r2;
-  M[3] += a1 * r0;
-  M[4] += a1 * r1;
-  M[5] += a1 * r2;
-  M[6] += a2 * r0;
-  M[7] += a2 * r1;
-  M[8] += a2 * r2;
-}
-
-// explicit adjugate inverse (main.cu:1221-1232); returns false (zero
-// operator) when |det| <= 1e-20.
-__device__ __forceinline__ bool inv3(const double M[9], double inv[9]) {
-  const double a = M[0], b = M[1], c = M[2];
-  const double d = M[3], e = M[4], f2 = M[5];
-  const double g = M[6], h = M[7], k = M[8];
-  const double det = a * (e * k - f2 * h) - b * (d * k - f2 * g) + c * (d * h - e * g);
-  if (!(fabs(det) > 1e-20)) {
-    for (int i = 0; i < 9; ++i) inv[i] = 0.0;
-    return false;
-  }
-  inv[0] = (e * k - f2 * h) / det;
-  inv[1] = (c * h - b * k) / det;
-  inv[2] = (b * f2 - c * e) / det;
-  inv[3] = (f2 * g - d * k) / det;
-  inv[4] = (a * k - c * g) / det;
-  inv[5] = (c * d - a * f2) / det;
-  inv[6] = (d * h - e * g) / det;
-  inv[7] = (b * g - a * h) / det;
-  inv[8] = (a * e - b * d) / det;
-  return true;
-}
 
 __global__ __launch_bounds__(256) void k_lsq_gsv1(MeshView m, BCView bc,
                                                   const double* __restrict__ phi,

@@ -87,4 +87,19 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
 
 void default_pcg_options(an_pcg_options* o);
 
+// discrete adjoint of the steady scalar transport solve (adjoint.hip)
+struct DevMesh;
+struct MeshView;
+struct BCView;
+void launch_gamma_from_cells(const MeshView& m, const double* k, double* gam, hipStream_t s);
+void launch_dk_from_dgamma(const MeshView& m, const double* dgam, double* dk, hipStream_t s);
+void launch_lsq_adjoint(const MeshView& m, BCView bc, const double* w, double* z, double* phibar,
+                        double* phibarB, hipStream_t s);
+an_status transport_adjoint(const DevMesh& d, const MeshView& mv, const double* flux,
+                            const double* gam, const double* Su, const double* Sp, BCView bc,
+                            int upwind, int nonOrth, int nOuter, int maxIter, double relTol,
+                            double absTol, const double* phi, const double* dJdphi,
+                            double* dgam, double* dbc, double* psiOut, hipStream_t s,
+                            an_solve_info* info);
+
 }  // namespace an

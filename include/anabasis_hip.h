@@ -257,6 +257,46 @@ an_status an_solve_steady_scalar_transport(
     const double *bcValBF, const an_transport_options *opt, const double *x0,
     double *phi, an_memspace mem, an_solve_info *info);
 
+/* --------------------------------------------------------------- adjoint */
+/* Discrete adjoint of libscalar::solve_steady_scalar_transport
+ * (LS/src/scalar_transport_library.cu:475-512; assembly :250-357).  No
+ * counterpart in the reference, which has no adjoint.
+ *
+ * For any functional J(phi) of the converged transport solution phi (the
+ * non-orthogonal fixed point, or the single orthogonal solve when
+ * opt->nonOrth == 0), given dJdphi[nCells] at phi, returns
+ * dJdgammaFace[nFaces] = dJ/d gammaFace, dJdbcBF[nBF] = dJ/d bcValBF (the
+ * Dirichlet values, or the NeumannGradient values), and optionally the
+ * adjoint field[nCells].  The adjoint system A^T psi = dJ/dphi + (dc/dphi)^T
+ * psi is iterated opt->nNonOrthCorr+1 times (the transpose of the forward
+ * deferred-correction loop), each pass a BiCGSTAB+Jacobi solve with opt's
+ * tolerances; the correction's transpose recomputes the least-squares
+ * operator instead of storing it. */
+an_status an_transport_adjoint(an_mesh_t mesh, const double *faceFlux,
+                               const double *gammaFace, const double *Su,
+                               const double *Sp, const int *bcTypeBF,
+                               const double *bcValBF,
+                               const an_transport_options *opt,
+                               const double *phi, const double *dJdphi,
+                               double *dJdgammaFace, double *dJdbcBF,
+                               double *adjoint, an_memspace mem,
+                               an_solve_info *info);
+/* Per-cell diffusivity -> face diffusivity: gamma_f = (1-lambda_f) k_P +
+ * lambda_f k_N on internal faces (lambda_f the face interpolation weight of
+ * LS face_interp_lambda, scalar_transport_library.cu:128-135), k_P on
+ * boundary faces.  No reference counterpart (the reference takes gammaFace). */
+an_status an_face_gamma_from_cells(an_mesh_t mesh, const double *kCell,
+                                   double *gammaFace, an_memspace mem);
+/* Chain rule of an_face_gamma_from_cells: dJdkCell = (d gamma/d k)^T dJdgammaFace. */
+an_status an_dgamma_to_cells(an_mesh_t mesh, const double *dJdgammaFace,
+                             double *dJdkCell, an_memspace mem);
+/* Transpose of the LP least-squares gradient (an_lsq_gradient mode
+ * AN_GRAD_LP, LP/src/gradient.cu:21-83) applied to w[3*nCells]:
+ * phibar = (d grad/d phi)^T w, phibarBF = (d grad/d bcValBF)^T w. */
+an_status an_lsq_gradient_adjoint(an_mesh_t mesh, const int *bcTypeBF,
+                                  const double *bcValBF, const double *w,
+                                  double *phibar, double *phibarBF,
+                                  an_memspace mem);
 /* ------------------------------------------------------------------ SIMPLE */
 typedef struct {
   double rho, mu;
