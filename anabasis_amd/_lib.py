@@ -129,6 +129,12 @@ SIGNATURES = {
     "an_solve_steady_scalar_transport": (_I, [_P, _P, _P, _P, _P, _P, _P,
                                               C.POINTER(TransportOptions), _P, _P, _I,
                                               C.POINTER(SolveInfo)]),
+    "an_batch_create": (_I, [_I, _P, _P, _P, C.POINTER(PcgOptions), _I, C.POINTER(_P)]),
+    "an_batch_set_values": (_I, [_P, _P, _I]),
+    "an_batch_solve": (_I, [_P, _P, _P, _I, C.POINTER(SolveInfo)]),
+    "an_batch_tuning": (_I, [_P, _pI, _P]),
+    "an_batch_use_graph": (_I, [_P, _I]),
+    "an_batch_destroy": (_I, [_P]),
     "an_transport_adjoint": (_I, [_P, _P, _P, _P, _P, _P, _P, C.POINTER(TransportOptions), _P,
                                   _P, _P, _P, _P, _I, C.POINTER(SolveInfo)]),
     "an_face_gamma_from_cells": (_I, [_P, _P, _P, _I]),

@@ -10,6 +10,7 @@
 #include "kernels.hpp"
 #include "simple.hpp"
 #include "solver.hpp"
+#include "batch.hpp"
 
 namespace an {
 static thread_local std::string g_err;
@@ -901,6 +902,79 @@ an_status an_transport_adjoint(an_mesh_t mesh, const double* faceFlux, const dou
   if (info) *info = last;
   if (st == AN_ENOCONV) set_last_error("adjoint BiCGSTAB did not converge");
   return st;
+  AN_CATCH
+}
+
+// ------------------------------------------------------------- batch ----
+struct an_batch_s {
+  std::unique_ptr<BatchSolver> b;
+};
+
+an_status an_batch_create(int nProblems, const int* nRows, const int* const* rowOffsets,
+                          const int* const* cols, const an_pcg_options* opt, an_memspace mem,
+                          an_batch_t* out) {
+  AN_TRY
+  if (nProblems <= 0 || !nRows || !rowOffsets || !cols || !out)
+    throw Error(AN_EINVAL, "NULL argument");
+  an_pcg_options o;
+  default_pcg_options(&o);
+  if (opt) o = *opt;
+  std::vector<int> rows(nRows, nRows + nProblems);
+  std::vector<const int*> ro(rowOffsets, rowOffsets + nProblems), ci(cols, cols + nProblems);
+  auto h = std::make_unique<an_batch_s>();
+  h->b.reset(new BatchSolver(nProblems, rows, ro, ci, mem == AN_DEVICE, o));
+  *out = h.release();
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_set_values(an_batch_t b, const double* const* values, an_memspace mem) {
+  AN_TRY
+  if (!b || !values) throw Error(AN_EINVAL, "NULL argument");
+  std::vector<const double*> v(values, values + b->b->nP);
+  b->b->set_values(v, mem == AN_DEVICE);
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_solve(an_batch_t b, const double* const* rhs, double* const* x,
+                         an_memspace mem, an_solve_info* info) {
+  AN_TRY
+  if (!b || !rhs || !x) throw Error(AN_EINVAL, "NULL argument");
+  const int nP = b->b->nP;
+  std::vector<const double*> bb(rhs, rhs + nP);
+  std::vector<double*> xx(x, x + nP);
+  std::vector<an_solve_info> inf;
+  an_status st = b->b->solve(bb, xx, mem == AN_DEVICE, inf);
+  if (info)
+    for (int p = 0; p < nP; ++p) info[p] = inf[p];
+  if (st == AN_ENOCONV) set_last_error("batched PCG: a problem did not converge");
+  return st;
+  AN_CATCH
+}
+
+an_status an_batch_tuning(an_batch_t b, int* rowsPerWave, double* seconds) {
+  AN_TRY
+  if (!b) throw Error(AN_EINVAL, "NULL argument");
+  if (rowsPerWave) *rowsPerWave = b->b->lay.rows;
+  if (seconds)
+    for (size_t i = 0; i < 3; ++i) seconds[i] = i < b->b->tuneSeconds.size() ? b->b->tuneSeconds[i] : 0.0;
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_use_graph(an_batch_t b, int enable) {
+  AN_TRY
+  if (!b) throw Error(AN_EINVAL, "NULL argument");
+  b->b->useGraph = enable != 0;
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_destroy(an_batch_t b) {
+  AN_TRY
+  delete b;
+  return AN_OK;
   AN_CATCH
 }
 

@@ -363,6 +363,59 @@ def bicgstab3(rowOffsets, cols, values, bs, x0s=None, maxIter=1000, relTol=0.0, 
     return xs, list(info)
 
 
+class BatchPCG:
+    """Many independent SPD systems solved together (an_batch_*): one
+    block-diagonal CSR, one AMG hierarchy, per-problem PCG scalars, the
+    iteration replayed from a captured HIP graph."""
+
+    def __init__(self, patterns, **opts):
+        self.ro = [i32(r) for r, _ in patterns]
+        self.ci = [i32(c) for _, c in patterns]
+        self.nP = len(patterns)
+        self.n = [r.size - 1 for r in self.ro]
+        self.opt = pcg_options(**opts)
+        rows = np.array(self.n, np.int32)
+        rop = (C.c_void_p * self.nP)(*[r.ctypes.data for r in self.ro])
+        cip = (C.c_void_p * self.nP)(*[c.ctypes.data for c in self.ci])
+        h = C.c_void_p()
+        check(lib().an_batch_create(self.nP, ptr(rows), rop, cip, C.byref(self.opt), L.AN_HOST,
+                                    C.byref(h)))
+        self._h = h
+
+    def set_values(self, values):
+        vs = [f64(v) for v in values]
+        vp = (C.c_void_p * self.nP)(*[v.ctypes.data for v in vs])
+        check(lib().an_batch_set_values(self._h, vp, L.AN_HOST))
+
+    def solve(self, rhs, x0=None):
+        bs = [f64(b) for b in rhs]
+        xs = [np.zeros(n) if x0 is None else f64(x0[p]).copy() for p, n in enumerate(self.n)]
+        bp = (C.c_void_p * self.nP)(*[b.ctypes.data for b in bs])
+        xp = (C.c_void_p * self.nP)(*[x.ctypes.data for x in xs])
+        info = (L.SolveInfo * self.nP)()
+        st = lib().an_batch_solve(self._h, bp, xp, L.AN_HOST, info)
+        check(st, allow=(L.AN_ENOCONV,))
+        return xs, list(info)
+
+    def use_graph(self, enable=True):
+        check(lib().an_batch_use_graph(self._h, int(enable)))
+
+    def tuning(self):
+        rows = C.c_int()
+        secs = np.zeros(3)
+        check(lib().an_batch_tuning(self._h, C.byref(rows), ptr(secs)))
+        return rows.value, secs
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().an_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 @dataclass
 class EllipticOptions:
     """LP/include/poisson_library.h:5-13 (hypre -> PCG/AMG options)."""

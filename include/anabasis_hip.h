@@ -257,6 +257,37 @@ an_status an_solve_steady_scalar_transport(
     const double *bcValBF, const an_transport_options *opt, const double *x0,
     double *phi, an_memspace mem, an_solve_info *info);
 
+/* ----------------------------------------------------------------- batch */
+/* Many independent sparse SPD systems (e.g. the MMS Poisson problems of
+ * apps/poisson_mms_lib run on a set of meshes) solved together: one
+ * block-diagonal CSR, one aggregation-AMG hierarchy that never couples
+ * problems, and per-problem PCG scalars, stop tests and iteration counts.
+ * The PCG iteration is captured once in a HIP graph and replayed until
+ * every problem has converged.  No reference counterpart (the reference
+ * solves one system per HYPRE call, LP/src/hypre_backend.cu:3-114).
+ * opt: as an_pcg_create (precision 1 = FP32 PCG+AMG inside FP64
+ * refinement).  Patterns are copied. */
+typedef struct an_batch_s *an_batch_t;
+an_status an_batch_create(int nProblems, const int *nRows,
+                          const int *const *rowOffsets, const int *const *cols,
+                          const an_pcg_options *opt, an_memspace mem,
+                          an_batch_t *out);
+/* Values of every problem; (re)builds the hierarchy and runs the
+ * autotuner (the block height of the batched kernels: 64, 32 or 16 rows per
+ * wave, timed on this matrix). */
+an_status an_batch_set_values(an_batch_t b, const double *const *values,
+                              an_memspace mem);
+/* Solves every problem; x[p] holds x0 on entry; info[nProblems]. */
+an_status an_batch_solve(an_batch_t b, const double *const *rhs,
+                         double *const *x, an_memspace mem,
+                         an_solve_info *info);
+/* Autotuner record: chosen rows per wave, and the timed SpMV seconds of the
+ * three candidates (64, 32, 16 rows) in seconds[3]. */
+an_status an_batch_tuning(an_batch_t b, int *rowsPerWave, double *seconds);
+/* enable != 0 (default): replay the captured graph; 0: issue the same
+ * launches one by one (a check that the graph reproduces eager results). */
+an_status an_batch_use_graph(an_batch_t b, int enable);
+an_status an_batch_destroy(an_batch_t b);
 /* --------------------------------------------------------------- adjoint */
 /* Discrete adjoint of libscalar::solve_steady_scalar_transport
  * (LS/src/scalar_transport_library.cu:475-512; assembly :250-357).  No
