@@ -466,14 +466,16 @@ static void build_members(int n, int nc, const int* agg, DevBuf<int>& moff, DevB
   AN_CHECK_LAUNCH();
 }
 
-static void galerkin(const DevCSR<double>& W, int nc, const int* agg, const int* moff,
+// Galerkin aggregate sum; returns false (C untouched) when a coarse row would
+// hold more than kGalCap entries -- the hierarchy then stops at this level.
+static bool galerkin(const DevCSR<double>& W, int nc, const int* agg, const int* moff,
                      const int* mem, DevCSR<double>& C, hipStream_t s) {
   DevBuf<int> cnt((size_t)nc + 1), overflow(1);
   overflow.zero(s);
   k_gal_count<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, agg, cnt.p,
                                                 overflow.p);
   AN_CHECK_LAUNCH();
-  if (read_int(overflow.p, s)) throw Error(7, "AMG coarse row exceeds 256 entries");
+  if (read_int(overflow.p, s)) return false;
   C.n = nc;
   C.rowOff.alloc((size_t)nc + 1);
   C.nnz = exclusive_scan(cnt.p, C.rowOff.p, nc, s);
@@ -482,6 +484,7 @@ static void galerkin(const DevCSR<double>& W, int nc, const int* agg, const int*
   k_gal_fill<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, W.vals.p, agg,
                                                C.rowOff.p, C.cols.p, C.vals.p);
   AN_CHECK_LAUNCH();
+  return true;
 }
 
 void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
@@ -510,24 +513,28 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
     DevBuf<int> agg;
     int cur = pair_pass(*L.A, opt.strength, agg, s);
     DevCSR<double> W;
+    bool fits;
     {
       DevBuf<int> mo, me;
       build_members(L.n, cur, agg.p, mo, me, s);
-      galerkin(*L.A, cur, agg.p, mo.p, me.p, W, s);
+      fits = galerkin(*L.A, cur, agg.p, mo.p, me.p, W, s);
     }
-    for (int p = 1; p < (opt.pairPasses > 0 ? opt.pairPasses : 1); ++p) {
+    for (int p = 1; fits && p < (opt.pairPasses > 0 ? opt.pairPasses : 1); ++p) {
       DevBuf<int> a2;
       int n2 = pair_pass(W, opt.strength, a2, s);
       DevCSR<double> W2;
       DevBuf<int> mo, me;
       build_members(W.n, n2, a2.p, mo, me, s);
-      galerkin(W, n2, a2.p, mo.p, me.p, W2, s);
+      fits = galerkin(W, n2, a2.p, mo.p, me.p, W2, s);
+      if (!fits) break;
       k_compose<<<grid_for(L.n, kB), kB, 0, s>>>(L.n, agg.p, a2.p);
       AN_CHECK_LAUNCH();
       W = std::move(W2);
       cur = n2;
     }
-    if (cur >= L.n || (double)cur > 0.9 * (double)L.n) {
+    // a coarse operator too dense for the aggregate-sum kernel, or too
+    // little coarsening: this level is the coarsest
+    if (!fits || cur >= L.n || (double)cur > 0.9 * (double)L.n) {
       L.isCoarsest = true;
       break;
     }

@@ -140,6 +140,127 @@ def run_momentum3(n):
                 note="host-array API: both timings include the PCIe upload of matrix and vectors")
 
 
+# ------------------------------------------------------- batched MMS ----
+def run_batch_mms(nprob, precision):
+    """nprob Dirichlet Poisson MMS problems on unit-cube tet meshes of n^3
+    blocks x 6 tets, n log-uniform in [8, 64] (seeded), each with a random
+    smooth manufactured solution u = sum_k a_k sin(pi w_k.x + c_k); solved
+    together by the batched PCG+AMG (graph-replayed iteration) to 1e-10
+    relative, and one after another by the single-system solver."""
+    from anabasis_amd import libpoisson as lp
+    rng = np.random.default_rng(SEED + 3)
+    sizes = np.round(np.exp(rng.uniform(np.log(8), np.log(64), nprob))).astype(int)
+    t0 = time.time()
+    pats, vals, rhs = [], [], []
+    for i, nb in enumerate(sizes):
+        m = lp.Mesh.tetcube(int(nb))
+        pat = lp.build_scalar_pattern(m)
+        a, wv, c = rng.normal(size=3), rng.integers(1, 3, size=(3, 3)), rng.uniform(0, 6.3, 3)
+        cc, xf = m.cc, m.xf
+        u = lambda x: sum(a[k] * np.sin(np.pi * x @ wv[k] + c[k]) for k in range(3))
+        f = sum(a[k] * np.pi ** 2 * (wv[k] @ wv[k]) * np.sin(np.pi * cc @ wv[k] + c[k])
+                for k in range(3))
+        nIF = m.nInternalFaces
+        t = np.ones(m.nBoundaryFaces, np.int32)
+        v, r = lp.assemble_scalar_elliptic_system(m, np.ones(m.nFaces), f, t, u(xf[nIF:]),
+                                                  np.zeros((m.nCells, 3)), False, nnz=pat.nnz)
+        pats.append((pat.rowOffsets, pat.cols))
+        vals.append(v)
+        rhs.append(r)
+        del m
+    cells = int(sum(p[0].size - 1 for p in pats))
+    log(f"batch MMS: {nprob} problems, {cells} cells, sizes {sorted(sizes.tolist())} "
+        f"({time.time() - t0:.1f}s)")
+    B = lp.BatchPCG(pats, relTol=1e-10, absTol=0.0, precision=precision)
+    t0 = time.time()
+    B.set_values(vals)
+    setup = time.time() - t0
+    B.solve(rhs)  # warm (captures the graph)
+    t0 = time.time()
+    xs, infos = B.solve(rhs)
+    t_batch = time.time() - t0
+    rows, secs = B.tuning()
+    # the same problems one at a time
+    t_seq = 0.0
+    worst = 0.0
+    for (ro, ci), v, b, x in zip(pats, vals, rhs, xs):
+        S = lp.PCGSolver(ro, ci, relTol=1e-10, absTol=0.0, precision=precision)
+        S.set_values(v)
+        S.solve(b)
+        t1 = time.time()
+        xi, _ = S.solve(b)
+        t_seq += time.time() - t1
+        worst = max(worst, float(np.linalg.norm(x - xi) / np.linalg.norm(xi)))
+        del S
+    its = [i.iterations for i in infos]
+    conv = all(i.converged for i in infos)
+    log(f"batch MMS prec={precision}: setup {setup:.2f}s batched solve {t_batch:.3f}s vs "
+        f"one-by-one {t_seq:.3f}s; its {min(its)}..{max(its)} converged={conv} "
+        f"rows/wave {rows} tuning {[round(t * 1e6, 1) for t in secs]}us")
+    return dict(problems=nprob, cells=cells, precision=precision, setup_s=setup,
+                batched_solve_s=t_batch, one_by_one_solve_s=t_seq,
+                iterations_min=min(its), iterations_max=max(its), all_converged=conv,
+                max_rel_diff_vs_single=worst, autotuner_rows_per_wave=rows,
+                autotuner_spmv_s={"64": secs[0], "32": secs[1], "16": secs[2]},
+                note="solve times exclude host->device copies of the values; "
+                     "batched = one graph-replayed iteration sequence for all problems")
+
+
+# ----------------------------------------------------------- adjoint ----
+def run_adjoint(n, ncorr):
+    """Scalar transport on an n^3 jittered hex box: k = exp(N(0, 0.5)) per
+    cell, frozen u = (sin y, sin z, sin x), xmin Dirichlet inlet profile
+    1 + 0.5 sin(pi y) sin(pi z), Dirichlet 0 on the side walls, xmax
+    zero-gradient outlet; J = area integral of phi over the outlet.  Times the forward solve and the adjoint (gradient with
+    respect to every k and every boundary value) and checks it along a random
+    direction in k and one in the inlet values against central differences
+    (eps = 1e-6)."""
+    from anabasis_amd import adjoint as ad
+    from anabasis_amd import libpoisson as lp
+    t0 = time.time()
+    m = lp.Mesh.hexbox(n, n, n, 1.0, 1.0, 1.0, jitter=0.2, seed=SEED + 4)
+    xf = m.xf
+    u = np.stack([np.sin(xf[:, 1]), np.sin(xf[:, 2]), np.sin(xf[:, 0])], axis=1)
+    flux = np.einsum("ij,ij->i", u, m.Sf)
+    rng = np.random.default_rng(SEED + 5)
+    k = np.exp(rng.normal(0.0, 0.5, m.nCells))
+    bp = m.bPatch[m.nInternalFaces:]
+    types = np.ones(m.nBoundaryFaces, np.int32)  # Dirichlet: inlet profile, walls 0
+    types[bp == m.patch_index("xmax") + 1] = 2  # zero-gradient outlet
+    inlet = bp == m.patch_index("xmin") + 1
+    xb = xf[m.nInternalFaces:]
+    vals = np.where(inlet, 1.0 + 0.5 * np.sin(np.pi * xb[:, 1]) * np.sin(np.pi * xb[:, 2]), 0.0)
+    log(f"adjoint mesh {n}^3: nC={m.nCells} ({time.time() - t0:.1f}s)")
+    P = ad.TransportProblem(m, flux, types, nNonOrthCorr=ncorr, absTol=1e-13)
+    Jf, dJ = ad.outlet_functional(m, "xmax")
+    t0 = time.time()
+    phi = P.solve(k, vals)
+    t_fwd = time.time() - t0
+    t0 = time.time()
+    dk, db, _ = P.adjoint(k, vals, phi, dJ(phi))
+    t_adj = time.time() - t0
+    # directional central differences (a single cell's or face's share of J
+    # is below the forward solve's accuracy at this size): k along
+    # k (1 + eps xi), the inlet values along eps eta, eps = 1e-6
+    eps = 1e-6
+    xi = rng.standard_normal(m.nCells)
+    dkv = eps * k * xi
+    fd_k = (Jf(P.solve(k + dkv, vals)) - Jf(P.solve(k - dkv, vals))) / 2.0
+    ad_k = float(dk @ dkv)
+    eta = np.where(inlet, rng.standard_normal(m.nBoundaryFaces), 0.0)
+    fd_b = (Jf(P.solve(k, vals + eps * eta)) - Jf(P.solve(k, vals - eps * eta))) / 2.0
+    ad_b = float(db @ (eps * eta))
+    rel_k = abs(fd_k - ad_k) / abs(fd_k)
+    rel_b = abs(fd_b - ad_b) / abs(fd_b)
+    log(f"adjoint: forward {t_fwd:.2f}s adjoint {t_adj:.2f}s; directional FD vs adjoint: "
+        f"k {rel_k:.1e}, inlet values {rel_b:.1e}")
+    return dict(cells=m.nCells, nNonOrthCorr=ncorr, forward_s=t_fwd, adjoint_s=t_adj,
+                J=Jf(phi), fd_dir_k=fd_k, adjoint_dir_k=ad_k, fd_rel_err_k=rel_k,
+                fd_dir_bc=fd_b, adjoint_dir_bc=ad_b, fd_rel_err_bc=rel_b,
+                note="host-array API (includes PCIe copies); gradient w.r.t. all "
+                     f"{m.nCells} k and {m.nBoundaryFaces} boundary values in one adjoint")
+
+
 # ------------------------------------------------------- copy probe ----
 def copy_probe(nbytes):
     """Three device-to-device copies of nbytes (HIP's blit kernel): reads
@@ -241,6 +362,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-psolve", action="store_true")
     ap.add_argument("--mom3-n", type=int, default=100, help="3-RHS momentum box size (0: skip)")
+    ap.add_argument("--batch-problems", type=int, default=48, help="batched MMS problems (0: skip)")
+    ap.add_argument("--adjoint-n", type=int, default=128, help="adjoint box size (0: skip)")
+    ap.add_argument("--adjoint-ncorr", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--only-psolve", action="store_true",
@@ -293,6 +417,11 @@ def main():
                 "alg_bytes_per_launch": alg, "avg_launch_s": t,
                 "launches_timed": ps["fp64"]["spmv_calls"]}
     mom3 = run_momentum3(args.mom3_n) if args.mom3_n > 0 else None
+    batch = None
+    if args.batch_problems > 0:
+        batch = {"fp64": run_batch_mms(args.batch_problems, 0),
+                 "mixed": run_batch_mms(args.batch_problems, 1)}
+    adj = run_adjoint(args.adjoint_n, args.adjoint_ncorr) if args.adjoint_n > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline_simple(m, vel, prs, args.cpu_iters)
     out = {
         "metric": "SIMPLE outer iterations/s (FP64), cylinder channel ~2M cells",
@@ -314,6 +443,8 @@ def main():
         "simple_mixed": mixed,
         "psolve": ps,
         "momentum3": mom3,
+        "batch_mms": batch,
+        "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -88,6 +88,7 @@ typedef struct {
 } or_level;
 
 #define OR_MATCH_ROUNDS 6
+#define OR_GAL_CAP 256
 
 /* one pairwise matching pass; returns number of aggregates, agg filled */
 static int pair_match(int n, const int *ro, const int *ci, const double *va,
@@ -383,9 +384,9 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
     int curN = L->n;
     int *wro = L->ro, *wci = L->ci;
     double *wva = L->va;
-    int owned = 0;
+    int owned = 0, wide = 0;
     int passes = o->pairPasses > 0 ? o->pairPasses : 1;
-    for (int p = 0; p < passes; ++p) {
+    for (int p = 0; p < passes && !wide; ++p) {
       int *a1 = (int *)malloc(sizeof(int) * (size_t)curN);
       int nc1 = pair_match(curN, wro, wci, wva, o->strength, a1);
       int *mo, *me;
@@ -395,6 +396,10 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
       galerkin(wro, wci, wva, nc1, a1, mo, me, &nro, &nci, &nva);
       free(mo);
       free(me);
+      /* a coarse row wider than OR_GAL_CAP entries stops the hierarchy at
+         this level (the GPU aggregate-sum kernel's row capacity) */
+      for (int I = 0; I < nc1 && !wide; ++I)
+        if (nro[I + 1] - nro[I] > OR_GAL_CAP) wide = 1;
       for (int i = 0; i < L->n; ++i) agg[i] = a1[agg[i]];
       free(a1);
       if (owned) {
@@ -408,7 +413,7 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
       owned = 1;
       curN = nc1;
     }
-    if (curN >= L->n || (double)curN > 0.9 * (double)L->n) {
+    if (wide || curN >= L->n || (double)curN > 0.9 * (double)L->n) {
       free(agg);
       free(wro);
       free(wci);
