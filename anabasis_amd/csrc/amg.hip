@@ -316,10 +316,10 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
   yc[I] = (T)((double)omdc[I] * (double)bI);
 }
 
-// e_i = y_i + xc[agg_i]
+// e_i = y_i + xc[agg_i]  (e may be y: an in-place, per-element update)
 template <typename T>
-__global__ void k_prolong(int n, const T* __restrict__ y, const T* __restrict__ xc,
-                          const int* __restrict__ agg, T* __restrict__ e) {
+__global__ void k_prolong(int n, const T* y, const T* __restrict__ xc,
+                          const int* __restrict__ agg, T* e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) e[i] = (T)((double)y[i] + (double)xc[agg[i]]);
 }
@@ -632,9 +632,12 @@ struct LevelData<float> {
 };
 
 // x = V-cycle(b) on level l; y = omdinv o b (computed here when null).
+// sweeps > 1: x starts at y (the first sweep from zero) and takes sweeps-1
+// further Jacobi sweeps before the residual; after the coarse correction
+// every sweep is a Jacobi sweep.
 template <typename T>
 static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* x,
-                   hipStream_t s) {
+                   hipStream_t s, int sweeps) {
   using D = LevelData<T>;
   AmgLevel& L = lv[l];
   const DevCSR<T>& A = D::A(L);
@@ -670,25 +673,44 @@ static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* 
   AmgLevel& C = lv[l + 1];
   T* r = D::r(L);
   const int G = grid_for(A.rb.count, kW);
+  auto sweep = [&](const T* from, T* to) {  // to = from + omdinv (b - A from)
+    k_level_rows<T, 1><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
+                                        A.vals.p, D::omd(L), b, from, to);
+  };
+  // pre-smoothing: x0 = y; extra sweeps ping-pong between x and r
+  const T* xs = y;
+  for (int k = 1; k < sweeps; ++k) {
+    T* to = (xs == x) ? r : x;
+    sweep(xs, to);
+    xs = to;
+  }
+  T* res = (xs == r) ? x : r;
   k_level_rows<T, 0><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      D::omd(L), b, y, r);
-  k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, r, D::omd(C),
+                                      D::omd(L), b, xs, res);
+  k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, res, D::omd(C),
                                                   D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
-  vcycle<T>(lv, l + 1, D::b(C), D::y(C), D::x(C), s);
-  T* e = r;  // the residual is dead once restricted
-  k_prolong<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, y, D::x(C), L.agg.p, e);
-  k_level_rows<T, 1><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      D::omd(L), b, e, x);
+  vcycle<T>(lv, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
+  // e = x_pre + P xc, then the post-smoothing sweeps ping-pong between r
+  // and x; e starts in the buffer from which an odd/even number of sweeps
+  // ends in x (in place when that buffer already holds x_pre)
+  T* e = (sweeps % 2 == 1) ? r : x;
+  k_prolong<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, xs, D::x(C), L.agg.p, e);
+  const T* cur = e;
+  for (int k = 0; k < sweeps; ++k) {
+    T* to = (cur == r) ? x : r;
+    sweep(cur, to);
+    cur = to;
+  }
   AN_CHECK_LAUNCH();
 }
 
 void AmgHierarchy::apply(const double* r, double* z, hipStream_t s) {
-  vcycle<double>(lv, 0, r, nullptr, z, s);
+  vcycle<double>(lv, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
 }
 
 void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s) {
-  vcycle<float>(lv, 0, r, nullptr, z, s);
+  vcycle<float>(lv, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
 }
 
 }  // namespace an

@@ -11,6 +11,7 @@ struct AmgOptions {
   double strength = 0.25;
   int pairPasses = 2;
   double omega = 0.75;
+  int sweeps = 1;  // Jacobi sweeps per side of the coarse correction
 };
 
 struct AmgLevel {

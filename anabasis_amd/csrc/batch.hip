@@ -244,6 +244,7 @@ BatchSolver::BatchSolver(int nProblems, const std::vector<int>& rows,
   amg.opt.strength = o.strength;
   amg.opt.pairPasses = o.pairPasses;
   amg.opt.omega = o.jacobiOmega;
+  amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
   scal.alloc((size_t)S_COUNT * nP);
   scal.zero(s);
   tol2.alloc((size_t)2 * nP);  // outer, inner

@@ -31,6 +31,7 @@ void default_pcg_options(an_pcg_options* o) {
   o->jacobiOmega = 0.75;
   o->innerRelTol = 1e-4;
   o->profile = 0;
+  o->sweeps = 1;
 }
 
 static constexpr int kB = 256;
@@ -59,6 +60,7 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
   amg.opt.strength = o.strength;
   amg.opt.pairPasses = o.pairPasses;
   amg.opt.omega = o.jacobiOmega;
+  amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
   AN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   AN_HIP(hipStreamSynchronize(s));
 }

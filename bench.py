@@ -54,9 +54,12 @@ def simple_case(cells):
     return m, vel, prs
 
 
+AMG_OVERRIDES = {}
+
+
 def run_simple(m, vel, prs, precision, warmup, steps):
     from anabasis_amd import simple as sm
-    par = sm.default_params(precision=precision)
+    par = sm.default_params(precision=precision, pressure=dict(AMG_OVERRIDES))
     S = sm.Simple(m, par, vel, prs)
     t0 = time.time()
     st, _ = S.iterate(warmup)
@@ -308,7 +311,7 @@ def run_psolve(n, reps):
     res = {}
     for prec, name in ((0, "fp64"), (1, "mixed")):
         S = lp.PCGSolver(pat.rowOffsets, pat.cols, relTol=1e-10, absTol=0.0, precision=prec,
-                         profile=1)
+                         profile=1, **AMG_OVERRIDES)
         ts = time.time()
         S.set_values(vals, rebuild=True)
         setup = time.time() - ts
@@ -365,6 +368,9 @@ def main():
     ap.add_argument("--batch-problems", type=int, default=48, help="batched MMS problems (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=128, help="adjoint box size (0: skip)")
     ap.add_argument("--adjoint-ncorr", type=int, default=12)
+    ap.add_argument("--amg", type=str, default="",
+                    help="AMG/PCG option overrides, e.g. 'sweeps=2,pairPasses=2,innerRelTol=1e-5' "
+                         "(applied to the SIMPLE pressure solver and the pressure-solve section)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--only-psolve", action="store_true",
@@ -379,6 +385,9 @@ def main():
         sys.exit(2)
     from anabasis_amd import _lib as L
     L.check(L.lib().an_set_device(0))
+    for kv in filter(None, args.amg.split(",")):
+        k, v = kv.split("=")
+        AMG_OVERRIDES[k.strip()] = float(v) if any(c in v for c in ".e") else int(v)
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb << 20)
     if args.only_psolve:
@@ -436,6 +445,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (generated mesh, cases/cylinder.case controls and BCs)",
+        "amg_overrides": AMG_OVERRIDES,
         "config": {"workload": f"generic_simple_v1 cylinder channel, {m.nCells} hex cells, "
                                "nVelNonOrthCorr 1, nNonOrthCorr 10, AMG rebuild every 10",
                    "cells": m.nCells, "faces": m.nFaces},

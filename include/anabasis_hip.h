@@ -158,6 +158,8 @@ typedef struct {
   double jacobiOmega;  /* AMG: damped-Jacobi smoother weight */
   double innerRelTol;  /* mixed precision: inner FP32 solve reduction */
   int profile;         /* 1: time every SpMV / V-cycle with HIP events */
+  int sweeps;          /* AMG: damped-Jacobi sweeps before and after the
+                          coarse correction on every level (>= 1) */
 } an_pcg_options;
 void an_pcg_default_options(an_pcg_options *o);
 

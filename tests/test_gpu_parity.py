@@ -156,6 +156,12 @@ def test_spmv_amg_vcycle_bitexact(meshes, omeshes, kind):
     z = S.amg_apply(x)
     zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64)
     assert nl >= 3 and np.array_equal(z, zo)
+    for sweeps in (2, 3):  # multi-sweep smoothing (buffer ping-pong on every level)
+        S2 = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64, sweeps=sweeps)
+        S2.set_values(vals)
+        z2 = S2.amg_apply(x)
+        zo2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps)
+        assert np.array_equal(z2, zo2), sweeps
 
 
 def test_pcg_amg_converges_to_direct_solution(meshes, omeshes):
