@@ -43,7 +43,8 @@ class PcgOptions(C.Structure):
     _fields_ = [("maxIter", C.c_int), ("relTol", C.c_double), ("absTol", C.c_double),
                 ("precision", C.c_int), ("coarseSize", C.c_int), ("maxLevels", C.c_int),
                 ("strength", C.c_double), ("pairPasses", C.c_int), ("jacobiOmega", C.c_double),
-                ("innerRelTol", C.c_double), ("profile", C.c_int), ("sweeps", C.c_int)]
+                ("innerRelTol", C.c_double), ("profile", C.c_int), ("sweeps", C.c_int),
+                ("kcycle", C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -302,15 +302,30 @@ __global__ void k_scale(int n, const T* __restrict__ omdinv, const T* __restrict
 }
 
 // bc_I = sum of the members' residuals (ascending member order);
-// yc_I = omdinv_c,I * bc_I for the coarse level's own down kernel
+// yc_I = omdinv_c,I * bc_I for the coarse level's own down kernel.
+// Members are read in batches of 8 (all index loads, then all gathers;
+// slots past the end re-read the last member and are not added), so a
+// thread waits on two memory round trips per batch, not two per member.
 template <typename T>
 __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
                            const T* __restrict__ r, const T* __restrict__ omdc,
                            T* __restrict__ bc, T* __restrict__ yc) {
+  constexpr int U = 8;
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
+  const int m0 = moff[I], m1 = moff[I + 1];  // m1 > m0: aggregates are not empty
   double s = 0.0;
-  for (int m = moff[I]; m < moff[I + 1]; ++m) s += (double)r[mem[m]];
+  for (int b = m0; b < m1; b += U) {
+    int idx[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) idx[u] = mem[min(b + u, m1 - 1)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = (double)r[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b + u < m1) s += v[u];
+  }
   const T bI = (T)s;
   bc[I] = bI;
   yc[I] = (T)((double)omdc[I] * (double)bI);
@@ -376,6 +391,99 @@ __global__ void k_jacobi(int n, const int* __restrict__ ro, const int* __restric
   double s = 0.0;
   for (int k = ro[i]; k < ro[i + 1]; ++k) s += (double)va[k] * (double)x[ci[k]];
   y[i] = (T)((double)x[i] + (double)omdinv[i] * ((double)b[i] - s));
+}
+
+// K-cycle products on a coarse level (a CSR-stream pass gathering c):
+//   NC 2: w = A c stored in w; partial sums of c.w and c.b
+//   NC 3: w = A c not stored; partial sums of c.v1, c.w and c.b
+// The products use w rounded to T, the value an FP32 cycle stores.
+template <typename T, int NC>
+__global__ __launch_bounds__(kB) void k_kprod(int nrb, const int2* __restrict__ rbs,
+                                              const int* __restrict__ ro,
+                                              const int* __restrict__ ci,
+                                              const T* __restrict__ va, const T* __restrict__ c,
+                                              const T* __restrict__ b, const T* __restrict__ v1,
+                                              T* __restrict__ w, double* __restrict__ partial) {
+  __shared__ double prod[kW][kCap];
+  __shared__ double sh[NC * kW];
+  const int lb = xcd_block(gridDim.x);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const WaveRows q = wave_rows(nrb, rbs, lb * kW + wv);
+  const int i = q.r0 + lane;
+  int kb = 0, ke = 0;
+  double d[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) d[k] = 0.0;
+  if (i < q.r1) {
+    kb = ro[i];
+    ke = ro[i + 1];
+  }
+  if (q.stream && q.k1 > q.k0)
+    stage_products(q, lane, ci, va, [&](int j) { return (double)c[j]; }, prod[wv]);
+  wave_sync();
+  if (i < q.r1) {
+    double s = 0.0;
+    if (q.stream) {
+      for (int k = kb; k < ke; ++k) s += prod[wv][k - q.k0];
+    } else {
+      for (int k = kb; k < ke; ++k) s += (double)va[k] * (double)c[ci[k]];
+    }
+    const T wi = (T)s;
+    const double cv = (double)c[i];
+    if (NC == 2) {
+      w[i] = wi;
+      d[0] = cv * (double)wi;
+      d[1] = cv * (double)b[i];
+    } else {
+      d[0] = cv * (double)v1[i];
+      d[1] = cv * (double)wi;
+      d[NC - 1] = cv * (double)b[i];
+    }
+  }
+  block_sums<NC>(d, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) partial[(size_t)lb * NC + k] = d[k];
+}
+
+// Coefficients of the two flexible-CG steps from ks = {c1.Ac1, c1.b,
+// c2.v1, c2.Ac2, c2.b}; a zero c1.Ac1 (beta <= 0) drops the terms it divides.
+struct KCoef {
+  double a1, gr, a2;
+};
+__device__ __forceinline__ double kcyc_a1(const double* ks) {
+  return ks[0] > 0.0 ? ks[1] / ks[0] : 0.0;
+}
+__device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
+  KCoef k;
+  k.a1 = kcyc_a1(ks);
+  k.gr = ks[0] > 0.0 ? ks[2] / ks[0] : 0.0;
+  const double beta = ks[3] - ks[2] * k.gr;
+  k.a2 = beta > 0.0 ? ks[4] / beta : 0.0;
+  return k;
+}
+
+// b -= a1 v1 (the second step's right-hand side, in place); y = omdinv o b
+template <typename T>
+__global__ void k_kres(int n, const double* __restrict__ ks, const T* __restrict__ v1,
+                       const T* __restrict__ omd, T* __restrict__ b, T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a1 = kcyc_a1(ks);
+  const T bi = (T)((double)b[i] - a1 * (double)v1[i]);
+  b[i] = bi;
+  y[i] = (T)((double)omd[i] * (double)bi);
+}
+
+// x = (a1 - a2 g / rho1) c1 + a2 c2, c2 held in x
+template <typename T>
+__global__ void k_kcomb(int n, const double* __restrict__ ks, const T* __restrict__ c1,
+                        T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const KCoef k = kcyc_coef(ks);
+  const double al1 = k.a1 - k.a2 * k.gr;
+  x[i] = (T)(al1 * (double)c1[i] + k.a2 * (double)x[i]);
 }
 
 // ------------------------------------------------------------ host -----
@@ -565,8 +673,28 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
       for (int j = 0; j < Lc.n; ++j) invT[(size_t)j * Lc.n + i] = inv[(size_t)i * Lc.n + j];
     Lc.coarseInvT.upload(invT, s);
   }
+  int maxNb = 0;
+  for (size_t l = 1; l < lv.size(); ++l) {
+    AmgLevel& L = lv[l];
+    L.kLevel = (int)l <= opt.kcycle && !L.isCoarsest;
+    if (!L.kLevel) continue;
+    L.kc.alloc((size_t)L.n);
+    L.kv.alloc((size_t)L.n);
+    maxNb = std::max(maxNb, grid_for(L.A->rb.count, kW));
+  }
+  if (maxNb > 0) {
+    kred.init(maxNb, s);
+    kscal.alloc(8 * lv.size());
+    kscal.zero(s);
+  }
   AN_HIP(hipStreamSynchronize(s));
   built = true;
+}
+
+bool AmgHierarchy::flexible() const {
+  for (const auto& L : lv)
+    if (L.kLevel) return true;
+  return false;
 }
 
 void AmgHierarchy::refresh_fine(hipStream_t s) {
@@ -605,6 +733,10 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
     L.x32.alloc((size_t)L.n);
     L.r32.alloc((size_t)L.n);
     L.y32.alloc((size_t)L.n);
+    if (L.kLevel) {
+      L.kc32.alloc((size_t)L.n);
+      L.kv32.alloc((size_t)L.n);
+    }
   }
   has32 = true;
 }
@@ -620,6 +752,8 @@ struct LevelData<double> {
   static double* x(AmgLevel& L) { return L.x.p; }
   static double* r(AmgLevel& L) { return L.r.p; }
   static double* y(AmgLevel& L) { return L.y.p; }
+  static double* kc(AmgLevel& L) { return L.kc.p; }
+  static double* kv(AmgLevel& L) { return L.kv.p; }
 };
 template <>
 struct LevelData<float> {
@@ -629,6 +763,8 @@ struct LevelData<float> {
   static float* x(AmgLevel& L) { return L.x32.p; }
   static float* r(AmgLevel& L) { return L.r32.p; }
   static float* y(AmgLevel& L) { return L.y32.p; }
+  static float* kc(AmgLevel& L) { return L.kc32.p; }
+  static float* kv(AmgLevel& L) { return L.kv32.p; }
 };
 
 // x = V-cycle(b) on level l; y = omdinv o b (computed here when null).
@@ -636,9 +772,13 @@ struct LevelData<float> {
 // further Jacobi sweeps before the residual; after the coarse correction
 // every sweep is a Jacobi sweep.
 template <typename T>
-static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* x,
-                   hipStream_t s, int sweeps) {
+static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps);
+
+template <typename T>
+static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStream_t s,
+                   int sweeps) {
   using D = LevelData<T>;
+  std::vector<AmgLevel>& lv = H.lv;
   AmgLevel& L = lv[l];
   const DevCSR<T>& A = D::A(L);
   if (L.isCoarsest) {
@@ -690,7 +830,10 @@ static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* 
   k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, res, D::omd(C),
                                                   D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
-  vcycle<T>(lv, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
+  if (C.kLevel)
+    kcorrect<T>(H, l + 1, s, sweeps);
+  else
+    vcycle<T>(H, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
   // e = x_pre + P xc, then the post-smoothing sweeps ping-pong between r
   // and x; e starts in the buffer from which an odd/even number of sweeps
   // ends in x (in place when that buffer already holds x_pre)
@@ -705,12 +848,50 @@ static void vcycle(std::vector<AmgLevel>& lv, int l, const T* b, const T* y, T* 
   AN_CHECK_LAUNCH();
 }
 
+// K-cycle correction on level c for the right-hand side in D::b (with y =
+// omdinv o b), result in D::x (Notay & Vassilevski 2008, "Recursive
+// Krylov-based multigrid cycles": two flexible-CG steps preconditioned by
+// the level's own cycle, always both steps):
+//   c1 = B b; v1 = A c1; rho1 = c1.v1; a1 = c1.b / rho1; b -= a1 v1
+//   c2 = B b; g = c2.v1; beta = c2.Ac2 - g^2/rho1; a2 = c2.b / beta
+//   x = (a1 - a2 g / rho1) c1 + a2 c2
+// The inner products are two-stage block sums like every other reduction.
+template <typename T>
+static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
+  using D = LevelData<T>;
+  AmgLevel& L = H.lv[c];
+  const DevCSR<T>& A = D::A(L);
+  T* b = D::b(L);
+  T* y = D::y(L);
+  T* x = D::x(L);
+  T* c1 = D::kc(L);
+  T* v1 = D::kv(L);
+  double* ks = H.kscal.p + 8 * (size_t)c;
+  const int nb = grid_for(A.rb.count, kW);
+  if (nb > H.kred.cap) throw Error(7, "K-cycle reducer too small");
+  const T* none = nullptr;
+  vcycle<T>(H, c, b, y, c1, s, sweeps);
+  k_kprod<T, 2><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, c1,
+                                  b, none, v1, H.kred.partial.p);
+  AN_CHECK_LAUNCH();
+  finalize(H.kred, nb, 2, ks, s);
+  k_kres<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, ks, v1, D::omd(L), b, y);
+  AN_CHECK_LAUNCH();
+  vcycle<T>(H, c, b, y, x, s, sweeps);
+  k_kprod<T, 3><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, x, b,
+                                  v1, nullptr, H.kred.partial.p);
+  AN_CHECK_LAUNCH();
+  finalize(H.kred, nb, 3, ks + 2, s);
+  k_kcomb<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, ks, c1, x);
+  AN_CHECK_LAUNCH();
+}
+
 void AmgHierarchy::apply(const double* r, double* z, hipStream_t s) {
-  vcycle<double>(lv, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
+  vcycle<double>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
 }
 
 void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s) {
-  vcycle<float>(lv, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
+  vcycle<float>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
 }
 
 }  // namespace an

@@ -12,6 +12,7 @@ struct AmgOptions {
   int pairPasses = 2;
   double omega = 0.75;
   int sweeps = 1;  // Jacobi sweeps per side of the coarse correction
+  int kcycle = 0;  // levels 1..kcycle (not the coarsest): K-cycle correction
 };
 
 struct AmgLevel {
@@ -25,6 +26,10 @@ struct AmgLevel {
   // FP32 copies for the mixed-precision V-cycle
   DevCSR<float> A32;
   DevBuf<float> omdinv32, b32, x32, r32, y32;
+  // K-cycle level: first coarse correction c1 and its product A c1
+  DevBuf<double> kc, kv;
+  DevBuf<float> kc32, kv32;
+  bool kLevel = false;
   bool isCoarsest = false;
 };
 
@@ -45,6 +50,12 @@ class AmgHierarchy {
   void apply(const double* r, double* z, hipStream_t s);
   void apply32(const float* r, float* z, hipStream_t s);
   int levels() const { return (int)lv.size(); }
+  // some level takes the K-cycle correction: the preconditioner is not a
+  // fixed linear operator, so the outer CG uses the flexible beta
+  bool flexible() const;
+
+  Reducer kred;          // K-cycle inner products (one level at a time)
+  DevBuf<double> kscal;  // 8 scalars per level: rho1, c1.b, c2.v1, c2.Ac2, c2.b
 };
 
 // Host dense inverse used for the coarsest level (Cholesky; pivots below

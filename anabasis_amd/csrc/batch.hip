@@ -245,6 +245,7 @@ BatchSolver::BatchSolver(int nProblems, const std::vector<int>& rows,
   amg.opt.pairPasses = o.pairPasses;
   amg.opt.omega = o.jacobiOmega;
   amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
+  amg.opt.kcycle = 0;  // the K-cycle's inner products would mix the problems
   scal.alloc((size_t)S_COUNT * nP);
   scal.zero(s);
   tol2.alloc((size_t)2 * nP);  // outer, inner

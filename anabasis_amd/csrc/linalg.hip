@@ -243,6 +243,34 @@ void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hi
 }
 
 template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot2(int n, const T* __restrict__ a,
+                                                 const T* __restrict__ b,
+                                                 const T* __restrict__ c,
+                                                 double* __restrict__ partial) {
+  __shared__ double sh[2 * kWaves];
+  double v[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double bi = (double)b[i];
+    v[0] += (double)a[i] * bi;
+    v[1] += bi * (double)c[i];
+  }
+  block_sums<2>(v, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * (size_t)blockIdx.x] = v[0];
+    partial[2 * (size_t)blockIdx.x + 1] = v[1];
+  }
+}
+
+template <typename T>
+void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Reducer& red,
+          hipStream_t s) {
+  const int nb = red_blocks(n);
+  k_dot2<T><<<nb, kBlock, 0, s>>>(n, a, b, c, red.partial.p);
+  AN_CHECK_LAUNCH();
+  finalize(red, nb, 2, scal + slot, s);
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
                                                        T* __restrict__ r,
                                                        const T* __restrict__ p,
@@ -275,16 +303,17 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, T* __restrict__ p,
                                                     const T* __restrict__ z,
                                                     const double* __restrict__ scal, int num,
-                                                    int den, int first) {
-  const double beta = first ? 0.0 : scal[num] / scal[den];
+                                                    int den, int first, double sign) {
+  const double beta = first ? 0.0 : sign * scal[num] / scal[den];
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     p[i] = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
 }
 
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,
-                   hipStream_t s) {
-  k_pcg_dir<T><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, p, z, scal, num, den, first ? 1 : 0);
+                   hipStream_t s, double sign) {
+  k_pcg_dir<T><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, p, z, scal, num, den, first ? 1 : 0,
+                                                      sign);
   AN_CHECK_LAUNCH();
 }
 
@@ -326,13 +355,17 @@ template void residual<double, float>(const DevCSR<double>&, const double*, cons
                                       double*, Reducer&, hipStream_t);
 template void dot<double>(int, const double*, const double*, double*, int, Reducer&, hipStream_t);
 template void dot<float>(int, const float*, const float*, double*, int, Reducer&, hipStream_t);
+template void dot2<double>(int, const double*, const double*, const double*, double*, int,
+                           Reducer&, hipStream_t);
+template void dot2<float>(int, const float*, const float*, const float*, double*, int, Reducer&,
+                          hipStream_t);
 template void pcg_update<double>(int, double*, double*, const double*, const double*, double*, int,
                                  int, Reducer&, hipStream_t);
 template void pcg_update<float>(int, float*, float*, const float*, const float*, double*, int, int,
                                 Reducer&, hipStream_t);
 template void pcg_direction<double>(int, double*, const double*, double*, int, int, bool,
-                                    hipStream_t);
+                                    hipStream_t, double);
 template void pcg_direction<float>(int, float*, const float*, double*, int, int, bool,
-                                   hipStream_t);
+                                   hipStream_t, double);
 
 }  // namespace an

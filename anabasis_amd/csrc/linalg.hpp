@@ -84,9 +84,33 @@ __device__ __forceinline__ void stage_products(const WaveRows& q, int lane,
   }
 }
 
+// NC block sums, each in the tree order of a single block sum (valid in
+// thread 0 afterwards); sh holds NC * (blockDim.x / 64) doubles.
+template <int NC>
+__device__ __forceinline__ void block_sums(double* v, double* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    for (int off = 32; off > 0; off >>= 1) v[c] += __shfl_down(v[c], off, 64);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sh[c * nw + w] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+      for (int i = 0; i < nw; ++i) t += sh[c * nw + i];
+    v[c] = t;
+  }
+}
+
 // Scalar slots of a solver (device memory).
 enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 5, S_TMP2 = 6,
-                  S_AL = 7, S_OM = 8, S_R0V = 9, S_TS = 10, S_TT = 11, S_SS = 12, S_COUNT = 16 };
+                  S_AL = 7, S_OM = 8, S_R0V = 9, S_TS = 10, S_TT = 11, S_SS = 12,
+                  S_FRZ = 13, S_FZQ = 14,  // flexible CG: r.z, z.q
+                  S_COUNT = 16 };
 
 // Deterministic reduction workspace: per-block partials, summed in index
 // order by a one-block finalize kernel.
@@ -109,14 +133,19 @@ void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* sc
 template <typename T>
 void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hipStream_t s);
 
+// scal[slot] = a.b, scal[slot + 1] = b.c
+template <typename T>
+void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Reducer& red,
+          hipStream_t s);
+
 // x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den]
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s);
-// p = z + beta p, beta = scal[num]/scal[den]  (first: p = z)
+// p = z + beta p, beta = sign * scal[num]/scal[den]  (first: p = z)
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,
-                   hipStream_t s);
+                   hipStream_t s, double sign = 1.0);
 // r = b - A x ; scal[S_RR] = r.r (FP64 residual, output converted to RT)
 template <typename VT, typename RT>
 void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, double* scal,

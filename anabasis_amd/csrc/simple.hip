@@ -773,6 +773,8 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
                      "Continuing with approximate pCorr.\n",
                      info.iterations, info.finalRelResNorm);
       tP += info.solveSeconds;
+      pcgCum += info.iterations;
+      ++pSolveCum;
       pcgLast = info.iterations;
       pcgTotal += info.iterations;
       if (pass == nPass) {
@@ -863,6 +865,8 @@ void SimpleSolver::patch_forces(int patch, int normalSign, double* out) {
 
 void SimpleSolver::timings(double* out) {
   for (int i = 0; i < 6; ++i) out[i] = tAcc[i];
+  out[6] = (double)pcgCum;
+  out[7] = (double)pSolveCum;
 }
 
 }  // namespace an

@@ -61,6 +61,7 @@ class SimpleSolver {
   DevBuf<double> scal;
   PinnedScalars hs;
   double tAcc[6] = {0, 0, 0, 0, 0, 0};
+  long long pcgCum = 0, pSolveCum = 0;  // pressure PCG iterations / solves
   hipEvent_t ev[8];
 
   void grad(int bcKind, const double* f, DevBuf<double>* out);

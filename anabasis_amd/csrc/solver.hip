@@ -32,6 +32,7 @@ void default_pcg_options(an_pcg_options* o) {
   o->innerRelTol = 1e-4;
   o->profile = 0;
   o->sweeps = 1;
+  o->kcycle = 0;
 }
 
 static constexpr int kB = 256;
@@ -61,6 +62,7 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
   amg.opt.pairPasses = o.pairPasses;
   amg.opt.omega = o.jacobiOmega;
   amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
+  amg.opt.kcycle = o.kcycle > 0 ? o.kcycle : 0;
   AN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   AN_HIP(hipStreamSynchronize(s));
 }
@@ -136,6 +138,14 @@ double PcgSolver::read_scalar(int slot) {
   return hs.h[slot];
 }
 
+// The next preconditioner application is queued before the host knows
+// ||r|| (hiding the round trip) unless the last two residual norms predict
+// convergence in this iteration: then an application would be wasted work
+// on the stream, which short solves (warm-started corrections) feel most.
+static inline bool speculate(double rn, double rnPrev, double tol) {
+  return !(rnPrev > 0.0 && rn * (rn / rnPrev) <= tol);
+}
+
 // FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.
 int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut) {
   (void)b;
@@ -143,12 +153,17 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
   double rn = std::sqrt(hs.h[S_RR]);
   *rnormOut = rn;
   if (rn <= tol) return 0;
+  // flexible CG (K-cycle preconditioner): alpha = r.z / p.q with r.z in
+  // S_FRZ, beta = -(z.q)/(p.q) (Polak-Ribiere: z.(r_new - r_old) / r_old.z_old
+  // with r_new - r_old = -alpha q); otherwise beta = rho_new / rho_old
+  const bool flex = amg.flexible();
   amg.apply(r.p, z.p, s);
-  dot<double>(n, r.p, z.p, scal.p, S_RHO0, red, s);
+  dot<double>(n, r.p, z.p, scal.p, flex ? S_FRZ : S_RHO0, red, s);
   pcg_direction<double>(n, p.p, z.p, scal.p, 0, 0, true, s);
   int it = 1;
+  double rnPrev = 0.0;
   for (; it <= maxIter; ++it) {
-    const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+    const int rhoOld = flex ? S_FRZ : (it & 1) ? S_RHO0 : S_RHO1;
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
@@ -156,36 +171,48 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));
-    // next direction while the host waits for ||r||
-    EvPair* e2 = next_ev(vcEv, vcUsed);
-    if (e2) AN_HIP(hipEventRecord(e2->a, s));
-    amg.apply(r.p, z.p, s);
-    if (e2) AN_HIP(hipEventRecord(e2->b, s));
-    dot<double>(n, r.p, z.p, scal.p, rhoNew, red, s);
-    pcg_direction<double>(n, p.p, z.p, scal.p, rhoNew, rhoOld, false, s);
+    auto next_direction = [&]() {
+      EvPair* e2 = next_ev(vcEv, vcUsed);
+      if (e2) AN_HIP(hipEventRecord(e2->a, s));
+      amg.apply(r.p, z.p, s);
+      if (e2) AN_HIP(hipEventRecord(e2->b, s));
+      if (flex) {
+        dot2<double>(n, r.p, z.p, q.p, scal.p, S_FRZ, red, s);
+        pcg_direction<double>(n, p.p, z.p, scal.p, S_FZQ, S_PQ, false, s, -1.0);
+      } else {
+        dot<double>(n, r.p, z.p, scal.p, rhoNew, red, s);
+        pcg_direction<double>(n, p.p, z.p, scal.p, rhoNew, rhoOld, false, s);
+      }
+    };
+    const bool spec = speculate(rn, rnPrev, tol);
+    if (spec) next_direction();  // queued while the host waits for ||r||
     AN_HIP(hipEventSynchronize(ev));
+    rnPrev = rn;
     rn = std::sqrt(hs.h[S_RR]);
     *rnormOut = rn;
     if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) return -it;
     if (rn <= tol) break;
+    if (!spec) next_direction();
   }
   AN_HIP(hipStreamSynchronize(s));
   return it > maxIter ? -maxIter : it;
 }
 
 // FP32 inner PCG from x = 0 on A32 with FP32 AMG.
-int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double* rnormOut) {
+int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double rn0,
+                     double* rnormOut) {
   const int n = A.n;
   const DevCSR<float>& A32 = amg.lv[0].A32;
   fill_zero(x, sizeof(float) * (size_t)n, s);
   AN_HIP(hipMemcpyAsync(r32.p, b, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  const bool flex = amg.flexible();
   amg.apply32(r32.p, z32.p, s);
-  dot<float>(n, r32.p, z32.p, scal.p, S_RHO0, red, s);
+  dot<float>(n, r32.p, z32.p, scal.p, flex ? S_FRZ : S_RHO0, red, s);
   pcg_direction<float>(n, p32.p, z32.p, scal.p, 0, 0, true, s);
   int it = 1;
-  double rn = 0.0;
+  double rn = rn0, rnPrev = 0.0;
   for (; it <= maxIter; ++it) {
-    const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+    const int rhoOld = flex ? S_FRZ : (it & 1) ? S_RHO0 : S_RHO1;
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
@@ -193,17 +220,28 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double* 
     pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));
-    EvPair* e2 = next_ev(vcEv, vcUsed);
-    if (e2) AN_HIP(hipEventRecord(e2->a, s));
-    amg.apply32(r32.p, z32.p, s);
-    if (e2) AN_HIP(hipEventRecord(e2->b, s));
-    dot<float>(n, r32.p, z32.p, scal.p, rhoNew, red, s);
-    pcg_direction<float>(n, p32.p, z32.p, scal.p, rhoNew, rhoOld, false, s);
+    auto next_direction = [&]() {
+      EvPair* e2 = next_ev(vcEv, vcUsed);
+      if (e2) AN_HIP(hipEventRecord(e2->a, s));
+      amg.apply32(r32.p, z32.p, s);
+      if (e2) AN_HIP(hipEventRecord(e2->b, s));
+      if (flex) {
+        dot2<float>(n, r32.p, z32.p, q32.p, scal.p, S_FRZ, red, s);
+        pcg_direction<float>(n, p32.p, z32.p, scal.p, S_FZQ, S_PQ, false, s, -1.0);
+      } else {
+        dot<float>(n, r32.p, z32.p, scal.p, rhoNew, red, s);
+        pcg_direction<float>(n, p32.p, z32.p, scal.p, rhoNew, rhoOld, false, s);
+      }
+    };
+    const bool spec = speculate(rn, rnPrev, tol);
+    if (spec) next_direction();
     AN_HIP(hipEventSynchronize(ev));
+    rnPrev = rn;
     rn = std::sqrt(hs.h[S_RR]);
     *rnormOut = rn;
     if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) break;
     if (rn <= tol) break;
+    if (!spec) next_direction();
   }
   AN_HIP(hipStreamSynchronize(s));
   return it > maxIter ? maxIter : it;
@@ -214,8 +252,14 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
   const int n = A.n;
   spmvUsed = vcUsed = 0;
   auto t0 = std::chrono::steady_clock::now();
+  // ||b|| and the initial residual in one round trip (the residual is
+  // unused when b == 0)
   dot<double>(n, b, b, scal.p, S_BB, red, s);
-  const double bnorm = std::sqrt(read_scalar(S_BB));
+  residual<double, double>(A, b, x, r.p, scal.p, red, s);
+  AN_HIP(hipMemcpyAsync(hs.h + S_RR, scal.p + S_RR, (S_BB - S_RR + 1) * sizeof(double),
+                        hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(hs.h[S_BB]);
   an_solve_info inf{};
   inf.levels = amg.levels();
   inf.setupSeconds = lastSetupSeconds;
@@ -226,8 +270,6 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
     inf.converged = 1;
   } else {
     const double tol = std::fmax(opt.relTol * bnorm, opt.absTol);
-    residual<double, double>(A, b, x, r.p, scal.p, red, s);
-    read_scalar(S_RR);
     double rn = std::sqrt(hs.h[S_RR]);
     if (opt.precision != 1) {
       int it = pcg64(b, x, tol, opt.maxIter, &rn);
@@ -244,7 +286,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
         double rin_n = 0.0;
         // the FP32 PCG overwrites r32; keep the right-hand side in q32
         AN_HIP(hipMemcpyAsync(q32.p, rin, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
-        int it = pcg32(q32.p, e32.p, innerTol, opt.maxIter - total, &rin_n);
+        int it = pcg32(q32.p, e32.p, innerTol, opt.maxIter - total, rn, &rin_n);
         total += it;
         axpy_f2d(n, x, e32.p, s);
         residual<double, double>(A, b, x, r.p, scal.p, red, s);

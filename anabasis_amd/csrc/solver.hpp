@@ -48,7 +48,8 @@ class PcgSolver {
 
  private:
   int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
-  int pcg32(const float* b, float* x, double tol, int maxIter, double* rnormOut);
+  // rn0: norm of b (the outer residual) for the speculation rule
+  int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
   double read_scalar(int slot);
   EvPair* next_ev(std::vector<EvPair>& pool, size_t& used);
   void collect_profile(an_solve_info* inf);

@@ -233,10 +233,10 @@ class Simple:
         return out
 
     def timings(self):
-        out = np.empty(6)
+        out = np.empty(8)
         check(lib().an_simple_timings(self._h, ptr(out)))
-        return dict(zip(("mom_asm", "mom_solve", "p_asm_setup", "p_solve", "other", "total"),
-                        out.tolist()))
+        return dict(zip(("mom_asm", "mom_solve", "p_asm_setup", "p_solve", "other", "total",
+                         "p_pcg_iterations", "p_solves"), out.tolist()))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -65,16 +65,22 @@ def run_simple(m, vel, prs, precision, warmup, steps):
     st, _ = S.iterate(warmup)
     log(f"SIMPLE prec={precision} warmup {warmup} its {time.time() - t0:.2f}s "
         f"massRes {st.massRes:.3e}")
+    tm0 = S.timings()
     t0 = time.time()
     st, conv = S.iterate(steps)
     dt = time.time() - t0
     tm = S.timings()
+    timed = {k: tm[k] - tm0[k] for k in tm}  # the timed iterations only
     force = S.patch_forces("patch_3_0", -1)
+    per_it = timed["p_solve"] / max(timed["p_pcg_iterations"], 1)
     log(f"SIMPLE prec={precision}: {steps} its {dt:.2f}s -> {steps / dt:.3f} it/s; last massRes "
-        f"{st.massRes:.3e} bicgstab {list(st.bicgstabIts)} pcgTotal {st.pcgTotal}")
+        f"{st.massRes:.3e} bicgstab {list(st.bicgstabIts)} pcgTotal {st.pcgTotal}; timed: "
+        f"p_solve {timed['p_solve'] / steps * 1e3:.1f} ms/it, "
+        f"{timed['p_pcg_iterations'] / steps:.1f} PCG its/it over "
+        f"{timed['p_solves'] / steps:.0f} solves, {per_it * 1e6:.0f} us per PCG it")
     return dict(iters_per_s=steps / dt, seconds=dt, massRes=st.massRes,
                 bicgstab_its=list(st.bicgstabIts), pcg_total=st.pcgTotal, converged=conv,
-                phase_seconds_cumulative=tm, force_cylinder=force[6:9].tolist(),
+                phase_seconds_timed=timed, force_cylinder=force[6:9].tolist(),
                 fields={k: S.field(k) for k in "uvwp"})
 
 

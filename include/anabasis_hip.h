@@ -160,6 +160,11 @@ typedef struct {
   int profile;         /* 1: time every SpMV / V-cycle with HIP events */
   int sweeps;          /* AMG: damped-Jacobi sweeps before and after the
                           coarse correction on every level (>= 1) */
+  int kcycle;          /* AMG: coarse levels 1..kcycle (never the coarsest)
+                          take their correction from two flexible-CG steps
+                          preconditioned by the next cycle (K-cycle); the
+                          outer CG then uses the flexible (Polak-Ribiere)
+                          beta.  0: V-cycle.  Ignored by the batched solver */
 } an_pcg_options;
 void an_pcg_default_options(an_pcg_options *o);
 
@@ -382,7 +387,8 @@ an_status an_simple_patch_forces(an_simple_t s, int patch, int normalSign,
                                  double *out);
 /* Accumulated seconds per phase (hipEvent timed): out[0] momentum assembly,
  * [1] momentum solve, [2] pressure assembly+setup, [3] pressure solve,
- * [4] other kernels, [5] total. */
+ * [4] other kernels, [5] total; then counts over the same iterations:
+ * [6] pressure PCG iterations, [7] pressure solves (out holds 8 values). */
 an_status an_simple_timings(an_simple_t s, double *out);
 an_status an_simple_destroy(an_simple_t s);
 

@@ -221,6 +221,11 @@ typedef struct {
   int pairPasses;      /* pairwise passes per level (2 -> ~4:1) */
   double jacobiOmega;  /* Jacobi smoother damping */
   int sweeps;          /* pre = post sweeps */
+  int kcycle;          /* coarse levels 1..kcycle (never the coarsest) take
+                          their correction from two flexible-CG steps
+                          preconditioned by the next cycle (K-cycle); the
+                          outer PCG then uses the flexible (Polak-Ribiere)
+                          beta.  0: V-cycle */
 } or_amg_options;
 
 int or_pcg_amg(int n, const int *rowOffsets, const int *cols,
@@ -233,10 +238,15 @@ int or_amg_aggregate(int n, const int *rowOffsets, const int *cols,
                      const double *vals, const or_amg_options *opt,
                      int *agg);
 
-/* One V-cycle z = M^-1 r with a freshly built hierarchy; returns levels. */
+/* One cycle z = M^-1 r with a freshly built hierarchy; returns levels. */
 int or_amg_vcycle(int n, const int *rowOffsets, const int *cols,
                   const double *vals, const or_amg_options *opt,
                   const double *r, double *z);
+/* The K-cycle step (two flexible-CG steps preconditioned by the plain cycle)
+ * on the given matrix as its own level; returns levels. */
+int or_amg_kstep(int n, const int *rowOffsets, const int *cols,
+                 const double *vals, const or_amg_options *opt,
+                 const double *b, double *x);
 
 /* Reusable PCG + AMG: hierarchy kept between solves; set_values with
  * rebuild == 0 refreshes only the level-0 smoother (stale coarse levels). */

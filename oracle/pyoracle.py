@@ -44,13 +44,14 @@ def _i(a):
 class AmgOptions(C.Structure):
     _fields_ = [("maxIter", C.c_int), ("relTol", C.c_double), ("absTol", C.c_double),
                 ("coarseSize", C.c_int), ("maxLevels", C.c_int), ("strength", C.c_double),
-                ("pairPasses", C.c_int), ("jacobiOmega", C.c_double), ("sweeps", C.c_int)]
+                ("pairPasses", C.c_int), ("jacobiOmega", C.c_double), ("sweeps", C.c_int),
+                ("kcycle", C.c_int)]
 
 
 def amg_options(maxIter=4000, relTol=0.0, absTol=1e-12, coarseSize=256, maxLevels=25,
-                strength=0.25, pairPasses=2, jacobiOmega=0.75, sweeps=1):
+                strength=0.25, pairPasses=2, jacobiOmega=0.75, sweeps=1, kcycle=0):
     return AmgOptions(maxIter, relTol, absTol, coarseSize, maxLevels, strength, pairPasses,
-                      jacobiOmega, sweeps)
+                      jacobiOmega, sweeps, kcycle)
 
 
 class OMesh:
@@ -186,6 +187,14 @@ def amg_aggregate(ro, ci, va, **opts):
     o = amg_options(**opts)
     nc = lib().or_amg_aggregate(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), C.byref(o), _p(agg))
     return agg, nc
+
+
+def amg_kstep(ro, ci, va, b, **opts):
+    n = len(ro) - 1
+    x = np.empty(n)
+    o = amg_options(**opts)
+    nl = lib().or_amg_kstep(n, _p(_i(ro)), _p(_i(ci)), _p(_d(va)), C.byref(o), _p(_d(b)), _p(x))
+    return x, nl
 
 
 def amg_vcycle(ro, ci, va, r, **opts):

@@ -321,7 +321,8 @@ typedef struct {
   int nLevels;
   or_level lv[40];
   double **xb, **bb, **rb; /* per-level work vectors */
-  int sweeps;
+  double **kc, **kv;        /* K-cycle levels: first correction, its product */
+  int sweeps, kcycle;
 } or_amg;
 
 static void amg_free(or_amg *H) {
@@ -341,10 +342,14 @@ static void amg_free(or_amg *H) {
     free(H->xb[l]);
     free(H->bb[l]);
     free(H->rb[l]);
+    free(H->kc[l]);
+    free(H->kv[l]);
   }
   free(H->xb);
   free(H->bb);
   free(H->rb);
+  free(H->kc);
+  free(H->kv);
 }
 
 static void level_diag(or_level *L, double omega) {
@@ -363,6 +368,7 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
                       const double *va, const or_amg_options *o) {
   memset(H, 0, sizeof(*H));
   H->sweeps = o->sweeps > 0 ? o->sweeps : 1;
+  H->kcycle = o->kcycle > 0 ? o->kcycle : 0;
   int maxL = o->maxLevels > 1 ? o->maxLevels : 2;
   if (maxL > 40) maxL = 40;
   or_level *L = &H->lv[0];
@@ -437,11 +443,24 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
   H->xb = (double **)calloc((size_t)H->nLevels, sizeof(double *));
   H->bb = (double **)calloc((size_t)H->nLevels, sizeof(double *));
   H->rb = (double **)calloc((size_t)H->nLevels, sizeof(double *));
+  H->kc = (double **)calloc((size_t)H->nLevels, sizeof(double *));
+  H->kv = (double **)calloc((size_t)H->nLevels, sizeof(double *));
   for (int k = 0; k < H->nLevels; ++k) {
     H->xb[k] = (double *)calloc((size_t)H->lv[k].n, sizeof(double));
     H->bb[k] = (double *)calloc((size_t)H->lv[k].n, sizeof(double));
     H->rb[k] = (double *)calloc((size_t)H->lv[k].n, sizeof(double));
+    if (k >= 1 && k <= H->kcycle && !H->lv[k].isCoarsest) {
+      H->kc[k] = (double *)calloc((size_t)H->lv[k].n, sizeof(double));
+      H->kv[k] = (double *)calloc((size_t)H->lv[k].n, sizeof(double));
+    }
   }
+}
+
+/* 1 when some level takes the K-cycle correction (outer PCG then flexible) */
+static int amg_flexible(const or_amg *H) {
+  for (int k = 1; k < H->nLevels; ++k)
+    if (H->kc[k]) return 1;
+  return 0;
 }
 
 /* one damped Jacobi sweep x += omega D^-1 (b - A x) (Jacobi: uses old x) */
@@ -453,6 +472,36 @@ static void jacobi_sweep(const or_level *L, const double *b, double *x,
     tmp[i] = x[i] + L->omdinv[i] * (b[i] - s);
   }
   memcpy(x, tmp, sizeof(double) * (size_t)L->n);
+}
+
+static void vcycle(or_amg *H, int l, const double *b, double *x);
+
+/* K-cycle coarse correction on level c (Notay & Vassilevski 2008, "Recursive
+ * Krylov-based multigrid cycles", two flexible-CG steps with the next cycle
+ * as preconditioner, no early exit):
+ *   c1 = B b, v1 = A c1, rho1 = c1.v1, a1 = c1.b / rho1
+ *   b <- b - a1 v1 (in place: the level's right-hand side buffer)
+ *   c2 = B b, w = A c2, g = c2.v1, d = c2.w, e = c2.b
+ *   beta = d - g^2/rho1, a2 = e / beta
+ *   x = (a1 - a2 g / rho1) c1 + a2 c2
+ * with a zero rho1 (resp. beta <= 0) dropping the terms it divides. */
+static void kcorrect(or_amg *H, int c, double *b, double *x) {
+  or_level *L = &H->lv[c];
+  const int n = L->n;
+  double *c1 = H->kc[c], *v1 = H->kv[c], *w = H->rb[c];
+  vcycle(H, c, b, c1);
+  or_spmv(n, L->ro, L->ci, L->va, c1, v1);
+  const double rho1 = dotv(n, c1, v1), g1 = dotv(n, c1, b);
+  const double a1 = rho1 > 0.0 ? g1 / rho1 : 0.0;
+  for (int i = 0; i < n; ++i) b[i] = b[i] - a1 * v1[i];
+  vcycle(H, c, b, x);
+  or_spmv(n, L->ro, L->ci, L->va, x, w);
+  const double g = dotv(n, x, v1), d = dotv(n, x, w), e = dotv(n, x, b);
+  const double gr = rho1 > 0.0 ? g / rho1 : 0.0;
+  const double beta = d - g * gr;
+  const double a2 = beta > 0.0 ? e / beta : 0.0;
+  const double al1 = a1 - a2 * gr;
+  for (int i = 0; i < n; ++i) x[i] = al1 * c1[i] + a2 * x[i];
 }
 
 static void vcycle(or_amg *H, int l, const double *b, double *x) {
@@ -497,7 +546,10 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
     bc[I] = s;
   }
   double *xc = H->xb[l + 1];
-  vcycle(H, l + 1, bc, xc);
+  if (H->kc[l + 1])
+    kcorrect(H, l + 1, bc, xc);
+  else
+    vcycle(H, l + 1, bc, xc);
   for (int i = 0; i < L->n; ++i) x[i] = x[i] + xc[L->agg[i]];
   for (int s = 0; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
 }
@@ -532,6 +584,7 @@ int or_pcg_amg(int n, const int *ro, const int *ci, const double *va,
   double *q = (double *)malloc(sizeof(double) * (size_t)n);
   double bnorm = sqrt(dotv(n, b, b));
   int it = 0, conv = 0;
+  const int flex = amg_flexible(&H);
   double rnorm = 0.0, tol = 0.0, rho = 0.0;
   if (bnorm == 0.0) {
     memset(x, 0, sizeof(double) * (size_t)n);
@@ -570,7 +623,10 @@ int or_pcg_amg(int n, const int *ro, const int *ci, const double *va,
     }
     vcycle(&H, 0, r, z);
     double rhoNew = dotv(n, r, z);
-    double beta = rhoNew / rho;
+    /* flexible (Polak-Ribiere) beta for the K-cycle, = rhoNew/rho for a
+       fixed preconditioner: -(z.(r_new - r_old))/rho with r_new - r_old =
+       -alpha q, alpha = rho/pq */
+    double beta = flex ? -dotv(n, z, q) / pq : rhoNew / rho;
     for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     rho = rhoNew;
   }
@@ -687,6 +743,26 @@ int or_amg_vcycle(int n, const int *ro, const int *ci, const double *va,
   return nl;
 }
 
+/* The K-cycle's two flexible-CG steps applied on the given matrix itself,
+ * with the plain cycle of its own hierarchy as the inner preconditioner
+ * (test hook for the Galerkin conditions of the step). */
+int or_amg_kstep(int n, const int *ro, const int *ci, const double *va,
+                 const or_amg_options *opt, const double *b, double *x) {
+  or_amg H;
+  or_amg_options o = *opt;
+  o.kcycle = 0;
+  amg_setup(&H, n, ro, ci, va, &o);
+  H.kc[0] = (double *)calloc((size_t)n, sizeof(double));
+  H.kv[0] = (double *)calloc((size_t)n, sizeof(double));
+  double *bw = (double *)malloc(sizeof(double) * (size_t)n);
+  memcpy(bw, b, sizeof(double) * (size_t)n);
+  kcorrect(&H, 0, bw, x);
+  free(bw);
+  int nl = H.nLevels;
+  amg_free(&H);
+  return nl;
+}
+
 /* ---------------------------------------------- reusable PCG + AMG ---- */
 /* Hierarchy kept between solves, as the reference keeps its BoomerAMG setup
  * between pAmgRebuildEvery rebuilds (GSV1 main.cu:3407-3411); new fine
@@ -732,12 +808,13 @@ int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
   int n = h->n;
   const int *ro = h->ro, *ci = h->ci;
   const double *va = h->va;
-  double *r = (double *)malloc(sizeof(double) * (size_t)n);
+  double *r = (double *)calloc((size_t)n, sizeof(double));
   double *z = (double *)malloc(sizeof(double) * (size_t)n);
   double *p = (double *)malloc(sizeof(double) * (size_t)n);
   double *q = (double *)malloc(sizeof(double) * (size_t)n);
   double bnorm = sqrt(dotv(n, b, b));
   int it = 0, conv = 0;
+  const int flex = amg_flexible(&h->H);
   double rnorm = 0.0, tol = 0.0, rho = 0.0;
   if (bnorm == 0.0) {
     memset(x, 0, sizeof(double) * (size_t)n);
@@ -773,7 +850,7 @@ int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
     }
     vcycle(&h->H, 0, r, z);
     double rhoNew = dotv(n, r, z);
-    double beta = rhoNew / rho;
+    double beta = flex ? -dotv(n, z, q) / pq : rhoNew / rho;
     for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     rho = rhoNew;
   }

@@ -164,6 +164,55 @@ def test_spmv_amg_vcycle_bitexact(meshes, omeshes, kind):
         assert np.array_equal(z2, zo2), sweeps
 
 
+@pytest.mark.parametrize("kind", ["hex", "cyl"])
+def test_kcycle_matches_oracle(meshes, omeshes, kind):
+    """K-cycle preconditioner: the oracle's cycle; its inner products are
+    two-stage block sums (another summation order than the oracle's serial
+    dot), so the two agree to rounding, and the GPU result is reproducible
+    bit for bit."""
+    m, om = meshes[kind], omeshes[kind]
+    dirich = ("xmin",) if kind == "hex" else ("patch_2_0",)
+    vals = laplacian(om, m, dirich)
+    x = np.random.default_rng(2).standard_normal(om.nCells)
+    for kc in ((1,) if kind == "hex" else (1, 2)):
+        for sweeps in (1, 2):
+            S = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64, kcycle=kc, sweeps=sweeps)
+            S.set_values(vals)
+            z = S.amg_apply(x)
+            assert np.array_equal(z, S.amg_apply(x))
+            zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, kcycle=kc,
+                                  sweeps=sweeps)
+            zv, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps)
+            assert nl >= kc + 2
+            assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo), (kc, sweeps)
+            assert np.linalg.norm(zv - zo) > 1e-4 * np.linalg.norm(zo)
+
+
+def test_pcg_kcycle_flexible_converges(meshes, omeshes):
+    m, om = meshes["cyl"], omeshes["cyl"]
+    vals = laplacian(om, m, ("patch_2_0",))
+    b = np.random.default_rng(8).standard_normal(om.nCells)
+    A = sp.csr_matrix((vals, om.cols, om.rowOffsets))
+    xd = spla.spsolve(A.tocsc(), b)
+    xo, ito, rr, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-12, absTol=0.0,
+                               coarseSize=64, kcycle=2)
+    V = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-12, absTol=0.0, coarseSize=64)
+    V.set_values(vals)
+    _, iv = V.solve(b)
+    for precision in (0, 1):
+        S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-12, absTol=0.0, coarseSize=64,
+                         kcycle=2, precision=precision)
+        S.set_values(vals)
+        x, info = S.solve(b)
+        assert info.converged and info.finalRelResNorm <= 1e-12
+        assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+        if precision == 0:
+            assert abs(info.iterations - ito) <= 2
+            assert info.iterations < iv.iterations
+            x2, _ = S.solve(b)
+            assert np.array_equal(x, x2)
+
+
 def test_pcg_amg_converges_to_direct_solution(meshes, omeshes):
     m, om = meshes["hex"], omeshes["hex"]
     vals = laplacian(om, m)

@@ -362,6 +362,75 @@ def test_vcycle_is_linear_and_symmetric():
     assert r1 @ z1 > 0 and r2 @ z2 > 0
 
 
+def test_kcycle_step_satisfies_galerkin_conditions():
+    """The K-cycle step is two flexible-CG steps: with c1 = B b and c2 =
+    B(b - a1 A c1), a1 = c1.b / c1.Ac1 (B = the plain cycle, applied here
+    separately), the result is the A-orthogonal projection of the solution
+    onto span{c1, c2}: x = V y with (V^T A V) y = V^T b."""
+    om, vals = spd_system(12, 11, 10)
+    A = csr(om, vals)
+    b = np.random.default_rng(5).standard_normal(om.nCells)
+    kw = dict(coarseSize=40)
+    x, nl = O.amg_kstep(om.rowOffsets, om.cols, vals, b, **kw)
+    assert nl >= 3
+    c1, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, b, **kw)
+    a1 = (c1 @ b) / (c1 @ (A @ c1))
+    c2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, b - a1 * (A @ c1), **kw)
+    V = np.stack([c1, c2], 1)
+    y = np.linalg.solve(V.T @ (A @ V), V.T @ b)
+    assert np.linalg.norm(V @ y - x) <= 1e-10 * np.linalg.norm(x)
+    # and it is a better correction than one cycle (A-norm error)
+    xs = spla.spsolve(A.tocsc(), b)
+    ea = lambda e: np.sqrt(e @ (A @ e))
+    assert ea(xs - x) < ea(xs - c1)
+
+
+def test_kcycle_level_one_composes_with_the_fine_level():
+    """kcycle=1 on a three-level hierarchy: the fine level's pre-smoothing,
+    restriction, prolongation and post-smoothing around the K step of the
+    level-1 Galerkin matrix (rebuilt here with scipy from the aggregates)."""
+    om, vals = spd_system(12, 11, 10, seed=3)
+    A = csr(om, vals)
+    n = om.nCells
+    kw = dict(coarseSize=40, maxLevels=3)
+    agg, nc = O.amg_aggregate(om.rowOffsets, om.cols, vals)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, nc))
+    A1 = (P.T @ A @ P).tocsr()
+    A1.sort_indices()
+    b = np.random.default_rng(6).standard_normal(n)
+    z, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, b, kcycle=1, **kw)
+    assert nl == 3
+    omd = 0.75 / A.diagonal()
+    x0 = omd * b
+    bc = P.T @ (b - A @ x0)
+    xc, nl1 = O.amg_kstep(A1.indptr.astype(np.int32), A1.indices.astype(np.int32), A1.data, bc,
+                          coarseSize=40, maxLevels=2)
+    assert nl1 == 2
+    x = x0 + P @ xc
+    x = x + omd * (b - A @ x)
+    assert np.linalg.norm(z - x) <= 1e-10 * np.linalg.norm(z)
+    zv, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, b, **kw)
+    assert np.linalg.norm(zv - z) > 1e-3 * np.linalg.norm(z)
+    # homogeneous of degree one (exact: scaling by two is exact)
+    z2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, 2.0 * b, kcycle=1, **kw)
+    assert np.array_equal(z2, 2.0 * z)
+
+
+def test_pcg_kcycle_converges_in_fewer_iterations():
+    om, vals = spd_system(16, 15, 14)
+    A = csr(om, vals)
+    b = np.random.default_rng(7).standard_normal(om.nCells)
+    xd = spla.spsolve(A.tocsc(), b)
+    its = {}
+    for kc in (0, 1, 2):
+        x, it, rr, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-12, absTol=0.0,
+                                 coarseSize=40, kcycle=kc)
+        assert it > 0 and rr <= 1e-12
+        assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+        its[kc] = it
+    assert its[1] <= 0.75 * its[0] and its[2] <= its[1], its
+
+
 # ------------------------------------------------- LP / LS boundary rules
 @pytest.mark.parametrize("path", ["lp", "ls"])
 def test_linear_field_exact_with_neumann_data(path):
