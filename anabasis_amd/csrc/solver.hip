@@ -27,12 +27,12 @@ void default_pcg_options(an_pcg_options* o) {
   o->coarseSize = 256;
   o->maxLevels = 25;
   o->strength = 0.25;
-  o->pairPasses = 2;
-  o->jacobiOmega = 0.75;
+  o->pairPasses = 3;
+  o->jacobiOmega = 0.85;
   o->innerRelTol = 1e-4;
   o->profile = 0;
   o->sweeps = 1;
-  o->kcycle = 0;
+  o->kcycle = 2;
 }
 
 static constexpr int kB = 256;

@@ -49,7 +49,7 @@ class AmgOptions(C.Structure):
 
 
 def amg_options(maxIter=4000, relTol=0.0, absTol=1e-12, coarseSize=256, maxLevels=25,
-                strength=0.25, pairPasses=2, jacobiOmega=0.75, sweeps=1, kcycle=0):
+                strength=0.25, pairPasses=3, jacobiOmega=0.85, sweeps=1, kcycle=2):
     return AmgOptions(maxIter, relTol, absTol, coarseSize, maxLevels, strength, pairPasses,
                       jacobiOmega, sweeps, kcycle)
 
@@ -234,7 +234,8 @@ def simple_params_from(gp):
     pr = gp.pressure
     amg = amg_options(maxIter=gp.pMaxit, relTol=gp.pRelTol, absTol=gp.pTol,
                       coarseSize=pr.coarseSize, maxLevels=pr.maxLevels, strength=pr.strength,
-                      pairPasses=pr.pairPasses, jacobiOmega=pr.jacobiOmega)
+                      pairPasses=pr.pairPasses, jacobiOmega=pr.jacobiOmega, sweeps=pr.sweeps,
+                      kcycle=pr.kcycle)
     return SimpleParams(gp.rho, gp.mu, gp.nVelNonOrthCorr, gp.nNonOrthCorr, gp.uRelax,
                         gp.pRelax, gp.corrTol, gp.tolMass, gp.tolVel, gp.velMaxit, gp.velTol,
                         gp.velRelTol, gp.pMaxit, gp.pTol, gp.pRelTol, gp.pAmgRebuildEvery, amg)

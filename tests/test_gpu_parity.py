@@ -145,7 +145,7 @@ def test_spmv_amg_vcycle_bitexact(meshes, omeshes, kind):
     m, om = meshes[kind], omeshes[kind]
     dirich = ("xmin",) if kind == "hex" else ("patch_2_0",)
     vals = laplacian(om, m, dirich)
-    S = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64, kcycle=0)
     S.set_values(vals)
     x = np.random.default_rng(1).standard_normal(om.nCells)
     assert np.array_equal(S.spmv(x), O.spmv(om.rowOffsets, om.cols, vals, x))
@@ -154,13 +154,14 @@ def test_spmv_amg_vcycle_bitexact(meshes, omeshes, kind):
     assert nc == rnc and np.array_equal(agg, ragg)
     # whole hierarchy (Galerkin sums, smoother weights, dense coarse inverse)
     z = S.amg_apply(x)
-    zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64)
+    zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, kcycle=0)
     assert nl >= 3 and np.array_equal(z, zo)
     for sweeps in (2, 3):  # multi-sweep smoothing (buffer ping-pong on every level)
-        S2 = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64, sweeps=sweeps)
+        S2 = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=64, sweeps=sweeps, kcycle=0)
         S2.set_values(vals)
         z2 = S2.amg_apply(x)
-        zo2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps)
+        zo2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps,
+                              kcycle=0)
         assert np.array_equal(z2, zo2), sweeps
 
 
@@ -182,7 +183,8 @@ def test_kcycle_matches_oracle(meshes, omeshes, kind):
             assert np.array_equal(z, S.amg_apply(x))
             zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, kcycle=kc,
                                   sweeps=sweeps)
-            zv, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps)
+            zv, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, sweeps=sweeps,
+                                 kcycle=0)
             assert nl >= kc + 2
             assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo), (kc, sweeps)
             assert np.linalg.norm(zv - zo) > 1e-4 * np.linalg.norm(zo)
@@ -196,7 +198,7 @@ def test_pcg_kcycle_flexible_converges(meshes, omeshes):
     xd = spla.spsolve(A.tocsc(), b)
     xo, ito, rr, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-12, absTol=0.0,
                                coarseSize=64, kcycle=2)
-    V = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-12, absTol=0.0, coarseSize=64)
+    V = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-12, absTol=0.0, coarseSize=64, kcycle=0)
     V.set_values(vals)
     _, iv = V.solve(b)
     for precision in (0, 1):
