@@ -243,6 +243,23 @@ __global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict_
   if (i < n) b[i] = (T)a[i];
 }
 
+// Coefficients of the two flexible-CG steps from ks = {c1.Ac1, c1.b,
+// c2.v1, c2.Ac2, c2.b}; a zero c1.Ac1 (beta <= 0) drops the terms it divides.
+struct KCoef {
+  double a1, gr, a2;
+};
+__device__ __forceinline__ double kcyc_a1(const double* ks) {
+  return ks[0] > 0.0 ? ks[1] / ks[0] : 0.0;
+}
+__device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
+  KCoef k;
+  k.a1 = kcyc_a1(ks);
+  k.gr = ks[0] > 0.0 ? ks[2] / ks[0] : 0.0;
+  const double beta = ks[3] - ks[2] * k.gr;
+  k.a2 = beta > 0.0 ? ks[4] / beta : 0.0;
+  return k;
+}
+
 // --------------------------------------------------------- V-cycle -----
 // One level of the V-cycle (omega-damped Jacobi, one sweep each way, from
 // x = 0), with y = omdinv o b supplied by the producer of b:
@@ -251,7 +268,10 @@ __global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict_
 //   prolong:  e_i = y_i + xc[agg_i]
 //   up:       x_i = e_i + omdinv_i (b_i - sum_j a_ij e_j)   (KIND 1, v = e)
 // so the two CSR-stream kernels gather a single vector, like an SpMV.
-template <typename T, int KIND>
+// DOT (KIND 1 only) adds
+// the partial sums of b_i x_i (DOT 1) and x_i qv_i (DOT 2) of the result and
+// finishes them in the kernel: the preconditioned CG's r.z and z.q.
+template <typename T, int KIND, int DOT = 0>
 __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restrict__ rbs,
                                                    const int* __restrict__ ro,
                                                    const int* __restrict__ ci,
@@ -259,12 +279,15 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
                                                    const T* __restrict__ omdinv,
                                                    const T* __restrict__ b,
                                                    const T* __restrict__ v,
-                                                   T* __restrict__ out) {
+                                                   T* __restrict__ out,
+                                                   const T* __restrict__ qv, RedArgs ra) {
   __shared__ double prod[kW][kCap];
+  __shared__ double sh[2 * kW];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
   const int i = q.r0 + lane;
+  auto vj = [&](int j) { return (double)v[j]; };
   int kb = 0, ke = 0;
   double bi = 0.0, vi = 0.0, oi = 0.0;
   if (i < q.r1) {
@@ -276,21 +299,46 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
       oi = (double)omdinv[i];
     }
   }
-  if (q.stream && q.k1 > q.k0)
-    stage_products(q, lane, ci, va, [&](int j) { return (double)v[j]; }, prod[w]);
+  if (q.stream && q.k1 > q.k0) stage_products(q, lane, ci, va, vj, prod[w]);
   wave_sync();
+  double d[2] = {0.0, 0.0};
   if (i < q.r1) {
     double s = 0.0;
     if (q.stream) {
       for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     } else {
-      for (int k = kb; k < ke; ++k) s += (double)va[k] * (double)v[ci[k]];
+      for (int k = kb; k < ke; ++k) s += (double)va[k] * vj(ci[k]);
     }
-    if (KIND == 0)
+    if (KIND == 0) {
       out[i] = (T)(bi - s);
-    else
-      out[i] = (T)(vi + oi * (bi - s));
+    } else {
+      const T o = (T)(vi + oi * (bi - s));
+      out[i] = o;
+      if (DOT >= 1) d[0] = bi * (double)o;
+      if (DOT == 2) d[1] = (double)o * (double)qv[i];
+    }
   }
+  if constexpr (DOT > 0) {
+    block_sums<DOT>(d, sh);
+    store_partials<DOT>(d, lb, ra);
+  }
+}
+
+// r.z (and z.q) of a preconditioner application that ended on the coarsest
+// level (no up kernel to carry them)
+template <typename T, int DOT>
+__global__ __launch_bounds__(kB) void k_apply_dots(int n, const T* __restrict__ r,
+                                                   const T* __restrict__ z,
+                                                   const T* __restrict__ qv, RedArgs ra) {
+  __shared__ double sh[2 * kW];
+  double d[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    const double zi = (double)z[i];
+    d[0] += (double)r[i] * zi;
+    if (DOT == 2) d[1] += zi * (double)qv[i];
+  }
+  block_sums<DOT>(d, sh);
+  store_partials<DOT>(d, blockIdx.x, ra);
 }
 
 // y = omdinv o b (level-0 entry of the V-cycle)
@@ -331,12 +379,25 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
   yc[I] = (T)((double)omdc[I] * (double)bI);
 }
 
-// e_i = y_i + xc[agg_i]  (e may be y: an in-place, per-element update)
-template <typename T>
-__global__ void k_prolong(int n, const T* y, const T* __restrict__ xc,
-                          const int* __restrict__ agg, T* e) {
+// e_i = y_i + xc[agg_i]  (e may be y: an in-place, per-element update).
+// KC: the coarse level took the K-cycle correction; xc holds its second
+// step c2 and the correction (T)((a1 - a2 g/rho1) c1 + a2 c2) is formed
+// here from the step's scalars ks.
+template <typename T, bool KC>
+__global__ void k_prolong(int n, const T* y, const T* __restrict__ xc, const T* __restrict__ c1,
+                          const double* __restrict__ ks, const int* __restrict__ agg, T* e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) e[i] = (T)((double)y[i] + (double)xc[agg[i]]);
+  if (i >= n) return;
+  const int a = agg[i];
+  double xcv;
+  if (KC) {
+    const KCoef k = kcyc_coef(ks);
+    const double al1 = k.a1 - k.a2 * k.gr;
+    xcv = (double)(T)(al1 * (double)c1[a] + k.a2 * (double)xc[a]);
+  } else {
+    xcv = (double)xc[a];
+  }
+  e[i] = (T)((double)y[i] + xcv);
 }
 
 // coarsest: x_i = sum_j Ainv[i][j] b_j, Ainv stored column-major (FP64).
@@ -403,7 +464,7 @@ __global__ __launch_bounds__(kB) void k_kprod(int nrb, const int2* __restrict__ 
                                               const int* __restrict__ ci,
                                               const T* __restrict__ va, const T* __restrict__ c,
                                               const T* __restrict__ b, const T* __restrict__ v1,
-                                              T* __restrict__ w, double* __restrict__ partial) {
+                                              T* __restrict__ w, RedArgs ra) {
   __shared__ double prod[kW][kCap];
   __shared__ double sh[NC * kW];
   const int lb = xcd_block(gridDim.x);
@@ -441,26 +502,7 @@ __global__ __launch_bounds__(kB) void k_kprod(int nrb, const int2* __restrict__ 
     }
   }
   block_sums<NC>(d, sh);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < NC; ++k) partial[(size_t)lb * NC + k] = d[k];
-}
-
-// Coefficients of the two flexible-CG steps from ks = {c1.Ac1, c1.b,
-// c2.v1, c2.Ac2, c2.b}; a zero c1.Ac1 (beta <= 0) drops the terms it divides.
-struct KCoef {
-  double a1, gr, a2;
-};
-__device__ __forceinline__ double kcyc_a1(const double* ks) {
-  return ks[0] > 0.0 ? ks[1] / ks[0] : 0.0;
-}
-__device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
-  KCoef k;
-  k.a1 = kcyc_a1(ks);
-  k.gr = ks[0] > 0.0 ? ks[2] / ks[0] : 0.0;
-  const double beta = ks[3] - ks[2] * k.gr;
-  k.a2 = beta > 0.0 ? ks[4] / beta : 0.0;
-  return k;
+  store_partials<NC>(d, lb, ra);
 }
 
 // b -= a1 v1 (the second step's right-hand side, in place); y = omdinv o b
@@ -475,16 +517,6 @@ __global__ void k_kres(int n, const double* __restrict__ ks, const T* __restrict
   y[i] = (T)((double)omd[i] * (double)bi);
 }
 
-// x = (a1 - a2 g / rho1) c1 + a2 c2, c2 held in x
-template <typename T>
-__global__ void k_kcomb(int n, const double* __restrict__ ks, const T* __restrict__ c1,
-                        T* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const KCoef k = kcyc_coef(ks);
-  const double al1 = k.a1 - k.a2 * k.gr;
-  x[i] = (T)(al1 * (double)c1[i] + k.a2 * (double)x[i]);
-}
 
 // ------------------------------------------------------------ host -----
 std::vector<double> dense_inverse_host(int n, const std::vector<int>& ro,
@@ -767,20 +799,22 @@ struct LevelData<float> {
   static float* kv(AmgLevel& L) { return L.kv32.p; }
 };
 
-// x = V-cycle(b) on level l; y = omdinv o b (computed here when null).
+// x = cycle(b) on level l; y = omdinv o b (computed here when null).
 // sweeps > 1: x starts at y (the first sweep from zero) and takes sweeps-1
 // further Jacobi sweeps before the residual; after the coarse correction
-// every sweep is a Jacobi sweep.
+// every sweep is a Jacobi sweep.  dots (entry level only): the last up
+// kernel also finishes r.z (and z.q) of the result.
 template <typename T>
 static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps);
 
 template <typename T>
 static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStream_t s,
-                   int sweeps) {
+                   int sweeps, const ApplyDots* dots = nullptr) {
   using D = LevelData<T>;
   std::vector<AmgLevel>& lv = H.lv;
   AmgLevel& L = lv[l];
   const DevCSR<T>& A = D::A(L);
+  const T* qv = dots ? (const T*)dots->q : nullptr;
   if (L.isCoarsest) {
     if (L.coarseInvT.p) {
       // dense level only when n <= 4096: at most 64 chunks of either width
@@ -802,6 +836,15 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
                                                      D::omd(L), b, src, dst);
       }
     }
+    if (dots) {
+      const int nb = std::min(grid_for(L.n, kB), 1024);
+      if (qv)
+        k_apply_dots<T, 2><<<nb, kB, 0, s>>>(L.n, b, x, qv, dots->ra);
+      else
+        k_apply_dots<T, 1><<<nb, kB, 0, s>>>(L.n, b, x, qv, dots->ra);
+      AN_CHECK_LAUNCH();
+      finalize_parts(dots->ra, nb, qv ? 2 : 1, s);
+    }
     AN_CHECK_LAUNCH();
     return;
   }
@@ -813,49 +856,71 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   AmgLevel& C = lv[l + 1];
   T* r = D::r(L);
   const int G = grid_for(A.rb.count, kW);
-  auto sweep = [&](const T* from, T* to) {  // to = from + omdinv (b - A from)
-    k_level_rows<T, 1><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
-                                        A.vals.p, D::omd(L), b, from, to);
+  const RedArgs noRed{nullptr, nullptr};
+  auto sweep = [&](const T* from, T* to, bool last) {  // to = from + omdinv (b - A from)
+    const int2* rbs = A.rb.start.p;
+    const int nrb = A.rb.count;
+    if (last && dots && qv)
+      k_level_rows<T, 1, 2><<<G, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
+                                             D::omd(L), b, from, to, qv, dots->ra);
+    else if (last && dots)
+      k_level_rows<T, 1, 1><<<G, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
+                                             D::omd(L), b, from, to, qv, dots->ra);
+    else
+      k_level_rows<T, 1><<<G, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p, D::omd(L),
+                                          b, from, to, qv, noRed);
+    if (last && dots) {
+      AN_CHECK_LAUNCH();
+      finalize_parts(dots->ra, G, qv ? 2 : 1, s);
+    }
   };
   // pre-smoothing: x0 = y; extra sweeps ping-pong between x and r
   const T* xs = y;
   for (int k = 1; k < sweeps; ++k) {
     T* to = (xs == x) ? r : x;
-    sweep(xs, to);
+    sweep(xs, to, false);
     xs = to;
   }
   T* res = (xs == r) ? x : r;
   k_level_rows<T, 0><<<G, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      D::omd(L), b, xs, res);
+                                      D::omd(L), b, xs, res, qv, noRed);
   k_restrict<T><<<grid_for(L.nc, kB), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, res, D::omd(C),
                                                   D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
-  if (C.kLevel)
-    kcorrect<T>(H, l + 1, s, sweeps);
-  else
-    vcycle<T>(H, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
-  // e = x_pre + P xc, then the post-smoothing sweeps ping-pong between r
-  // and x; e starts in the buffer from which an odd/even number of sweeps
-  // ends in x (in place when that buffer already holds x_pre)
+  // e = x_pre + P xc (with a K-cycle level's combination formed here),
+  // then the post-smoothing sweeps ping-pong between r and x; e starts in
+  // the buffer from which an odd/even number of sweeps ends in x (in place
+  // when that buffer already holds x_pre)
   T* e = (sweeps % 2 == 1) ? r : x;
-  k_prolong<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, xs, D::x(C), L.agg.p, e);
+  if (C.kLevel) {
+    kcorrect<T>(H, l + 1, s, sweeps);
+    const double* ks = H.kscal.p + 8 * (size_t)(l + 1);
+    k_prolong<T, true><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, xs, D::x(C), D::kc(C), ks,
+                                                        L.agg.p, e);
+  } else {
+    vcycle<T>(H, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
+    k_prolong<T, false><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, xs, D::x(C), nullptr, nullptr,
+                                                         L.agg.p, e);
+  }
   const T* cur = e;
   for (int k = 0; k < sweeps; ++k) {
     T* to = (cur == r) ? x : r;
-    sweep(cur, to);
+    sweep(cur, to, k == sweeps - 1);
     cur = to;
   }
   AN_CHECK_LAUNCH();
 }
 
 // K-cycle correction on level c for the right-hand side in D::b (with y =
-// omdinv o b), result in D::x (Notay & Vassilevski 2008, "Recursive
-// Krylov-based multigrid cycles": two flexible-CG steps preconditioned by
-// the level's own cycle, always both steps):
+// omdinv o b) (Notay & Vassilevski 2008, "Recursive Krylov-based multigrid
+// cycles": two flexible-CG steps preconditioned by the level's own cycle,
+// always both steps):
 //   c1 = B b; v1 = A c1; rho1 = c1.v1; a1 = c1.b / rho1; b -= a1 v1
 //   c2 = B b; g = c2.v1; beta = c2.Ac2 - g^2/rho1; a2 = c2.b / beta
 //   x = (a1 - a2 g / rho1) c1 + a2 c2
-// The inner products are two-stage block sums like every other reduction.
+// c1 is left in D::kc, c2 in D::x and the scalars in kscal[8c..8c+5);
+// the parent level's prolongation forms x.  The inner products are
+// two-stage block sums like every other reduction.
 template <typename T>
 static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
   using D = LevelData<T>;
@@ -872,26 +937,24 @@ static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
   const T* none = nullptr;
   vcycle<T>(H, c, b, y, c1, s, sweeps);
   k_kprod<T, 2><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, c1,
-                                  b, none, v1, H.kred.partial.p);
+                                  b, none, v1, H.kred.args(ks));
   AN_CHECK_LAUNCH();
   finalize(H.kred, nb, 2, ks, s);
   k_kres<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, ks, v1, D::omd(L), b, y);
   AN_CHECK_LAUNCH();
   vcycle<T>(H, c, b, y, x, s, sweeps);
   k_kprod<T, 3><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, x, b,
-                                  v1, nullptr, H.kred.partial.p);
+                                  v1, nullptr, H.kred.args(ks + 2));
   AN_CHECK_LAUNCH();
   finalize(H.kred, nb, 3, ks + 2, s);
-  k_kcomb<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, ks, c1, x);
-  AN_CHECK_LAUNCH();
 }
 
-void AmgHierarchy::apply(const double* r, double* z, hipStream_t s) {
-  vcycle<double>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
+void AmgHierarchy::apply(const double* r, double* z, hipStream_t s, const ApplyDots* dots) {
+  vcycle<double>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1), dots);
 }
 
-void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s) {
-  vcycle<float>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1));
+void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s, const ApplyDots* dots) {
+  vcycle<float>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1), dots);
 }
 
 }  // namespace an

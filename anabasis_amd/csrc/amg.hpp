@@ -33,6 +33,13 @@ struct AmgLevel {
   bool isCoarsest = false;
 };
 
+// Inner products finished by a preconditioner application z = M^-1 r:
+// out[0] = r.z and, when q is given (same precision as r), out[1] = z.q.
+struct ApplyDots {
+  const void* q;
+  RedArgs ra;
+};
+
 class AmgHierarchy {
  public:
   AmgOptions opt;
@@ -46,9 +53,9 @@ class AmgHierarchy {
   // coarse levels, as a HYPRE hierarchy reused between setups).
   void refresh_fine(hipStream_t s);
   void make_fp32(hipStream_t s);
-  // z = M^-1 r, one V-cycle from zero.
-  void apply(const double* r, double* z, hipStream_t s);
-  void apply32(const float* r, float* z, hipStream_t s);
+  // z = M^-1 r, one cycle from zero (V, or K on the kcycle levels)
+  void apply(const double* r, double* z, hipStream_t s, const ApplyDots* dots = nullptr);
+  void apply32(const float* r, float* z, hipStream_t s, const ApplyDots* dots = nullptr);
   int levels() const { return (int)lv.size(); }
   // some level takes the K-cycle correction: the preconditioner is not a
   // fixed linear operator, so the outer CG uses the flexible beta

@@ -86,12 +86,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // Two-stage deterministic reduction: every block stores its partial, then
-// k_finalize (one block) adds the partials in index order.  No global
-// atomics: a single device-wide ticket serialises across the XCDs.
-__device__ __forceinline__ void store_partial(double blockVal, int blockId,
-                                              double* __restrict__ partial) {
-  if (threadIdx.x == 0) partial[blockId] = blockVal;
-}
+// k_finalize (one block) adds the partials in index order.
 
 __global__ __launch_bounds__(1024) void k_finalize(int nparts, int comps,
                                                    const double* __restrict__ partial,
@@ -111,6 +106,11 @@ void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_
   AN_CHECK_LAUNCH();
 }
 
+void finalize_parts(const RedArgs& ra, int nparts, int comps, hipStream_t s) {
+  k_finalize<<<1, 1024, 0, s>>>(nparts, comps, ra.partial, ra.out);
+  AN_CHECK_LAUNCH();
+}
+
 void Reducer::init(int maxBlocks, hipStream_t s) {
   cap = maxBlocks;
   partial.alloc(4 * (size_t)maxBlocks);
@@ -127,8 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
                                                  const VT* __restrict__ vals,
                                                  const XT* __restrict__ x,
                                                  const double* __restrict__ b, YT* __restrict__ y,
-                                                 const YT* __restrict__ dw,
-                                                 double* __restrict__ partial) {
+                                                 const YT* __restrict__ dw, RedArgs ra) {
   __shared__ double prod[kWaves][kCap];
   __shared__ double sh[kWaves];
   const int lb = xcd_block(gridDim.x);
@@ -160,8 +159,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
     if (DOT == 2) dv = (double)v * (double)v;
   }
   if (DOT) {
-    double bs = block_sum(dv, sh);
-    store_partial(bs, lb, partial);
+    const double bs = block_sum(dv, sh);
+    store_partials<1>(&bs, lb, ra);
   }
 }
 
@@ -182,24 +181,24 @@ void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* sc
   const double* nob = nullptr;
   if (dotWith) {
     if (nb > red.cap) throw Error(7, "reducer too small");
-    double* part = red.partial.p;
+    const RedArgs ra = red.args(scal + slot);
     if (evStart)
       hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 1>, dim3(nb), dim3(kBlock), 0, s, evStart,
-                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, dotWith, part);
+                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, dotWith, ra);
     else
       k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y,
-                                                     dotWith, part);
+                                                     dotWith, ra);
     AN_CHECK_LAUNCH();
-    finalize(red, nb, 1, scal + slot, s);
+    finalize_parts(ra, nb, 1, s);
   } else {
     const XT* nodw = nullptr;
-    double* nopart = nullptr;
+    const RedArgs none{nullptr, nullptr};
     if (evStart)
       hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 0>, dim3(nb), dim3(kBlock), 0, s, evStart,
-                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, nodw, nopart);
+                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, nodw, none);
     else
       k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y, nodw,
-                                                     nopart);
+                                                     none);
   }
   AN_CHECK_LAUNCH();
 }
@@ -211,7 +210,7 @@ void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, doub
   if (nb > red.cap) throw Error(7, "reducer too small");
   k_spmv<VT, double, RT, 1, 2><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
                                                      A.cols.p, A.vals.p, x, b, r, nullptr,
-                                                     red.partial.p);
+                                                     red.args(scal + S_RR));
   AN_CHECK_LAUNCH();
   finalize(red, nb, 1, scal + S_RR, s);
 }
@@ -224,20 +223,19 @@ static inline int red_blocks(int n) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(int n, const T* __restrict__ a,
-                                                const T* __restrict__ b,
-                                                double* __restrict__ partial) {
+                                                const T* __restrict__ b, RedArgs ra) {
   __shared__ double sh[kWaves];
   double v = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     v += (double)a[i] * (double)b[i];
-  double bs = block_sum(v, sh);
-  store_partial(bs, blockIdx.x, partial);
+  const double bs = block_sum(v, sh);
+  store_partials<1>(&bs, blockIdx.x, ra);
 }
 
 template <typename T>
 void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hipStream_t s) {
   const int nb = red_blocks(n);
-  k_dot<T><<<nb, kBlock, 0, s>>>(n, a, b, red.partial.p);
+  k_dot<T><<<nb, kBlock, 0, s>>>(n, a, b, red.args(scal + slot));
   AN_CHECK_LAUNCH();
   finalize(red, nb, 1, scal + slot, s);
 }
@@ -245,8 +243,7 @@ void dot(int n, const T* a, const T* b, double* scal, int slot, Reducer& red, hi
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot2(int n, const T* __restrict__ a,
                                                  const T* __restrict__ b,
-                                                 const T* __restrict__ c,
-                                                 double* __restrict__ partial) {
+                                                 const T* __restrict__ c, RedArgs ra) {
   __shared__ double sh[2 * kWaves];
   double v[2] = {0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -255,17 +252,14 @@ __global__ __launch_bounds__(kBlock) void k_dot2(int n, const T* __restrict__ a,
     v[1] += bi * (double)c[i];
   }
   block_sums<2>(v, sh);
-  if (threadIdx.x == 0) {
-    partial[2 * (size_t)blockIdx.x] = v[0];
-    partial[2 * (size_t)blockIdx.x + 1] = v[1];
-  }
+  store_partials<2>(v, blockIdx.x, ra);
 }
 
 template <typename T>
 void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Reducer& red,
           hipStream_t s) {
   const int nb = red_blocks(n);
-  k_dot2<T><<<nb, kBlock, 0, s>>>(n, a, b, c, red.partial.p);
+  k_dot2<T><<<nb, kBlock, 0, s>>>(n, a, b, c, red.args(scal + slot));
   AN_CHECK_LAUNCH();
   finalize(red, nb, 2, scal + slot, s);
 }
@@ -275,8 +269,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
                                                        T* __restrict__ r,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
-                                                       const double* __restrict__ scal, int num,
-                                                       int den, double* __restrict__ partial) {
+                                                       const double* scal, int num, int den,
+                                                       RedArgs ra) {
   __shared__ double sh[kWaves];
   const double alpha = scal[num] / scal[den];
   double v = 0.0;
@@ -286,15 +280,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
     r[i] = ri;
     v += (double)ri * (double)ri;
   }
-  double bs = block_sum(v, sh);
-  store_partial(bs, blockIdx.x, partial);
+  const double bs = block_sum(v, sh);
+  store_partials<1>(&bs, blockIdx.x, ra);
 }
 
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s) {
   const int nb = red_blocks(n);
-  k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.partial.p);
+  k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR));
   AN_CHECK_LAUNCH();
   finalize(red, nb, 1, scal + S_RR, s);
 }

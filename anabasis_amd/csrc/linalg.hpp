@@ -114,13 +114,33 @@ enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 
 
 // Deterministic reduction workspace: per-block partials, summed in index
 // order by a one-block finalize kernel.
+// A reducing kernel's view of it: where its block partials go and where
+// the finalize kernel that follows it puts the sums.
+struct RedArgs {
+  double* partial;
+  double* out;
+};
 struct Reducer {
   DevBuf<double> partial;  // 4 doubles per block
   int cap = 0;
   void init(int maxBlocks, hipStream_t s);
+  RedArgs args(double* out) const { return RedArgs{partial.p, out}; }
 };
+
+// Block partials of NC sums (bs, valid in thread 0) -> partial[blockId*NC
+// + c], for finalize_parts() to add in index order.  (A last-block finish
+// inside the reducing kernel would need a device-scope fence and one
+// contended ticket per block: far slower here than a separate launch.)
+template <int NC>
+__device__ __forceinline__ void store_partials(const double* bs, int blockId, const RedArgs& ra) {
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ra.partial[(size_t)blockId * NC + c] = bs[c];
+}
+
 // out[c] = sum_i partial[i*comps + c], i < nparts (one block, fixed order)
 void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s);
+void finalize_parts(const RedArgs& ra, int nparts, int comps, hipStream_t s);
 
 // y = A x.  When dotWith != nullptr also computes dot(dotWith, y) into
 // scal[slot] (deterministically).  evStart/evStop, when given, record the

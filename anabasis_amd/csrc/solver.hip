@@ -33,6 +33,7 @@ void default_pcg_options(an_pcg_options* o) {
   o->profile = 0;
   o->sweeps = 1;
   o->kcycle = 2;
+  o->graphs = 0;  // replay measured no faster than launching here
 }
 
 static constexpr int kB = 256;
@@ -68,6 +69,8 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
 }
 
 PcgSolver::~PcgSolver() {
+  drop_graphs();
+  if (cs) (void)hipStreamDestroy(cs);
   if (ev) (void)hipEventDestroy(ev);
   for (auto& e : spmvEv) {
     (void)hipEventDestroy(e.a);
@@ -116,6 +119,7 @@ void PcgSolver::set_values_device(const double* vals, bool rebuild) {
   valuesSet = true;
   if (rebuild || !amg.built) {
     auto t0 = std::chrono::steady_clock::now();
+    drop_graphs();  // the captured cycles point into the old hierarchy
     amg.setup(A, s);
     if (opt.precision == 1) {
       amg.make_fp32(s);
@@ -130,6 +134,47 @@ void PcgSolver::set_values_device(const double* vals, bool rebuild) {
   } else {
     amg.refresh_fine(s);
   }
+}
+
+void PcgSolver::drop_graphs() {
+  for (auto& g : applyGraphs) (void)hipGraphExecDestroy(g.second);
+  applyGraphs.clear();
+}
+
+void PcgSolver::precondition(bool fp32, const ApplyDots& d) {
+  auto launch = [&](hipStream_t st) {
+    if (fp32)
+      amg.apply32(r32.p, z32.p, st, &d);
+    else
+      amg.apply(r.p, z.p, st, &d);
+  };
+  if (!opt.graphs) {
+    launch(s);
+    return;
+  }
+  const long long key = ((long long)(d.ra.out - scal.p) << 2) | (d.q ? 2 : 0) | (fp32 ? 1 : 0);
+  auto it = applyGraphs.find(key);
+  if (it == applyGraphs.end()) {
+    // captured on a private non-blocking stream (the solver's own stream may
+    // be the null stream, which cannot capture), replayed on the solver's
+    if (!cs) AN_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    AN_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+      launch(cs);
+    } catch (...) {
+      (void)hipStreamEndCapture(cs, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    AN_HIP(hipStreamEndCapture(cs, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    AN_HIP(e);
+    it = applyGraphs.emplace(key, exec).first;
+  }
+  AN_HIP(hipGraphLaunch(it->second, s));
 }
 
 double PcgSolver::read_scalar(int slot) {
@@ -157,8 +202,8 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
   // S_FRZ, beta = -(z.q)/(p.q) (Polak-Ribiere: z.(r_new - r_old) / r_old.z_old
   // with r_new - r_old = -alpha q); otherwise beta = rho_new / rho_old
   const bool flex = amg.flexible();
-  amg.apply(r.p, z.p, s);
-  dot<double>(n, r.p, z.p, scal.p, flex ? S_FRZ : S_RHO0, red, s);
+  const ApplyDots d0{nullptr, red.args(scal.p + (flex ? S_FRZ : S_RHO0))};
+  precondition(false, d0);  // r.z from the cycle's last kernel
   pcg_direction<double>(n, p.p, z.p, scal.p, 0, 0, true, s);
   int it = 1;
   double rnPrev = 0.0;
@@ -174,15 +219,14 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
-      amg.apply(r.p, z.p, s);
+      const ApplyDots dd = flex ? ApplyDots{q.p, red.args(scal.p + S_FRZ)}
+                                : ApplyDots{nullptr, red.args(scal.p + rhoNew)};
+      precondition(false, dd);
       if (e2) AN_HIP(hipEventRecord(e2->b, s));
-      if (flex) {
-        dot2<double>(n, r.p, z.p, q.p, scal.p, S_FRZ, red, s);
+      if (flex)
         pcg_direction<double>(n, p.p, z.p, scal.p, S_FZQ, S_PQ, false, s, -1.0);
-      } else {
-        dot<double>(n, r.p, z.p, scal.p, rhoNew, red, s);
+      else
         pcg_direction<double>(n, p.p, z.p, scal.p, rhoNew, rhoOld, false, s);
-      }
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();  // queued while the host waits for ||r||
@@ -206,8 +250,8 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
   fill_zero(x, sizeof(float) * (size_t)n, s);
   AN_HIP(hipMemcpyAsync(r32.p, b, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
   const bool flex = amg.flexible();
-  amg.apply32(r32.p, z32.p, s);
-  dot<float>(n, r32.p, z32.p, scal.p, flex ? S_FRZ : S_RHO0, red, s);
+  const ApplyDots d0{nullptr, red.args(scal.p + (flex ? S_FRZ : S_RHO0))};
+  precondition(true, d0);
   pcg_direction<float>(n, p32.p, z32.p, scal.p, 0, 0, true, s);
   int it = 1;
   double rn = rn0, rnPrev = 0.0;
@@ -223,15 +267,14 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
-      amg.apply32(r32.p, z32.p, s);
+      const ApplyDots dd = flex ? ApplyDots{q32.p, red.args(scal.p + S_FRZ)}
+                                : ApplyDots{nullptr, red.args(scal.p + rhoNew)};
+      precondition(true, dd);
       if (e2) AN_HIP(hipEventRecord(e2->b, s));
-      if (flex) {
-        dot2<float>(n, r32.p, z32.p, q32.p, scal.p, S_FRZ, red, s);
+      if (flex)
         pcg_direction<float>(n, p32.p, z32.p, scal.p, S_FZQ, S_PQ, false, s, -1.0);
-      } else {
-        dot<float>(n, r32.p, z32.p, scal.p, rhoNew, red, s);
+      else
         pcg_direction<float>(n, p32.p, z32.p, scal.p, rhoNew, rhoOld, false, s);
-      }
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();

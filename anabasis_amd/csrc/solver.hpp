@@ -1,6 +1,8 @@
 // solver.hpp -- PCG + AMG (FP64 and FP32-inner/FP64-refined) and BiCGSTAB.
 #pragma once
 
+#include <map>
+
 #include "../../include/anabasis_hip.h"
 #include "amg.hpp"
 
@@ -47,6 +49,13 @@ class PcgSolver {
   void apply_prec(const double* rr, double* zz) { amg.apply(rr, zz, s); }
 
  private:
+  // z = M^-1 r on the solver's own r/z (FP64) or r32/z32 (FP32) vectors,
+  // with r.z (and z.q) into d; replayed from a graph captured on first use
+  // per (precision, target, q) and dropped when the hierarchy is rebuilt
+  void precondition(bool fp32, const ApplyDots& d);
+  void drop_graphs();
+  std::map<long long, hipGraphExec_t> applyGraphs;
+  hipStream_t cs = nullptr;  // capture stream
   int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);

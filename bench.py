@@ -220,10 +220,12 @@ def run_adjoint(n, ncorr):
     """Scalar transport on an n^3 jittered hex box: k = exp(N(0, 0.5)) per
     cell, frozen u = (sin y, sin z, sin x), xmin Dirichlet inlet profile
     1 + 0.5 sin(pi y) sin(pi z), Dirichlet 0 on the side walls, xmax
-    zero-gradient outlet; J = area integral of phi over the outlet.  Times the forward solve and the adjoint (gradient with
-    respect to every k and every boundary value) and checks it along a random
+    zero-gradient outlet; J = area integral of phi over the outlet.  Times the
+    forward solve and the adjoint (gradient with respect to every k and every
+    boundary value) with ncorr non-orthogonal corrections, and checks the
+    gradient of the converged problem (40 corrections) along a positive random
     direction in k and one in the inlet values against central differences
-    (eps = 1e-6)."""
+    (eps = 1e-4)."""
     from anabasis_amd import adjoint as ad
     from anabasis_amd import libpoisson as lp
     t0 = time.time()
@@ -248,16 +250,21 @@ def run_adjoint(n, ncorr):
     t0 = time.time()
     dk, db, _ = P.adjoint(k, vals, phi, dJ(phi))
     t_adj = time.time() - t0
-    # directional central differences (a single cell's or face's share of J
-    # is below the forward solve's accuracy at this size): k along
-    # k (1 + eps xi), the inlet values along eps eta, eps = 1e-6
-    eps = 1e-6
-    xi = rng.standard_normal(m.nCells)
-    dkv = eps * k * xi
-    fd_k = (Jf(P.solve(k + dkv, vals)) - Jf(P.solve(k - dkv, vals))) / 2.0
+    # directional central differences of the converged problem (a single
+    # cell's or face's share of J is below the forward solve's accuracy at
+    # this size, and the ncorr-pass fixed point is not converged to the FD
+    # step's level): k along k (1 + eps xi), the inlet values along eps eta,
+    # xi, eta ~ U(0.5, 1.5)
+    del P
+    Pc = ad.TransportProblem(m, flux, types, nNonOrthCorr=40, absTol=1e-13)
+    phic = Pc.solve(k, vals)
+    dk, db, _ = Pc.adjoint(k, vals, phic, dJ(phic))
+    eps = 1e-4
+    dkv = eps * k * rng.uniform(0.5, 1.5, m.nCells)
+    fd_k = (Jf(Pc.solve(k + dkv, vals)) - Jf(Pc.solve(k - dkv, vals))) / 2.0
     ad_k = float(dk @ dkv)
-    eta = np.where(inlet, rng.standard_normal(m.nBoundaryFaces), 0.0)
-    fd_b = (Jf(P.solve(k, vals + eps * eta)) - Jf(P.solve(k, vals - eps * eta))) / 2.0
+    eta = np.where(inlet, rng.uniform(0.5, 1.5, m.nBoundaryFaces), 0.0)
+    fd_b = (Jf(Pc.solve(k, vals + eps * eta)) - Jf(Pc.solve(k, vals - eps * eta))) / 2.0
     ad_b = float(db @ (eps * eta))
     rel_k = abs(fd_k - ad_k) / abs(fd_k)
     rel_b = abs(fd_b - ad_b) / abs(fd_b)
@@ -266,6 +273,7 @@ def run_adjoint(n, ncorr):
     return dict(cells=m.nCells, nNonOrthCorr=ncorr, forward_s=t_fwd, adjoint_s=t_adj,
                 J=Jf(phi), fd_dir_k=fd_k, adjoint_dir_k=ad_k, fd_rel_err_k=rel_k,
                 fd_dir_bc=fd_b, adjoint_dir_bc=ad_b, fd_rel_err_bc=rel_b,
+                fd_check_nNonOrthCorr=40, fd_eps=eps,
                 note="host-array API (includes PCIe copies); gradient w.r.t. all "
                      f"{m.nCells} k and {m.nBoundaryFaces} boundary values in one adjoint")
 

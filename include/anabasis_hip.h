@@ -165,6 +165,10 @@ typedef struct {
                           preconditioned by the next cycle (K-cycle); the
                           outer CG then uses the flexible (Polak-Ribiere)
                           beta.  0: V-cycle.  Ignored by the batched solver */
+  int graphs;          /* 1: each preconditioner application is replayed
+                          from a captured HIP graph (captured once per
+                          hierarchy build) -- for hosts too slow to keep the
+                          GPU fed; 0 (default): launched kernel by kernel */
 } an_pcg_options;
 void an_pcg_default_options(an_pcg_options *o);
 

@@ -215,6 +215,28 @@ def test_pcg_kcycle_flexible_converges(meshes, omeshes):
             assert np.array_equal(x, x2)
 
 
+def test_pcg_graph_replay_matches_eager(meshes, omeshes):
+    """The preconditioner replayed from captured HIP graphs gives the eager
+    launch sequence's results bit for bit, over several solves (host work in
+    between) and after a hierarchy rebuild (graphs re-captured)."""
+    m, om = meshes["cyl"], omeshes["cyl"]
+    vals = laplacian(om, m, ("patch_2_0",))
+    rng = np.random.default_rng(9)
+    for precision in (0, 1):
+        kw = dict(relTol=1e-10, absTol=0.0, coarseSize=64, precision=precision)
+        E = lp.PCGSolver(om.rowOffsets, om.cols, graphs=0, **kw)
+        G = lp.PCGSolver(om.rowOffsets, om.cols, graphs=1, **kw)
+        for scale in (1.0, 2.0):
+            E.set_values(scale * vals, rebuild=True)
+            G.set_values(scale * vals, rebuild=True)
+            for _ in range(3):
+                b = rng.standard_normal(om.nCells)
+                xe, ie = E.solve(b)
+                xg, ig = G.solve(b)
+                assert ie.converged and ie.iterations == ig.iterations
+                assert np.array_equal(xe, xg), (precision, scale)
+
+
 def test_pcg_amg_converges_to_direct_solution(meshes, omeshes):
     m, om = meshes["hex"], omeshes["hex"]
     vals = laplacian(om, m)
