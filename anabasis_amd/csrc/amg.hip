@@ -150,79 +150,112 @@ __global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __rest
 }
 
 // ---------------------------------------------------------- Galerkin ---
-__global__ void k_gal_count(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
-                            const int* __restrict__ ro, const int* __restrict__ ci,
-                            const int* __restrict__ agg, int* __restrict__ cnt,
-                            int* __restrict__ overflow) {
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  int kJ[kGalCap];
-  int c = 0;
-  for (int m = moff[I]; m < moff[I + 1]; ++m) {
-    int i = mem[m];
-    for (int k = ro[i]; k < ro[i + 1]; ++k) {
-      int J = agg[ci[k]];
-      bool found = false;
-      for (int t = 0; t < c; ++t)
-        if (kJ[t] == J) {
-          found = true;
-          break;
-        }
-      if (!found) {
-        if (c == kGalCap) {
-          atomicOr(overflow, 1);
-          cnt[I] = c;
-          return;
-        }
-        kJ[c++] = J;
+// Aggregate sum C = P^T A P, one 64-lane wave per coarse row I.  The fine
+// entries of I's members are visited in member order and CSR order within
+// a row, 64 at a time; the distinct coarse columns are kept in an LDS list
+// in first-seen order, and a column's value is its first entry plus the
+// later ones in visiting order (the serial definition's order, so the sums
+// are bit-identical).  FILL 0 counts the distinct columns (more than
+// kGalCap: overflow); FILL 1 writes them sorted by column.
+static constexpr int kGalWaves = 4;
+template <bool FILL>
+__global__ __launch_bounds__(64 * kGalWaves) void k_gal_wave(
+    int nc, const int* __restrict__ moff, const int* __restrict__ mem, const int* __restrict__ ro,
+    const int* __restrict__ ci, const double* __restrict__ va, const int* __restrict__ agg,
+    int* __restrict__ cnt, int* __restrict__ overflow, const int* __restrict__ cro,
+    int* __restrict__ cci, double* __restrict__ cva) {
+  __shared__ int sJ[kGalWaves][kGalCap];
+  __shared__ double sV[kGalWaves][kGalCap];
+  __shared__ int eJ[kGalWaves][64], eT[kGalWaves][64];
+  __shared__ double eV[kGalWaves][64];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int I = blockIdx.x * kGalWaves + w;
+  if (I >= nc) return;  // the whole wave: no block barriers below
+  int* lJ = sJ[w];
+  double* lV = sV[w];
+  int nd = 0;
+  bool over = false;
+  const int m1 = moff[I + 1];
+  for (int m = moff[I]; m < m1 && !over; ++m) {
+    const int i = mem[m];
+    const int ke = ro[i + 1];
+    for (int k0 = ro[i]; k0 < ke; k0 += 64) {
+      const int k = k0 + lane;
+      const bool has = k < ke;
+      int J = -1;
+      double v = 0.0;
+      if (has) {
+        J = agg[ci[k]];
+        if (FILL) v = va[k];
       }
+      eJ[w][lane] = J;
+      if (FILL) eV[w][lane] = v;
+      wave_sync();
+      int t = -1;
+      if (has)
+        for (int q = 0; q < nd; ++q)
+          if (lJ[q] == J) {
+            t = q;
+            break;
+          }
+      bool first = has && t < 0;
+      if (first)
+        for (int l = 0; l < lane; ++l)
+          if (eJ[w][l] == J) {
+            first = false;
+            break;
+          }
+      const unsigned long long newMask = __ballot(first);
+      const int nnew = __popcll(newMask);
+      if (nd + nnew > kGalCap) {
+        over = true;
+        break;
+      }
+      if (first) {
+        t = nd + __popcll(newMask & ((1ull << lane) - 1ull));
+        lJ[t] = J;
+      }
+      wave_sync();
+      if (has && t < 0)  // a later repeat of a column new in this chunk
+        for (int q = nd; q < nd + nnew; ++q)
+          if (lJ[q] == J) {
+            t = q;
+            break;
+          }
+      const int ndOld = nd;
+      nd += nnew;
+      if (FILL) {
+        eT[w][lane] = t;
+        wave_sync();
+        // the lowest lane of each target adds the chunk's entries of that
+        // target in lane order onto the running value
+        bool lead = has;
+        for (int l = 0; l < lane && lead; ++l)
+          if (eT[w][l] == t) lead = false;
+        if (lead) {
+          double acc = (t >= ndOld) ? v : lV[t] + v;
+          for (int l = lane + 1; l < 64; ++l)
+            if (eT[w][l] == t) acc += eV[w][l];
+          lV[t] = acc;
+        }
+      }
+      wave_sync();
     }
   }
-  cnt[I] = c;
-}
-
-__global__ void k_gal_fill(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
-                           const int* __restrict__ ro, const int* __restrict__ ci,
-                           const double* __restrict__ va, const int* __restrict__ agg,
-                           const int* __restrict__ cro, int* __restrict__ cci,
-                           double* __restrict__ cva) {
-  int I = blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  int kJ[kGalCap];
-  double kV[kGalCap];
-  int c = 0;
-  for (int m = moff[I]; m < moff[I + 1]; ++m) {
-    int i = mem[m];
-    for (int k = ro[i]; k < ro[i + 1]; ++k) {
-      int J = agg[ci[k]];
-      int t = 0;
-      for (; t < c; ++t)
-        if (kJ[t] == J) break;
-      if (t == c) {
-        kJ[c] = J;
-        kV[c] = va[k];
-        ++c;
-      } else {
-        kV[t] += va[k];
-      }
+  if (!FILL) {
+    if (lane == 0) {
+      cnt[I] = nd;
+      if (over) atomicOr(overflow, 1);
     }
-  }
-  for (int a = 1; a < c; ++a) {
-    int j = kJ[a];
-    double v = kV[a];
-    int b = a - 1;
-    while (b >= 0 && kJ[b] > j) {
-      kJ[b + 1] = kJ[b];
-      kV[b + 1] = kV[b];
-      --b;
-    }
-    kJ[b + 1] = j;
-    kV[b + 1] = v;
+    return;
   }
   const int o = cro[I];
-  for (int t = 0; t < c; ++t) {
-    cci[o + t] = kJ[t];
-    cva[o + t] = kV[t];
+  for (int q = lane; q < nd; q += 64) {
+    const int J = lJ[q];
+    int rank = 0;
+    for (int p = 0; p < nd; ++p) rank += lJ[p] < J ? 1 : 0;
+    cci[o + rank] = J;
+    cva[o + rank] = lV[q];
   }
 }
 
@@ -612,8 +645,10 @@ static bool galerkin(const DevCSR<double>& W, int nc, const int* agg, const int*
                      const int* mem, DevCSR<double>& C, hipStream_t s) {
   DevBuf<int> cnt((size_t)nc + 1), overflow(1);
   overflow.zero(s);
-  k_gal_count<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, agg, cnt.p,
-                                                overflow.p);
+  const int gw = (nc + kGalWaves - 1) / kGalWaves;
+  k_gal_wave<false><<<gw, 64 * kGalWaves, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, nullptr,
+                                                  agg, cnt.p, overflow.p, nullptr, nullptr,
+                                                  nullptr);
   AN_CHECK_LAUNCH();
   if (read_int(overflow.p, s)) return false;
   C.n = nc;
@@ -621,13 +656,15 @@ static bool galerkin(const DevCSR<double>& W, int nc, const int* agg, const int*
   C.nnz = exclusive_scan(cnt.p, C.rowOff.p, nc, s);
   C.cols.alloc((size_t)C.nnz);
   C.vals.alloc((size_t)C.nnz);
-  k_gal_fill<<<grid_for(nc, 128), 128, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, W.vals.p, agg,
-                                               C.rowOff.p, C.cols.p, C.vals.p);
+  k_gal_wave<true><<<gw, 64 * kGalWaves, 0, s>>>(nc, moff, mem, W.rowOff.p, W.cols.p, W.vals.p,
+                                                 agg, nullptr, nullptr, C.rowOff.p, C.cols.p,
+                                                 C.vals.p);
   AN_CHECK_LAUNCH();
   return true;
 }
 
 void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
+  PoolScope scope(&pool);
   lv.clear();
   has32 = false;
   lv.reserve(opt.maxLevels + 1);
@@ -744,6 +781,7 @@ void AmgHierarchy::refresh_fine(hipStream_t s) {
 }
 
 void AmgHierarchy::make_fp32(hipStream_t s) {
+  PoolScope scope(&pool);
   for (auto& L : lv) {
     const DevCSR<double>& A = *L.A;
     L.A32.n = A.n;

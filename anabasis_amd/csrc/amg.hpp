@@ -42,6 +42,10 @@ struct ApplyDots {
 
 class AmgHierarchy {
  public:
+  // block cache for every buffer the setup allocates (declared first: it
+  // outlives the levels whose buffers come from it); rebuilds then reuse
+  // blocks instead of paying hipMalloc/hipFree per temporary
+  DevPool pool;
   AmgOptions opt;
   std::vector<AmgLevel> lv;
   bool built = false;

@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,30 +49,105 @@ void set_last_error(const std::string& msg);
 
 #define AN_CHECK_LAUNCH() AN_HIP(hipGetLastError())
 
+// Cache of device blocks (sizes rounded up to powers of two) for code that
+// allocates and frees many buffers, all used on one stream: a freed block
+// goes back to the cache instead of hipFree (which waits for the device)
+// and is handed out again in stream order.  A DevBuf allocated while a
+// PoolScope is active on its thread takes its block from that pool and
+// returns it there; the pool must outlive such buffers.
+class DevPool {
+ public:
+  DevPool() = default;
+  DevPool(const DevPool&) = delete;
+  DevPool& operator=(const DevPool&) = delete;
+  ~DevPool() {
+    for (auto& f : free_) (void)hipFree(f.second);
+  }
+  static size_t bucket(size_t bytes) {
+    size_t b = 256;
+    while (b < bytes) b <<= 1;
+    return b;
+  }
+  void* get(size_t bytes) {
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    AN_HIP(hipMalloc(&p, bytes));
+    return p;
+  }
+  void put(void* p, size_t bytes) { free_.emplace(bytes, p); }
+
+ private:
+  std::multimap<size_t, void*> free_;
+};
+inline thread_local DevPool* tlPool = nullptr;
+struct PoolScope {
+  DevPool* prev;
+  explicit PoolScope(DevPool* p) : prev(tlPool) { tlPool = p; }
+  ~PoolScope() { tlPool = prev; }
+  PoolScope(const PoolScope&) = delete;
+  PoolScope& operator=(const PoolScope&) = delete;
+};
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevPool* pool = nullptr;  // owner of p when it came from a pool
+  size_t bytes = 0;         // pool block size
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), pool(o.pool), bytes(o.bytes) {
+    o.p = nullptr;
+    o.n = 0;
+    o.pool = nullptr;
+    o.bytes = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      pool = o.pool;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.n = 0;
+      o.pool = nullptr;
+      o.bytes = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     n = count;
-    if (count) AN_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    if (!count) return;
+    if (tlPool) {
+      pool = tlPool;
+      bytes = DevPool::bucket(count * sizeof(T));
+      p = (T*)pool->get(bytes);
+    } else {
+      AN_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    }
   }
   void ensure(size_t count) { if (count > n || !p) alloc(count); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool)
+        pool->put(p, bytes);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    pool = nullptr;
+    bytes = 0;
   }
   void upload(const T* h, size_t count, hipStream_t s = 0) {
     ensure(count);

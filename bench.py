@@ -55,11 +55,13 @@ def simple_case(cells):
 
 
 AMG_OVERRIDES = {}
+SIMPLE_OVERRIDES = {}
 
 
 def run_simple(m, vel, prs, precision, warmup, steps):
     from anabasis_amd import simple as sm
-    par = sm.default_params(precision=precision, pressure=dict(AMG_OVERRIDES))
+    par = sm.default_params(precision=precision, pressure=dict(AMG_OVERRIDES),
+                            **SIMPLE_OVERRIDES)
     S = sm.Simple(m, par, vel, prs)
     t0 = time.time()
     st, _ = S.iterate(warmup)
@@ -385,6 +387,8 @@ def main():
     ap.add_argument("--amg", type=str, default="",
                     help="AMG/PCG option overrides, e.g. 'sweeps=2,pairPasses=2,innerRelTol=1e-5' "
                          "(applied to the SIMPLE pressure solver and the pressure-solve section)")
+    ap.add_argument("--simple-params", type=str, default="",
+                    help="SIMPLE parameter overrides k=v,... (e.g. pAmgRebuildEvery=1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=1)
     ap.add_argument("--only-psolve", action="store_true",
@@ -399,9 +403,10 @@ def main():
         sys.exit(2)
     from anabasis_amd import _lib as L
     L.check(L.lib().an_set_device(0))
-    for kv in filter(None, args.amg.split(",")):
-        k, v = kv.split("=")
-        AMG_OVERRIDES[k.strip()] = float(v) if any(c in v for c in ".e") else int(v)
+    for arg, dst in ((args.amg, AMG_OVERRIDES), (args.simple_params, SIMPLE_OVERRIDES)):
+        for kv in filter(None, arg.split(",")):
+            k, v = kv.split("=")
+            dst[k.strip()] = float(v) if any(c in v for c in ".e") else int(v)
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb << 20)
     if args.only_psolve:
@@ -460,6 +465,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated mesh, cases/cylinder.case controls and BCs)",
         "amg_overrides": AMG_OVERRIDES,
+        "simple_overrides": SIMPLE_OVERRIDES,
         "config": {"workload": f"generic_simple_v1 cylinder channel, {m.nCells} hex cells, "
                                "nVelNonOrthCorr 1, nNonOrthCorr 10, AMG rebuild every 10",
                    "cells": m.nCells, "faces": m.nFaces},
