@@ -233,8 +233,9 @@ __global__ __launch_bounds__(64 * kGalWaves) void k_gal_wave(
         for (int l = 0; l < lane && lead; ++l)
           if (eT[w][l] == t) lead = false;
         if (lead) {
+          const int nE = min(64, ke - k0);  // entries in this chunk
           double acc = (t >= ndOld) ? v : lV[t] + v;
-          for (int l = lane + 1; l < 64; ++l)
+          for (int l = lane + 1; l < nE; ++l)
             if (eT[w][l] == t) acc += eV[w][l];
           lV[t] = acc;
         }
