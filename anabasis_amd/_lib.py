@@ -44,7 +44,7 @@ class PcgOptions(C.Structure):
                 ("precision", C.c_int), ("coarseSize", C.c_int), ("maxLevels", C.c_int),
                 ("strength", C.c_double), ("pairPasses", C.c_int), ("jacobiOmega", C.c_double),
                 ("innerRelTol", C.c_double), ("profile", C.c_int), ("sweeps", C.c_int),
-                ("kcycle", C.c_int), ("graphs", C.c_int)]
+                ("kcycle", C.c_int), ("graphs", C.c_int), ("rhsProjection", C.c_int)]
 
 
 class SolveInfo(C.Structure):

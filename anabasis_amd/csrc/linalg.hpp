@@ -110,7 +110,9 @@ __device__ __forceinline__ void block_sums(double* v, double* sh) {
 enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 5, S_TMP2 = 6,
                   S_AL = 7, S_OM = 8, S_R0V = 9, S_TS = 10, S_TT = 11, S_SS = 12,
                   S_FRZ = 13, S_FZQ = 14,  // flexible CG: r.z, z.q
-                  S_COUNT = 16 };
+                  S_PROJ = 16,             // 16 projection coefficients
+                  S_NU = 32,               // ||A w||^2 of a new direction, then ||A x||^2
+                  S_COUNT = 40 };
 
 // Deterministic reduction workspace: per-block partials, summed in index
 // order by a one-block finalize kernel.

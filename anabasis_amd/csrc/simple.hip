@@ -124,6 +124,172 @@ __global__ __launch_bounds__(kB) void k_mom(MeshView m, MomArgs a, double* __res
   rhs[c] = r;
 }
 
+// Momentum assembly for the three components at once, in two passes.
+// k_mom_faces: one thread per internal face computes what both of its cells
+// need, reading the face once: the non-orthogonal correction of every
+// active component, corr[c][f] = corrPsi (grad_f q_c . tc), and (FULL) the
+// matrix contributions dP/oP (owner side) and dN/oN (neighbour side) --
+// the very expressions of k_mom, so the values carry the same bits.
+// k_mom_cells: one thread per cell adds them in its incidence (face) order
+// and does the boundary faces and relaxation as k_mom, giving every
+// component's rhs and (FULL) the matrix and rAU bit for bit as three k_mom
+// launches would (the x component takes the FULL path, y and z read the
+// relaxed diagonal it produced, as their rhs-only launches do).
+struct Mom3Args {
+  const int* bcT;
+  const double* qB[3];
+  const double *uB, *vB, *wB;
+  const double* gq[3][3];  // [component][x, y, z] gradient of each iterate
+  const double *uC, *vC, *wC;
+  const double* gpc[3];
+  const double* qIt[3];    // iterates used by the relaxation
+  double* rhs[3];
+  double rho, mu, corrPsi, uRelax;
+  int active;              // bit c: component c is assembled
+};
+
+template <bool FULL>
+__global__ __launch_bounds__(kB) void k_mom_faces(MeshView m, Mom3Args a,
+                                                  double* __restrict__ corr,
+                                                  double* __restrict__ coef) {
+  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (f >= m.nIF) return;
+  const double nd = m.fDpn[f];
+  if (nd <= 1e-14) return;  // skipped by the cells as well
+  const double af = m.fA[f];
+  const double dx = m.fDx[f], dy = m.fDy[f], dz = m.fDz[f];
+  const double nx = m.fNx[f], ny = m.fNy[f], nz = m.fNz[f];
+  const double tcx = a.mu * (af * nx - (af / nd) * dx);
+  const double tcy = a.mu * (af * ny - (af / nd) * dy);
+  const double tcz = a.mu * (af * nz - (af / nd) * dz);
+  const double lam = m.fLam[f];
+  const int2 pn = m.fPN[f];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!((a.active >> c) & 1)) continue;
+    const double* gx = a.gq[c][0];
+    const double* gy = a.gq[c][1];
+    const double* gz = a.gq[c][2];
+    const double gfx = (1.0 - lam) * gx[pn.x] + lam * gx[pn.y];
+    const double gfy = (1.0 - lam) * gy[pn.x] + lam * gy[pn.y];
+    const double gfz = (1.0 - lam) * gz[pn.x] + lam * gz[pn.y];
+    corr[(size_t)c * m.nIF + f] = a.corrPsi * (gfx * tcx + gfy * tcy + gfz * tcz);
+  }
+  if (FULL) {
+    const double alpha = a.mu * af / nd;
+    const double ucf = (1.0 - lam) * a.uC[pn.x] + lam * a.uC[pn.y];
+    const double vcf = (1.0 - lam) * a.vC[pn.x] + lam * a.vC[pn.y];
+    const double wcf = (1.0 - lam) * a.wC[pn.x] + lam * a.wC[pn.y];
+    const double F = a.rho * af * (ucf * nx + vcf * ny + wcf * nz);
+    const size_t n = (size_t)m.nIF;
+    coef[f] = (alpha + F * (1.0 - lam));
+    coef[n + f] = (-alpha + F * lam);
+    coef[2 * n + f] = (alpha - F * lam);
+    coef[3 * n + f] = (-alpha - F * (1.0 - lam));
+  }
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
+                                                  const double* __restrict__ corr,
+                                                  const double* __restrict__ coef,
+                                                  double* __restrict__ vals,
+                                                  double* __restrict__ rAU) {
+  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (c >= m.nC) return;
+  const double V = m.cgeo[c].w;
+  const size_t n = (size_t)m.nIF;
+  double diag = 0.0;
+  double r[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) r[q] = ((a.active >> q) & 1) ? -a.gpc[q][c] * V : 0.0;
+  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
+    const int e = m.cfEnt[k];
+    const int f = e >> 1, side = e & 1;
+    const int slotInfo = m.cfSlot[k];
+    if (m.fDpn[f] <= 1e-14) {  // main.cu:1826 -- the face contributes nothing
+      if (FULL && (slotInfo & 1)) vals[slotInfo >> 1] = 0.0;
+      continue;
+    }
+    if (FULL) {
+      const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+      if (side == 0) {
+        diag += coef[f];
+        vals[slotInfo >> 1] = old + coef[n + f];
+      } else {
+        diag += coef[2 * n + f];
+        vals[slotInfo >> 1] = old + coef[3 * n + f];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (!((a.active >> q) & 1)) continue;
+      const double cr = corr[(size_t)q * n + f];
+      if (side == 0)
+        r[q] += cr;
+      else
+        r[q] += (-cr);
+    }
+  }
+  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
+    const int b = m.cbFace[j];
+    const double nd = m.bDpn[b];
+    if (nd <= 1e-14) continue;
+    const double af = m.bA[b];
+    const double dx = m.bDx[b], dy = m.bDy[b], dz = m.bDz[b];
+    const double nx = m.bNx[b], ny = m.bNy[b], nz = m.bNz[b];
+    const double alpha = a.mu * af / nd;
+    const double tcx = a.mu * (af * nx - (af / nd) * dx);
+    const double tcy = a.mu * (af * ny - (af / nd) * dy);
+    const double tcz = a.mu * (af * nz - (af / nd) * dz);
+    const bool dir = a.bcT[b] == 1;
+    if (dir) {
+      if (FULL) diag += alpha;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        if (!((a.active >> q) & 1)) continue;
+        const double g = a.qB[q][b];
+        const double cr =
+            a.corrPsi * (a.gq[q][0][c] * tcx + a.gq[q][1][c] * tcy + a.gq[q][2][c] * tcz);
+        r[q] += (alpha * g + cr);
+      }
+    }
+    const double ucb = dir ? a.uB[b] : a.uC[c];
+    const double vcb = dir ? a.vB[b] : a.vC[c];
+    const double wcb = dir ? a.wB[b] : a.wC[c];
+    const double F = a.rho * af * (ucb * nx + vcb * ny + wcb * nz);
+    if (dir) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if ((a.active >> q) & 1) r[q] += (-F * a.qB[q][b]);
+    } else if (FULL) {
+      diag += F;
+    }
+  }
+  const int dp = m.diagPos[c];
+  const bool relax = a.uRelax < 0.999999;
+  const double invRelax = relax ? 1.0 / a.uRelax : 1.0;
+  double aPold;
+  if (FULL) {
+    double aR = diag;
+    if (relax) {
+      aR = diag * invRelax;
+      if (a.active & 1) r[0] += (invRelax - 1.0) * diag * a.qIt[0][c];
+    }
+    vals[dp] = aR;
+    rAU[c] = (fabs(aR) > 1e-30) ? V / aR : 0.0;
+    aPold = aR * a.uRelax;
+  } else {
+    aPold = vals[dp] * a.uRelax;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (!((a.active >> q) & 1)) continue;
+    if (relax && !(FULL && q == 0)) r[q] += (invRelax - 1.0) * aPold * a.qIt[q][c];
+    a.rhs[q][c] = r[q];
+  }
+}
+
 // ----------------------------------------------------------- pressure ----
 __global__ __launch_bounds__(kB) void k_p_asm(MeshView m, const int* __restrict__ bcP,
                                               const double* __restrict__ rAU,
@@ -532,12 +698,15 @@ SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int*
     g[k].alloc(nC);
   }
   qNew.alloc(nC);
-  if (par.fuseMomentumRhs)
+  if (par.fuseMomentumRhs) {
     for (int c = 0; c < 3; ++c) {
       for (int d = 0; d < 3; ++d) g3[c][d].alloc(nC);
       rhs3[c].alloc(nC);
       qNew3[c].alloc(nC);
     }
+    momCorr.alloc(3 * (size_t)M.nIF);
+    momCoef.alloc(4 * (size_t)M.nIF);
+  }
   p.alloc(nC);
   p.zero(s);
   pOld.alloc(nC);
@@ -689,16 +858,41 @@ void SimpleSolver::momentum_fused(an_simple_stats* st) {
       BCView bc{bcU.p, bcUv[c].p};
       launch_lsq_gradient(mv, 0, bc, q[c].p, g3[c][0].p, g3[c][1].p, g3[c][2].p, s);
     }
-    for (int c = 0; c < 3; ++c) {  // x first: its first pass builds the shared matrix
-      if (!((active >> c) & 1)) continue;
-      const MomArgs a = mom_args(c, g3[c]);
-      if (c == 0 && it == 1)
-        k_mom<true><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs3[c].p, rAU.p);
-      else
-        k_mom<false><<<grid_for(nC, kB), kB, 0, s>>>(mv, a, Amom.vals.p, rhs3[c].p, rAU.p);
-      AN_CHECK_LAUNCH();
-      AN_HIP(hipMemcpyAsync(qNew3[c].p, q[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    // the three components' rhs (and in the first pass the shared matrix,
+    // built as the x component's full assembly) from one face pass and one
+    // cell pass
+    Mom3Args a3;
+    a3.bcT = bcU.p;
+    a3.uB = bcUv[0].p;
+    a3.vB = bcUv[1].p;
+    a3.wB = bcUv[2].p;
+    a3.uC = qOld[0].p;
+    a3.vC = qOld[1].p;
+    a3.wC = qOld[2].p;
+    for (int c = 0; c < 3; ++c) {
+      a3.qB[c] = bcUv[c].p;
+      for (int d = 0; d < 3; ++d) a3.gq[c][d] = g3[c][d].p;
+      a3.gpc[c] = gp[c].p;
+      a3.qIt[c] = q[c].p;
+      a3.rhs[c] = rhs3[c].p;
     }
+    a3.rho = par.rho;
+    a3.mu = par.mu;
+    a3.corrPsi = 1.0;
+    a3.uRelax = par.uRelax;
+    a3.active = active;
+    const int Gf = grid_for(M.nIF, kB), Gc = grid_for(nC, kB);
+    if (it == 1) {  // active == 7 here: the x component builds the matrix
+      k_mom_faces<true><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
+      k_mom_cells<true><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p);
+    } else {
+      k_mom_faces<false><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
+      k_mom_cells<false><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p);
+    }
+    AN_CHECK_LAUNCH();
+    for (int c = 0; c < 3; ++c)
+      if ((active >> c) & 1)
+        AN_HIP(hipMemcpyAsync(qNew3[c].p, q[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
     mark(1);
     const double* b3[3] = {rhs3[0].p, rhs3[1].p, rhs3[2].p};
     double* x3[3] = {qNew3[0].p, qNew3[1].p, qNew3[2].p};

@@ -57,6 +57,7 @@ class SimpleSolver {
   // right-hand sides and new iterates
   Bicg3Work bw3;
   DevBuf<double> g3[3][3], rhs3[3], qNew3[3];
+  DevBuf<double> momCorr, momCoef;  // per internal face: 3 corrections, 4 matrix terms
   Reducer red;
   DevBuf<double> scal;
   PinnedScalars hs;

@@ -34,6 +34,7 @@ void default_pcg_options(an_pcg_options* o) {
   o->sweeps = 1;
   o->kcycle = 2;
   o->graphs = 0;  // replay measured no faster than launching here
+  o->rhsProjection = 0;
 }
 
 static constexpr int kB = 256;
@@ -117,6 +118,7 @@ void PcgSolver::collect_profile(an_solve_info* inf) {
 void PcgSolver::set_values_device(const double* vals, bool rebuild) {
   AN_HIP(hipMemcpyAsync(A.vals.p, vals, sizeof(double) * (size_t)A.nnz, hipMemcpyDeviceToDevice, s));
   valuesSet = true;
+  projM = 0;  // the stored solutions belong to the old matrix
   if (rebuild || !amg.built) {
     auto t0 = std::chrono::steady_clock::now();
     drop_graphs();  // the captured cycles point into the old hierarchy
@@ -290,6 +292,109 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
   return it > maxIter ? maxIter : it;
 }
 
+// ------------------------------------------- projection onto previous solutions
+// Successive right-hand sides with one matrix (after Fischer 1998,
+// "Projection techniques for iterative solution of Ax = b with successive
+// right-hand sides", minimal-residual form): B = A X with the columns of B
+// orthonormal in the 2-norm.  The initial guess x0 (the caller's) is moved
+// to x0 + X c with c = B^T r0, which minimises ||b - A x|| over
+// x0 + span X; the residual follows as r0 - B c without an SpMV.  So the
+// start is never worse than x0 in the norm the stop test uses.
+static constexpr int kProjMax = 16;
+
+// out[i] = V_i . y, i < m (block partials, m per block)
+__global__ __launch_bounds__(kB) void k_mdot(int n, int m, const double* __restrict__ V,
+                                             const double* __restrict__ y,
+                                             double* __restrict__ partial) {
+  __shared__ double sh[kProjMax * (kB / 64)];
+  double d[kProjMax];
+#pragma unroll
+  for (int i = 0; i < kProjMax; ++i) d[i] = 0.0;
+  for (int j = blockIdx.x * kB + threadIdx.x; j < n; j += gridDim.x * kB) {
+    const double yj = y[j];
+#pragma unroll
+    for (int i = 0; i < kProjMax; ++i)
+      if (i < m) d[i] += V[(size_t)i * n + j] * yj;
+  }
+  block_sums<kProjMax>(d, sh);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < m; ++i) partial[(size_t)blockIdx.x * m + i] = d[i];
+}
+
+// y += sign * sum_i c_i V_i (terms in index order)
+__global__ __launch_bounds__(kB) void k_mcomb(int n, int m, const double* __restrict__ V,
+                                              const double* __restrict__ c, double sign,
+                                              double* __restrict__ y) {
+  const int j = blockIdx.x * kB + threadIdx.x;
+  if (j >= n) return;
+  double t = 0.0;
+  for (int i = 0; i < m; ++i) t += c[i] * V[(size_t)i * n + j];
+  y[j] = y[j] + sign * t;
+}
+
+// X_m = w / ||Aw||, B_m = Aw / ||Aw||; nu2 = {||Aw||^2 after the
+// orthogonalisation, ||A x||^2 before}.  A remainder below 1e-8 of ||A x||
+// (x already in the span up to rounding) is stored as zero: normalising
+// rounding noise would break B = A X.
+__global__ __launch_bounds__(kB) void k_pscale(int n, const double* __restrict__ w,
+                                               const double* __restrict__ Aw,
+                                               const double* __restrict__ nu2,
+                                               double* __restrict__ X, double* __restrict__ B) {
+  const int j = blockIdx.x * kB + threadIdx.x;
+  if (j >= n) return;
+  const double s = nu2[0] > 1e-16 * nu2[1] && nu2[0] > 0.0 ? 1.0 / sqrt(nu2[0]) : 0.0;
+  X[j] = w[j] * s;
+  B[j] = Aw[j] * s;
+}
+
+// c = B^T r into scal[S_PROJ ..]
+void PcgSolver::proj_coef(const double* rr) {
+  const int n = A.n;
+  const int nb = std::min(grid_for(n, kB), 1024);
+  k_mdot<<<nb, kB, 0, s>>>(n, projM, projB.p, rr, red.partial.p);
+  AN_CHECK_LAUNCH();
+  finalize(red, nb, projM, scal.p + S_PROJ, s);
+}
+
+// after r = b - A x: x += X c, r -= B c, scal[S_RR] = r.r
+void PcgSolver::project_initial(double* x) {
+  if (opt.rhsProjection <= 0 || projM == 0) return;
+  const int n = A.n;
+  const int G = grid_for(n, kB);
+  proj_coef(r.p);
+  k_mcomb<<<G, kB, 0, s>>>(n, projM, projX.p, scal.p + S_PROJ, 1.0, x);
+  k_mcomb<<<G, kB, 0, s>>>(n, projM, projB.p, scal.p + S_PROJ, -1.0, r.p);
+  AN_CHECK_LAUNCH();
+  dot<double>(n, r.p, r.p, scal.p, S_RR, red, s);
+}
+
+// adds the solution x: w = x, Aw = A x, Gram-Schmidt of Aw against B twice
+// (w follows with X), normalised by ||Aw||
+void PcgSolver::project_add(const double* x) {
+  const int maxV = std::min(opt.rhsProjection, kProjMax);
+  if (maxV <= 0 || projM >= maxV) return;
+  const int n = A.n;
+  const int G = grid_for(n, kB);
+  if (!projX.p) {
+    projX.alloc((size_t)maxV * n);
+    projB.alloc((size_t)maxV * n);
+  }
+  spmv<double, double>(A, x, q.p, nullptr, scal.p, 0, red, s);  // q = A x
+  dot<double>(n, q.p, q.p, scal.p, S_NU + 1, red, s);
+  AN_HIP(hipMemcpyAsync(p.p, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  for (int pass = 0; pass < 2 && projM > 0; ++pass) {
+    proj_coef(q.p);
+    k_mcomb<<<G, kB, 0, s>>>(n, projM, projX.p, scal.p + S_PROJ, -1.0, p.p);
+    k_mcomb<<<G, kB, 0, s>>>(n, projM, projB.p, scal.p + S_PROJ, -1.0, q.p);
+    AN_CHECK_LAUNCH();
+  }
+  dot<double>(n, q.p, q.p, scal.p, S_NU, red, s);
+  k_pscale<<<G, kB, 0, s>>>(n, p.p, q.p, scal.p + S_NU, projX.p + (size_t)projM * n,
+                            projB.p + (size_t)projM * n);
+  AN_CHECK_LAUNCH();
+  ++projM;
+}
+
 an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
   if (!valuesSet) throw Error(1, "solver has no matrix values");
   const int n = A.n;
@@ -299,6 +404,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
   // unused when b == 0)
   dot<double>(n, b, b, scal.p, S_BB, red, s);
   residual<double, double>(A, b, x, r.p, scal.p, red, s);
+  project_initial(x);
   AN_HIP(hipMemcpyAsync(hs.h + S_RR, scal.p + S_RR, (S_BB - S_RR + 1) * sizeof(double),
                         hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));
@@ -350,6 +456,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
     inf.finalRelResNorm = rn / bnorm;
     if (!inf.converged) st = AN_ENOCONV;
   }
+  if (bnorm != 0.0) project_add(x);
   inf.solveSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   collect_profile(&inf);
   if (info) *info = inf;

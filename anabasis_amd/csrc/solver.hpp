@@ -56,6 +56,13 @@ class PcgSolver {
   void drop_graphs();
   std::map<long long, hipGraphExec_t> applyGraphs;
   hipStream_t cs = nullptr;  // capture stream
+  // projection onto previous solutions (opt.rhsProjection): B = A X holds
+  // projM 2-norm orthonormal vectors, n doubles each
+  DevBuf<double> projX, projB;
+  int projM = 0;
+  void proj_coef(const double* rr);
+  void project_initial(double* x);
+  void project_add(const double* x);
   int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);

@@ -169,6 +169,12 @@ typedef struct {
                           from a captured HIP graph (captured once per
                           hierarchy build) -- for hosts too slow to keep the
                           GPU fed; 0 (default): launched kernel by kernel */
+  int rhsProjection;   /* successive right-hand sides with one matrix: keep
+                          up to this many (<= 16) previous solutions X
+                          (A X orthonormal) and move x0 to the point of
+                          x0 + span X with the smallest residual before
+                          iterating; the set is cleared when matrix values
+                          are set.  0 (default): off */
 } an_pcg_options;
 void an_pcg_default_options(an_pcg_options *o);
 

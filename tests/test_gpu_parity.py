@@ -237,6 +237,45 @@ def test_pcg_graph_replay_matches_eager(meshes, omeshes):
                 assert np.array_equal(xe, xg), (precision, scale)
 
 
+def test_pcg_rhs_projection(meshes, omeshes):
+    """Successive right-hand sides with one matrix: every solve still meets
+    the tolerance (true residual) and agrees with the plain solve; a repeated
+    right-hand side needs no iteration, nearby ones fewer; new values clear
+    the stored solutions."""
+    m, om = meshes["cyl"], omeshes["cyl"]
+    vals = laplacian(om, m, ("patch_2_0",))
+    A = sp.csr_matrix((vals, om.cols, om.rowOffsets))
+    rng = np.random.default_rng(11)
+    b1 = rng.standard_normal(om.nCells)
+    kw = dict(relTol=0.0, absTol=1e-10, coarseSize=64)
+    P = lp.PCGSolver(om.rowOffsets, om.cols, rhsProjection=8, **kw)
+    Q = lp.PCGSolver(om.rowOffsets, om.cols, **kw)
+    P.set_values(vals)
+    Q.set_values(vals)
+    x1, i1 = P.solve(b1)
+    assert i1.converged and np.linalg.norm(b1 - A @ x1) <= 1.5e-10
+    x2, i2 = P.solve(b1.copy())
+    assert i2.iterations <= 1 and np.linalg.norm(b1 - A @ x2) <= 1.5e-10
+    # right-hand sides of a contracting fixed-point iteration
+    # x_k = A^-1 (b1 + N x_{k-1}), as in the non-orthogonal correction passes
+    # (here N = A S, S = diag(0.5 +- 0.1), so A^-1 N = S)
+    N = A @ sp.diags(0.5 + 0.1 * (rng.random(om.nCells) - 0.5))
+    xp = xq = x1
+    itp = itq = 0
+    for k in range(6):
+        b = b1 + N @ xq
+        xp, ip = P.solve(b, x0=xp)
+        xq, iq = Q.solve(b, x0=xq)
+        assert ip.converged and np.linalg.norm(b - A @ xp) <= 1.5e-10
+        assert np.linalg.norm(xp - xq) <= 1e-8 * np.linalg.norm(xq)
+        itp += ip.iterations
+        itq += iq.iterations
+    assert itp < itq, (itp, itq)
+    P.set_values(2.0 * vals)  # new matrix: the set is cleared
+    x3, i3 = P.solve(b1)
+    assert np.linalg.norm(b1 - 2.0 * (A @ x3)) <= 1.5e-10 and i3.iterations >= i1.iterations - 2
+
+
 def test_pcg_amg_converges_to_direct_solution(meshes, omeshes):
     m, om = meshes["hex"], omeshes["hex"]
     vals = laplacian(om, m)
