@@ -187,6 +187,8 @@ struct DevMesh {
   DevBuf<double> fLam, fDpn, fA, fNx, fNy, fNz, fDx, fDy, fDz;
   DevBuf<int> bP, bPatch;
   DevBuf<double> bDpn, bA, bNx, bNy, bNz, bDx, bDy, bDz;
+  // non-orthogonal face vector T = A n - (A / n.d) d, internal and boundary
+  DevBuf<double> fTx, fTy, fTz, bTx, bTy, bTz;
   void to_device(hipStream_t s);
 };
 

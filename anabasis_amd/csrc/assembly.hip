@@ -50,6 +50,12 @@ MeshView view_of(const DevMesh& d) {
   m.bDx = d.bDx.p;
   m.bDy = d.bDy.p;
   m.bDz = d.bDz.p;
+  m.fTx = d.fTx.p;
+  m.fTy = d.fTy.p;
+  m.fTz = d.fTz.p;
+  m.bTx = d.bTx.p;
+  m.bTy = d.bTy.p;
+  m.bTz = d.bTz.p;
   return m;
 }
 

@@ -15,6 +15,7 @@ struct MeshView {
   const double *fLam, *fDpn, *fA, *fNx, *fNy, *fNz, *fDx, *fDy, *fDz;
   const int *bP, *bPatch;
   const double *bDpn, *bA, *bNx, *bNy, *bNz, *bDx, *bDy, *bDz;
+  const double *fTx, *fTy, *fTz, *bTx, *bTy, *bTz;  // T = A n - (A / n.d) d
 };
 MeshView view_of(const DevMesh& m);
 

@@ -25,6 +25,7 @@ void DevMesh::to_device(hipStream_t s) {
     const size_t ni = (size_t)(nIF > 0 ? nIF : 1);
     std::vector<int2> pn(ni);
     std::vector<double> lam(ni), dpn(ni), A(ni), nx(ni), ny(ni), nz(ni), dx(ni), dy(ni), dz(ni);
+    std::vector<double> tx(ni), ty(ni), tz(ni);
     for (int f = 0; f < nIF; ++f) {
       int P = m.owner[f], N = m.neigh[f];
       pn[f] = make_int2(P, N);
@@ -47,6 +48,11 @@ void DevMesh::to_device(hipStream_t s) {
       dx[f] = d0;
       dy[f] = d1;
       dz[f] = d2;
+      // T = A n - (A / n.d) d as the face kernels write it (unused when
+      // |n.d| <= 1e-30: those faces are skipped)
+      tx[f] = A[f] * n[0] - (A[f] / dpn[f]) * d0;
+      ty[f] = A[f] * n[1] - (A[f] / dpn[f]) * d1;
+      tz[f] = A[f] * n[2] - (A[f] / dpn[f]) * d2;
     }
     fPN.upload(pn, s);
     fLam.upload(lam, s);
@@ -58,9 +64,13 @@ void DevMesh::to_device(hipStream_t s) {
     fDx.upload(dx, s);
     fDy.upload(dy, s);
     fDz.upload(dz, s);
+    fTx.upload(tx, s);
+    fTy.upload(ty, s);
+    fTz.upload(tz, s);
     const size_t nb = (size_t)(nBF > 0 ? nBF : 1);
     std::vector<int> bp(nb), bpa(nb);
     std::vector<double> bdpn(nb), bA_(nb), bnx(nb), bny(nb), bnz(nb), bdx(nb), bdy(nb), bdz(nb);
+    std::vector<double> btx(nb), bty(nb), btz(nb);
     for (int b = 0; b < nBF; ++b) {
       int f = nIF + b;
       int P = m.owner[f];
@@ -78,6 +88,9 @@ void DevMesh::to_device(hipStream_t s) {
       bnx[b] = n[0];
       bny[b] = n[1];
       bnz[b] = n[2];
+      btx[b] = bA_[b] * n[0] - (bA_[b] / bdpn[b]) * d0;
+      bty[b] = bA_[b] * n[1] - (bA_[b] / bdpn[b]) * d1;
+      btz[b] = bA_[b] * n[2] - (bA_[b] / bdpn[b]) * d2;
     }
     bP.upload(bp, s);
     bPatch.upload(bpa, s);
@@ -89,6 +102,9 @@ void DevMesh::to_device(hipStream_t s) {
     bDx.upload(bdx, s);
     bDy.upload(bdy, s);
     bDz.upload(bdz, s);
+    bTx.upload(btx, s);
+    bTy.upload(bty, s);
+    bTz.upload(btz, s);
   }
   AN_HIP(hipStreamSynchronize(s));
   onDevice = true;

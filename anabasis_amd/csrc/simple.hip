@@ -156,12 +156,10 @@ __global__ __launch_bounds__(kB) void k_mom_faces(MeshView m, Mom3Args a,
   if (f >= m.nIF) return;
   const double nd = m.fDpn[f];
   if (nd <= 1e-14) return;  // skipped by the cells as well
-  const double af = m.fA[f];
-  const double dx = m.fDx[f], dy = m.fDy[f], dz = m.fDz[f];
-  const double nx = m.fNx[f], ny = m.fNy[f], nz = m.fNz[f];
-  const double tcx = a.mu * (af * nx - (af / nd) * dx);
-  const double tcy = a.mu * (af * ny - (af / nd) * dy);
-  const double tcz = a.mu * (af * nz - (af / nd) * dz);
+  // mu (A n - (A / n.d) d) with the precomputed T (the same value)
+  const double tcx = a.mu * m.fTx[f];
+  const double tcy = a.mu * m.fTy[f];
+  const double tcz = a.mu * m.fTz[f];
   const double lam = m.fLam[f];
   const int2 pn = m.fPN[f];
 #pragma unroll
@@ -176,6 +174,8 @@ __global__ __launch_bounds__(kB) void k_mom_faces(MeshView m, Mom3Args a,
     corr[(size_t)c * m.nIF + f] = a.corrPsi * (gfx * tcx + gfy * tcy + gfz * tcz);
   }
   if (FULL) {
+    const double af = m.fA[f];
+    const double nx = m.fNx[f], ny = m.fNy[f], nz = m.fNz[f];
     const double alpha = a.mu * af / nd;
     const double ucf = (1.0 - lam) * a.uC[pn.x] + lam * a.uC[pn.y];
     const double vcf = (1.0 - lam) * a.vC[pn.x] + lam * a.vC[pn.y];
@@ -387,15 +387,12 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
     if (fabs(dpn) > 1.0e-30) {
       const int2 pn = m.fPN[f];
       const double lam = m.fLam[f];
-      const double Af = m.fA[f];
       const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
       const double gfx = (1.0 - lam) * gx[pn.x] + lam * gx[pn.y];
       const double gfy = (1.0 - lam) * gy[pn.x] + lam * gy[pn.y];
       const double gfz = (1.0 - lam) * gz[pn.x] + lam * gz[pn.y];
-      const double tx = Af * m.fNx[f] - (Af / dpn) * m.fDx[f];
-      const double ty = Af * m.fNy[f] - (Af / dpn) * m.fDy[f];
-      const double tz = Af * m.fNz[f] - (Af / dpn) * m.fDz[f];
-      flux = rAUf * (gfx * tx + gfy * ty + gfz * tz);
+      // T = A n - (A / n.d) d, precomputed with the same expressions
+      flux = rAUf * (gfx * m.fTx[f] + gfy * m.fTy[f] + gfz * m.fTz[f]);
     }
   } else {
     const int b = f - m.nIF;
@@ -403,11 +400,7 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
       const double dpn = m.bDpn[b];
       if (fabs(dpn) > 1.0e-30) {
         const int P = m.bP[b];
-        const double Af = m.bA[b];
-        const double tx = Af * m.bNx[b] - (Af / dpn) * m.bDx[b];
-        const double ty = Af * m.bNy[b] - (Af / dpn) * m.bDy[b];
-        const double tz = Af * m.bNz[b] - (Af / dpn) * m.bDz[b];
-        flux = rAU[P] * (gx[P] * tx + gy[P] * ty + gz[P] * tz);
+        flux = rAU[P] * (gx[P] * m.bTx[b] + gy[P] * m.bTy[b] + gz[P] * m.bTz[b]);
       }
     }
   }
