@@ -750,7 +750,7 @@ void an_simple_default_params(an_simple_params* p) {
   p->pressure.strength = 0.1;
   // the non-orthogonal passes solve one matrix with successive right-hand
   // sides: start each from the projection onto the earlier passes' solutions
-  p->pressure.rhsProjection = 16;
+  p->pressure.rhsProjection = 6;
 }
 
 an_status an_simple_create(an_mesh_t mesh, const an_simple_params* par, const int* patchUType,

@@ -321,15 +321,23 @@ __global__ __launch_bounds__(kB) void k_mdot(int n, int m, const double* __restr
     for (int i = 0; i < m; ++i) partial[(size_t)blockIdx.x * m + i] = d[i];
 }
 
-// y += sign * sum_i c_i V_i (terms in index order)
-__global__ __launch_bounds__(kB) void k_mcomb(int n, int m, const double* __restrict__ V,
-                                              const double* __restrict__ c, double sign,
-                                              double* __restrict__ y) {
+// x += sx * X c and y += sy * B c in one pass (sums in index order); the
+// same c keeps B = A X between the pair
+__global__ __launch_bounds__(kB) void k_mcomb2(int n, int m, const double* __restrict__ X,
+                                               const double* __restrict__ B,
+                                               const double* __restrict__ c, double sx,
+                                               double* __restrict__ x, double sy,
+                                               double* __restrict__ y) {
   const int j = blockIdx.x * kB + threadIdx.x;
   if (j >= n) return;
-  double t = 0.0;
-  for (int i = 0; i < m; ++i) t += c[i] * V[(size_t)i * n + j];
-  y[j] = y[j] + sign * t;
+  double tx = 0.0, ty = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double ci = c[i];
+    tx += ci * X[(size_t)i * n + j];
+    ty += ci * B[(size_t)i * n + j];
+  }
+  x[j] = x[j] + sx * tx;
+  y[j] = y[j] + sy * ty;
 }
 
 // X_m = w / ||Aw||, B_m = Aw / ||Aw||; nu2 = {||Aw||^2 after the
@@ -362,14 +370,14 @@ void PcgSolver::project_initial(double* x) {
   const int n = A.n;
   const int G = grid_for(n, kB);
   proj_coef(r.p);
-  k_mcomb<<<G, kB, 0, s>>>(n, projM, projX.p, scal.p + S_PROJ, 1.0, x);
-  k_mcomb<<<G, kB, 0, s>>>(n, projM, projB.p, scal.p + S_PROJ, -1.0, r.p);
+  k_mcomb2<<<G, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, 1.0, x, -1.0, r.p);
   AN_CHECK_LAUNCH();
   dot<double>(n, r.p, r.p, scal.p, S_RR, red, s);
 }
 
-// adds the solution x: w = x, Aw = A x, Gram-Schmidt of Aw against B twice
-// (w follows with X), normalised by ||Aw||
+// adds the solution x: w = x, Aw = A x, one classical Gram-Schmidt pass of
+// Aw against B (w follows with X, so B = A X holds whatever the remaining
+// loss of orthogonality), normalised by ||Aw||
 void PcgSolver::project_add(const double* x) {
   const int maxV = std::min(opt.rhsProjection, kProjMax);
   if (maxV <= 0 || projM >= maxV) return;
@@ -382,10 +390,9 @@ void PcgSolver::project_add(const double* x) {
   spmv<double, double>(A, x, q.p, nullptr, scal.p, 0, red, s);  // q = A x
   dot<double>(n, q.p, q.p, scal.p, S_NU + 1, red, s);
   AN_HIP(hipMemcpyAsync(p.p, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
-  for (int pass = 0; pass < 2 && projM > 0; ++pass) {
+  if (projM > 0) {
     proj_coef(q.p);
-    k_mcomb<<<G, kB, 0, s>>>(n, projM, projX.p, scal.p + S_PROJ, -1.0, p.p);
-    k_mcomb<<<G, kB, 0, s>>>(n, projM, projB.p, scal.p + S_PROJ, -1.0, q.p);
+    k_mcomb2<<<G, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, -1.0, p.p, -1.0, q.p);
     AN_CHECK_LAUNCH();
   }
   dot<double>(n, q.p, q.p, scal.p, S_NU, red, s);
