@@ -189,6 +189,7 @@ struct DevMesh {
   DevBuf<double> bDpn, bA, bNx, bNy, bNz, bDx, bDy, bDz;
   // non-orthogonal face vector T = A n - (A / n.d) d, internal and boundary
   DevBuf<double> fTx, fTy, fTz, bTx, bTy, bTz;
+  DevBuf<double> gsvInv;  // GSV1 LSQ normal-matrix inverse per cell (9 x nC)
   void to_device(hipStream_t s);
 };
 

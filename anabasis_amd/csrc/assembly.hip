@@ -56,6 +56,7 @@ MeshView view_of(const DevMesh& d) {
   m.bTx = d.bTx.p;
   m.bTy = d.bTy.p;
   m.bTz = d.bTz.p;
+  m.gsvInv = d.gsvInv.p;
   return m;
 }
 

@@ -143,10 +143,100 @@ __global__ __launch_bounds__(256) void k_lsq_lp(MeshView m, BCView bc,
   gz[P] = g2;
 }
 
+// The GSV1 normal matrix is geometry only (every neighbour and every
+// boundary face enters it whatever its BC), so its inverse is computed once
+// per mesh (k_lsq_gsv1's expressions: the same bits) and stored per cell as
+// nine SoA arrays, read coalesced by the gradient kernel below.
+__global__ __launch_bounds__(256) void k_gsv1_inverse(MeshView m, double* __restrict__ inv9) {
+  const int P = blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= m.nC) return;
+  const double4 xP = m.cgeo[P];
+  double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = m.rowOff[P]; k < m.rowOff[P + 1]; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+  }
+  for (int j = m.cbOff[P]; j < m.cbOff[P + 1]; ++j) {
+    const int b = m.cbFace[j];
+    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    acc_M(M, r0, r1, r2, w);
+  }
+  double inv[9];
+  inv3(M, inv);
+  for (int i = 0; i < 9; ++i) inv9[(size_t)i * m.nC + P] = inv[i];
+}
+
+__global__ __launch_bounds__(256) void k_lsq_gsv1_inv(MeshView m, BCView bc,
+                                                      const double* __restrict__ phi,
+                                                      double* __restrict__ gx,
+                                                      double* __restrict__ gy,
+                                                      double* __restrict__ gz) {
+  const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (P >= m.nC) return;
+  const size_t nC = (size_t)m.nC;
+  double inv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) inv[i] = m.gsvInv[i * nC + P];
+  const double4 xP = m.cgeo[P];
+  const double phiP = phi[P];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const int re = m.rowOff[P + 1];
+  for (int k = m.rowOff[P]; k < re; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+    const double dphi = phi[N] - phiP;
+    sx += cx * dphi;
+    sy += cy * dphi;
+    sz += cz * dphi;
+  }
+  const int be = m.cbOff[P + 1];
+  for (int j = m.cbOff[P]; j < be; ++j) {
+    const int b = m.cbFace[j];
+    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+    const double val = (bc.type[b] == 1) ? bc.val[b] : phiP;
+    const double dphi = val - phiP;
+    sx += cx * dphi;
+    sy += cy * dphi;
+    sz += cz * dphi;
+  }
+  gx[P] = sx;
+  gy[P] = sy;
+  gz[P] = sz;
+}
+
+void build_gsv1_inverse(const MeshView& m, DevBuf<double>& inv9, hipStream_t s) {
+  inv9.alloc(9 * (size_t)(m.nC > 0 ? m.nC : 1));
+  k_gsv1_inverse<<<grid_for(m.nC, 256), 256, 0, s>>>(m, inv9.p);
+  AN_CHECK_LAUNCH();
+}
+
 void launch_lsq_gradient(const MeshView& m, int mode, BCView bc, const double* phi, double* gx,
                          double* gy, double* gz, hipStream_t s) {
   const int B = 256;
-  if (mode == 0)
+  if (mode == 0 && m.gsvInv)
+    k_lsq_gsv1_inv<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
+  else if (mode == 0)
     k_lsq_gsv1<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
   else
     k_lsq_lp<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);

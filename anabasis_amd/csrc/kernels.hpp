@@ -16,8 +16,11 @@ struct MeshView {
   const int *bP, *bPatch;
   const double *bDpn, *bA, *bNx, *bNy, *bNz, *bDx, *bDy, *bDz;
   const double *fTx, *fTy, *fTz, *bTx, *bTy, *bTz;  // T = A n - (A / n.d) d
+  const double* gsvInv;  // GSV1 LSQ normal-matrix inverse, 9 x nC SoA (may be null)
 };
 MeshView view_of(const DevMesh& m);
+// the GSV1 gradient's per-cell normal-matrix inverse (geometry only)
+void build_gsv1_inverse(const MeshView& m, DevBuf<double>& inv9, hipStream_t s);
 
 // ------------------------------------------------ shared device helpers
 // LSQ normal matrix and its inverse (gradient.hip, adjoint.hip)

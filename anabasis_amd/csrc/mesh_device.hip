@@ -1,6 +1,7 @@
 // mesh_device.hip -- uploads a HostMesh into the HBM layout described in
 // an_internal.hpp and builds the GPU addressing.
 #include "an_internal.hpp"
+#include "kernels.hpp"
 
 namespace an {
 
@@ -105,6 +106,7 @@ void DevMesh::to_device(hipStream_t s) {
     bTx.upload(btx, s);
     bTy.upload(bty, s);
     bTz.upload(btz, s);
+    build_gsv1_inverse(view_of(*this), gsvInv, s);
   }
   AN_HIP(hipStreamSynchronize(s));
   onDevice = true;
