@@ -989,11 +989,13 @@ static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
 }
 
 void AmgHierarchy::apply(const double* r, double* z, hipStream_t s, const ApplyDots* dots) {
-  vcycle<double>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1), dots);
+  const double* y = dots && dots->yReady ? lv[0].y.p : nullptr;
+  vcycle<double>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots);
 }
 
 void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s, const ApplyDots* dots) {
-  vcycle<float>(*this, 0, r, nullptr, z, s, std::max(opt.sweeps, 1), dots);
+  const float* y = dots && dots->yReady ? lv[0].y32.p : nullptr;
+  vcycle<float>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots);
 }
 
 }  // namespace an

@@ -38,6 +38,7 @@ struct AmgLevel {
 struct ApplyDots {
   const void* q;
   RedArgs ra;
+  bool yReady = false;  // level 0's y = omdinv o r already written (by the PCG update)
 };
 
 class AmgHierarchy {

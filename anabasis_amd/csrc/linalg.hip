@@ -270,7 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const double* scal, int num, int den,
-                                                       RedArgs ra) {
+                                                       RedArgs ra, const T* __restrict__ omd,
+                                                       T* __restrict__ y) {
   __shared__ double sh[kWaves];
   const double alpha = scal[num] / scal[den];
   double v = 0.0;
@@ -278,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
     x[i] = (T)((double)x[i] + alpha * (double)p[i]);
     const T ri = (T)((double)r[i] - alpha * (double)q[i]);
     r[i] = ri;
+    if (y) y[i] = (T)((double)omd[i] * (double)ri);  // the preconditioner's first sweep
     v += (double)ri * (double)ri;
   }
   const double bs = block_sum(v, sh);
@@ -286,9 +288,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
 
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
-                Reducer& red, hipStream_t s) {
+                Reducer& red, hipStream_t s, const T* omd, T* y) {
   const int nb = red_blocks(n);
-  k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR));
+  k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR), omd,
+                                        y);
   AN_CHECK_LAUNCH();
   finalize(red, nb, 1, scal + S_RR, s);
 }
@@ -354,9 +357,9 @@ template void dot2<double>(int, const double*, const double*, const double*, dou
 template void dot2<float>(int, const float*, const float*, const float*, double*, int, Reducer&,
                           hipStream_t);
 template void pcg_update<double>(int, double*, double*, const double*, const double*, double*, int,
-                                 int, Reducer&, hipStream_t);
+                                 int, Reducer&, hipStream_t, const double*, double*);
 template void pcg_update<float>(int, float*, float*, const float*, const float*, double*, int, int,
-                                Reducer&, hipStream_t);
+                                Reducer&, hipStream_t, const float*, float*);
 template void pcg_direction<double>(int, double*, const double*, double*, int, int, bool,
                                     hipStream_t, double);
 template void pcg_direction<float>(int, float*, const float*, double*, int, int, bool,

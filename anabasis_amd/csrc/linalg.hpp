@@ -160,10 +160,11 @@ template <typename T>
 void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Reducer& red,
           hipStream_t s);
 
-// x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den]
+// x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den];
+// with y: also y = omd o r (the preconditioner's first Jacobi sweep)
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
-                Reducer& red, hipStream_t s);
+                Reducer& red, hipStream_t s, const T* omd = nullptr, T* y = nullptr);
 // p = z + beta p, beta = sign * scal[num]/scal[den]  (first: p = z)
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,

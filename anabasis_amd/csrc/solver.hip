@@ -154,7 +154,8 @@ void PcgSolver::precondition(bool fp32, const ApplyDots& d) {
     launch(s);
     return;
   }
-  const long long key = ((long long)(d.ra.out - scal.p) << 2) | (d.q ? 2 : 0) | (fp32 ? 1 : 0);
+  const long long key = ((long long)(d.ra.out - scal.p) << 3) | (d.yReady ? 4 : 0) |
+                        (d.q ? 2 : 0) | (fp32 ? 1 : 0);
   auto it = applyGraphs.find(key);
   if (it == applyGraphs.end()) {
     // captured on a private non-blocking stream (the solver's own stream may
@@ -215,14 +216,16 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                          e1 ? e1->b : nullptr);
-    pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s);
+    // the update also writes the preconditioner's y = omdinv o r
+    pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv.p,
+                       amg.lv[0].y.p);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
-      const ApplyDots dd = flex ? ApplyDots{q.p, red.args(scal.p + S_FRZ)}
-                                : ApplyDots{nullptr, red.args(scal.p + rhoNew)};
+      const ApplyDots dd = flex ? ApplyDots{q.p, red.args(scal.p + S_FRZ), true}
+                                : ApplyDots{nullptr, red.args(scal.p + rhoNew), true};
       precondition(false, dd);
       if (e2) AN_HIP(hipEventRecord(e2->b, s));
       if (flex)
@@ -263,14 +266,15 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                        e1 ? e1->b : nullptr);
-    pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s);
+    pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s,
+                      amg.lv[0].omdinv32.p, amg.lv[0].y32.p);
     AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     AN_HIP(hipEventRecord(ev, s));
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
-      const ApplyDots dd = flex ? ApplyDots{q32.p, red.args(scal.p + S_FRZ)}
-                                : ApplyDots{nullptr, red.args(scal.p + rhoNew)};
+      const ApplyDots dd = flex ? ApplyDots{q32.p, red.args(scal.p + S_FRZ), true}
+                                : ApplyDots{nullptr, red.args(scal.p + rhoNew), true};
       precondition(true, dd);
       if (e2) AN_HIP(hipEventRecord(e2->b, s));
       if (flex)
