@@ -134,10 +134,31 @@ __global__ void k_fill_members(int n, const int* __restrict__ agg, int* __restri
   if (i < n) mem[atomicAdd(&cursor[agg[i]], 1)] = i;
 }
 
+// members of each aggregate into ascending order: up to 16 sorted in
+// registers (odd-even transposition network, padding INT_MAX), larger
+// aggregates by insertion in place
 __global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __restrict__ mem) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
   int b = moff[I], e = moff[I + 1];
+  if (e - b <= 16) {
+    int v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = (b + u < e) ? mem[b + u] : 0x7fffffff;
+#pragma unroll
+    for (int round = 0; round < 16; ++round) {
+#pragma unroll
+      for (int u = round & 1; u + 1 < 16; u += 2) {
+        const int lo = min(v[u], v[u + 1]), hi = max(v[u], v[u + 1]);
+        v[u] = lo;
+        v[u + 1] = hi;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (b + u < e) mem[b + u] = v[u];
+    return;
+  }
   for (int a = b + 1; a < e; ++a) {
     int x = mem[a];
     int j = a - 1;
