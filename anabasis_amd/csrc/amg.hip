@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
   const int i = q.r0 + lane;
   auto vj = [&](int j) { return (double)v[j]; };
   int kb = 0, ke = 0;
-  double bi = 0.0, vi = 0.0, oi = 0.0;
+  double bi = 0.0, vi = 0.0, oi = 0.0, qi = 0.0;
   if (i < q.r1) {
     kb = ro[i];
     ke = ro[i + 1];
@@ -353,6 +353,7 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
       vi = (double)v[i];
       oi = (double)omdinv[i];
     }
+    if (DOT == 2) qi = (double)qv[i];
   }
   if (q.stream && q.k1 > q.k0) stage_products(q, lane, ci, va, vj, prod[w]);
   wave_sync();
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
       const T o = (T)(vi + oi * (bi - s));
       out[i] = o;
       if (DOT >= 1) d[0] = bi * (double)o;
-      if (DOT == 2) d[1] = (double)o * (double)qv[i];
+      if (DOT == 2) d[1] = (double)o * qi;
     }
   }
   if constexpr (DOT > 0) {

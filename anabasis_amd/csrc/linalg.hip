@@ -3,7 +3,7 @@
 //
 // SpMV (CSR-stream, wave granularity, after Greathouse & Daga's CSR-Adaptive,
 // SC'14): each 64-lane wavefront owns one row block (at most 64 consecutive
-// rows holding at most 768 nonzeros, built once per pattern).  The lanes
+// rows holding at most 512 nonzeros, built once per pattern).  The lanes
 // first stream the block's nonzeros with fully coalesced loads of vals/cols
 // (lane k reads nonzero k0+k), multiply by the gathered x and stage the
 // products in the wave's LDS slice; then lane i sums row i's products from
@@ -26,7 +26,7 @@ namespace an {
 
 static constexpr int kBlock = 256;
 static constexpr int kWaves = kBlock / 64;
-static constexpr int kCap = kStreamCap;  // staged products per wave (6 KB)
+static constexpr int kCap = kStreamCap;  // staged products per wave (4 KB)
 static constexpr int kRedBlocks = 1024;
 
 // ------------------------------------------------------------ row blocks
@@ -101,14 +101,46 @@ __global__ __launch_bounds__(1024) void k_finalize(int nparts, int comps,
   }
 }
 
-void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s) {
-  k_finalize<<<1, 1024, 0, s>>>(nparts, comps, red.partial.p, out);
+// The same sums for a fixed component count in one pass: every thread loads
+// all NC components of its partials (four partials in flight) and the NC
+// block sums share one barrier round.  Each component is added in the order
+// of k_finalize, so the results are the same bits.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_finalize_nc(int nparts,
+                                                      const double* __restrict__ partial,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[NC * 16];
+  double v[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[c] = 0.0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += partial[(size_t)i * NC + c];
+  block_sums<NC>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = v[c];
+}
+
+static void launch_finalize(int nparts, int comps, const double* partial, double* out,
+                            hipStream_t s) {
+  switch (comps) {
+    case 1: k_finalize_nc<1><<<1, 1024, 0, s>>>(nparts, partial, out); break;
+    case 2: k_finalize_nc<2><<<1, 1024, 0, s>>>(nparts, partial, out); break;
+    case 3: k_finalize_nc<3><<<1, 1024, 0, s>>>(nparts, partial, out); break;
+    case 6: k_finalize_nc<6><<<1, 1024, 0, s>>>(nparts, partial, out); break;
+    default: k_finalize<<<1, 1024, 0, s>>>(nparts, comps, partial, out);
+  }
   AN_CHECK_LAUNCH();
 }
 
+void finalize(const Reducer& red, int nparts, int comps, double* out, hipStream_t s) {
+  launch_finalize(nparts, comps, red.partial.p, out, s);
+}
+
 void finalize_parts(const RedArgs& ra, int nparts, int comps, hipStream_t s) {
-  k_finalize<<<1, 1024, 0, s>>>(nparts, comps, ra.partial, ra.out);
-  AN_CHECK_LAUNCH();
+  launch_finalize(nparts, comps, ra.partial, ra.out, s);
 }
 
 void Reducer::init(int maxBlocks, hipStream_t s) {
@@ -134,10 +166,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kWaves + w);
   const int row = q.r0 + lane;
+  // the row's own operands are loaded with the row offsets, so their latency
+  // hides behind the nonzero stream
   int kb = 0, ke = 0;
+  double bi = 0.0, wi = 0.0;
   if (row < q.r1) {
     kb = rowOff[row];
     ke = rowOff[row + 1];
+    if (MODE == 1) bi = b[row];
+    if (DOT == 1) wi = (double)dw[row];
   }
   if (q.stream && q.k1 > q.k0)
     stage_products(q, lane, cols, vals, [&](int j) { return (double)x[j]; }, prod[w]);
@@ -151,11 +188,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
       for (int k = kb; k < ke; ++k) s += (double)vals[k] * (double)x[cols[k]];
     YT v;
     if (MODE == 1)
-      v = (YT)(b[row] - s);
+      v = (YT)(bi - s);
     else
       v = (YT)s;
     y[row] = v;
-    if (DOT == 1) dv = (double)dw[row] * (double)v;
+    if (DOT == 1) dv = wi * (double)v;
     if (DOT == 2) dv = (double)v * (double)v;
   }
   if (DOT) {
