@@ -225,6 +225,74 @@ __global__ __launch_bounds__(256) void k_lsq_gsv1_inv(MeshView m, BCView bc,
   gz[P] = sz;
 }
 
+__global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) {
+  const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (P >= m.nC) return;
+  const size_t nC = (size_t)m.nC;
+  double inv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) inv[i] = m.gsvInv[i * nC + P];
+  const double4 xP = m.cgeo[P];
+  double phiP[3], sx[3], sy[3], sz[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    phiP[f] = a.phi[f][P];
+    sx[f] = sy[f] = sz[f] = 0.0;
+  }
+  const int re = m.rowOff[P + 1];
+  for (int k = m.rowOff[P]; k < re; ++k) {
+    const int N = m.cols[k];
+    if (N == P) continue;
+    const double4 xN = m.cgeo[N];
+    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double dphi = a.phi[f][N] - phiP[f];
+      sx[f] += cx * dphi;
+      sy[f] += cy * dphi;
+      sz[f] += cz * dphi;
+    }
+  }
+  const int be = m.cbOff[P + 1];
+  for (int j = m.cbOff[P]; j < be; ++j) {
+    const int b = m.cbFace[j];
+    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
+    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+    const double bx = w * r0, by = w * r1, bz = w * r2;
+    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+    const bool fixed = a.bcType[b] == 1;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double val = fixed ? a.bcVal[f][b] : phiP[f];
+      const double dphi = val - phiP[f];
+      sx[f] += cx * dphi;
+      sy[f] += cy * dphi;
+      sz[f] += cz * dphi;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    a.g[f][0][P] = sx[f];
+    a.g[f][1][P] = sy[f];
+    a.g[f][2][P] = sz[f];
+  }
+}
+
+void launch_lsq_gradient3(const MeshView& m, const Grad3Args& a, hipStream_t s) {
+  if (!m.gsvInv) throw Error(7, "three-field gradient needs the cached GSV1 inverse");
+  k_lsq_gsv1_inv3<<<grid_for(m.nC, 256), 256, 0, s>>>(m, a);
+  AN_CHECK_LAUNCH();
+}
+
 void build_gsv1_inverse(const MeshView& m, DevBuf<double>& inv9, hipStream_t s) {
   inv9.alloc(9 * (size_t)(m.nC > 0 ? m.nC : 1));
   k_gsv1_inverse<<<grid_for(m.nC, 256), 256, 0, s>>>(m, inv9.p);

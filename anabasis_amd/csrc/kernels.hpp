@@ -101,6 +101,17 @@ struct BCView {
 // mode 0: GSV1 operator semantics, 1: LP compute_lsq_gradient.  Output SoA.
 void launch_lsq_gradient(const MeshView& m, int mode, BCView bc, const double* phi, double* gx,
                          double* gy, double* gz, hipStream_t s);
+// GSV1 gradients of three fields sharing one BC type array (u, v, w): the
+// geometry, the cached inverse and the adjacency are read once; each field's
+// sums are those of launch_lsq_gradient (mode 0), so the results are the
+// same bits.  Needs the cached GSV1 inverse.
+struct Grad3Args {
+  const int* bcType;
+  const double* bcVal[3];
+  const double* phi[3];
+  double* g[3][3];  // g[field][component]
+};
+void launch_lsq_gradient3(const MeshView& m, const Grad3Args& a, hipStream_t s);
 
 // ----------------------------------------------------------- assembly
 void launch_elliptic_assemble(const MeshView& m, const double* gammaFace,

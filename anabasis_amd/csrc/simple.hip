@@ -846,10 +846,21 @@ void SimpleSolver::momentum_fused(an_simple_stats* st) {
   int active = 7;
   for (int it = 1; it <= nVel && active; ++it) {
     mark(0);
-    for (int c = 0; c < 3; ++c) {
-      if (!((active >> c) & 1)) continue;
-      BCView bc{bcU.p, bcUv[c].p};
-      launch_lsq_gradient(mv, 0, bc, q[c].p, g3[c][0].p, g3[c][1].p, g3[c][2].p, s);
+    if (active == 7 && mv.gsvInv) {
+      Grad3Args ga;
+      ga.bcType = bcU.p;
+      for (int c = 0; c < 3; ++c) {
+        ga.bcVal[c] = bcUv[c].p;
+        ga.phi[c] = q[c].p;
+        for (int d = 0; d < 3; ++d) ga.g[c][d] = g3[c][d].p;
+      }
+      launch_lsq_gradient3(mv, ga, s);
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        if (!((active >> c) & 1)) continue;
+        BCView bc{bcU.p, bcUv[c].p};
+        launch_lsq_gradient(mv, 0, bc, q[c].p, g3[c][0].p, g3[c][1].p, g3[c][2].p, s);
+      }
     }
     // the three components' rhs (and in the first pass the shared matrix,
     // built as the x component's full assembly) from one face pass and one
@@ -977,7 +988,8 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
     tAcc[3] += tP;
     st.massRes = maxabs(divCorr.p);
     k_p_update<<<grid_for(nC, kB), kB, 0, s>>>(nC, p.p, pCorr.p, par.pRelax);
-    grad(1, pCorr.p, g);
+    // g still holds grad(pCorr) of the last pass (pCorr has not changed
+    // since), which is what the velocity correction takes
     k_vel_correct<<<grid_for(nC, kB), kB, 0, s>>>(nC, q[0].p, q[1].p, q[2].p, rAU.p, g[0].p,
                                                   g[1].p, g[2].p);
     AN_CHECK_LAUNCH();
