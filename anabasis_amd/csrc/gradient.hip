@@ -173,59 +173,22 @@ __global__ __launch_bounds__(256) void k_gsv1_inverse(MeshView m, double* __rest
   for (int i = 0; i < 9; ++i) inv9[(size_t)i * m.nC + P] = inv[i];
 }
 
-__global__ __launch_bounds__(256) void k_lsq_gsv1_inv(MeshView m, BCView bc,
-                                                      const double* __restrict__ phi,
-                                                      double* __restrict__ gx,
-                                                      double* __restrict__ gy,
-                                                      double* __restrict__ gz) {
-  const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-  if (P >= m.nC) return;
-  const size_t nC = (size_t)m.nC;
-  double inv[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) inv[i] = m.gsvInv[i * nC + P];
-  const double4 xP = m.cgeo[P];
-  const double phiP = phi[P];
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  const int re = m.rowOff[P + 1];
-  for (int k = m.rowOff[P]; k < re; ++k) {
-    const int N = m.cols[k];
-    if (N == P) continue;
-    const double4 xN = m.cgeo[N];
-    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
-    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
-    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
-    const double bx = w * r0, by = w * r1, bz = w * r2;
-    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
-    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
-    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
-    const double dphi = phi[N] - phiP;
-    sx += cx * dphi;
-    sy += cy * dphi;
-    sz += cz * dphi;
-  }
-  const int be = m.cbOff[P + 1];
-  for (int j = m.cbOff[P]; j < be; ++j) {
-    const int b = m.cbFace[j];
-    const double r0 = m.bDx[b], r1 = m.bDy[b], r2 = m.bDz[b];
-    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
-    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
-    const double bx = w * r0, by = w * r1, bz = w * r2;
-    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
-    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
-    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
-    const double val = (bc.type[b] == 1) ? bc.val[b] : phiP;
-    const double dphi = val - phiP;
-    sx += cx * dphi;
-    sy += cy * dphi;
-    sz += cz * dphi;
-  }
-  gx[P] = sx;
-  gy[P] = sy;
-  gz[P] = sz;
-}
+// NF fields sharing one BC type array, with the cached inverse: a cell's
+// neighbours are taken kNbChunk at a time, their column indices first and
+// then all their centroid and value gathers, so a thread waits on two memory
+// round trips per chunk instead of two per neighbour.  The terms are added
+// in the row's order (the diagonal skipped), as in k_lsq_gsv1.
+static constexpr int kNbChunk = 4;
+template <int NF>
+struct GradFields {
+  const double* phi[NF];
+  const double* bval[NF];
+  double* g[NF][3];
+};
 
-__global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) {
+template <int NF>
+__global__ __launch_bounds__(256) void k_lsq_gsv1_inv(MeshView m, const int* __restrict__ bcType,
+                                                      GradFields<NF> a) {
   const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (P >= m.nC) return;
   const size_t nC = (size_t)m.nC;
@@ -233,30 +196,44 @@ __global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) 
 #pragma unroll
   for (int i = 0; i < 9; ++i) inv[i] = m.gsvInv[i * nC + P];
   const double4 xP = m.cgeo[P];
-  double phiP[3], sx[3], sy[3], sz[3];
+  double phiP[NF], sx[NF], sy[NF], sz[NF];
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
+  for (int f = 0; f < NF; ++f) {
     phiP[f] = a.phi[f][P];
     sx[f] = sy[f] = sz[f] = 0.0;
   }
-  const int re = m.rowOff[P + 1];
-  for (int k = m.rowOff[P]; k < re; ++k) {
-    const int N = m.cols[k];
-    if (N == P) continue;
-    const double4 xN = m.cgeo[N];
-    const double r0 = xN.x - xP.x, r1 = xN.y - xP.y, r2 = xN.z - xP.z;
-    const double rr = r0 * r0 + r1 * r1 + r2 * r2;
-    const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
-    const double bx = w * r0, by = w * r1, bz = w * r2;
-    const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
-    const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
-    const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+  const int rb = m.rowOff[P], re = m.rowOff[P + 1];
+  for (int k0 = rb; k0 < re; k0 += kNbChunk) {
+    int nb[kNbChunk];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const double dphi = a.phi[f][N] - phiP[f];
-      sx[f] += cx * dphi;
-      sy[f] += cy * dphi;
-      sz[f] += cz * dphi;
+    for (int u = 0; u < kNbChunk; ++u) nb[u] = m.cols[min(k0 + u, re - 1)];
+    double px[kNbChunk], py[kNbChunk], pz[kNbChunk], ph[kNbChunk][NF];
+#pragma unroll
+    for (int u = 0; u < kNbChunk; ++u) {
+      const double4 xN = m.cgeo[nb[u]];
+      px[u] = xN.x;
+      py[u] = xN.y;
+      pz[u] = xN.z;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) ph[u][f] = a.phi[f][nb[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNbChunk; ++u) {
+      if (k0 + u >= re || nb[u] == P) continue;
+      const double r0 = px[u] - xP.x, r1 = py[u] - xP.y, r2 = pz[u] - xP.z;
+      const double rr = r0 * r0 + r1 * r1 + r2 * r2;
+      const double w = 1.0 / (rr > 1e-30 ? rr : 1e-30);
+      const double bx = w * r0, by = w * r1, bz = w * r2;
+      const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
+      const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
+      const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double dphi = ph[u][f] - phiP[f];
+        sx[f] += cx * dphi;
+        sy[f] += cy * dphi;
+        sz[f] += cz * dphi;
+      }
     }
   }
   const int be = m.cbOff[P + 1];
@@ -269,10 +246,10 @@ __global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) 
     const double cx = inv[0] * bx + inv[1] * by + inv[2] * bz;
     const double cy = inv[3] * bx + inv[4] * by + inv[5] * bz;
     const double cz = inv[6] * bx + inv[7] * by + inv[8] * bz;
-    const bool fixed = a.bcType[b] == 1;
+    const bool fixed = bcType[b] == 1;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const double val = fixed ? a.bcVal[f][b] : phiP[f];
+    for (int f = 0; f < NF; ++f) {
+      const double val = fixed ? a.bval[f][b] : phiP[f];
       const double dphi = val - phiP[f];
       sx[f] += cx * dphi;
       sy[f] += cy * dphi;
@@ -280,7 +257,7 @@ __global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) 
     }
   }
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
+  for (int f = 0; f < NF; ++f) {
     a.g[f][0][P] = sx[f];
     a.g[f][1][P] = sy[f];
     a.g[f][2][P] = sz[f];
@@ -289,7 +266,13 @@ __global__ __launch_bounds__(256) void k_lsq_gsv1_inv3(MeshView m, Grad3Args a) 
 
 void launch_lsq_gradient3(const MeshView& m, const Grad3Args& a, hipStream_t s) {
   if (!m.gsvInv) throw Error(7, "three-field gradient needs the cached GSV1 inverse");
-  k_lsq_gsv1_inv3<<<grid_for(m.nC, 256), 256, 0, s>>>(m, a);
+  GradFields<3> f;
+  for (int c = 0; c < 3; ++c) {
+    f.phi[c] = a.phi[c];
+    f.bval[c] = a.bcVal[c];
+    for (int d = 0; d < 3; ++d) f.g[c][d] = a.g[c][d];
+  }
+  k_lsq_gsv1_inv<3><<<grid_for(m.nC, 256), 256, 0, s>>>(m, a.bcType, f);
   AN_CHECK_LAUNCH();
 }
 
@@ -303,7 +286,8 @@ void launch_lsq_gradient(const MeshView& m, int mode, BCView bc, const double* p
                          double* gy, double* gz, hipStream_t s) {
   const int B = 256;
   if (mode == 0 && m.gsvInv)
-    k_lsq_gsv1_inv<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
+    k_lsq_gsv1_inv<1><<<grid_for(m.nC, B), B, 0, s>>>(m, bc.type,
+                                                       GradFields<1>{{phi}, {bc.val}, {{gx, gy, gz}}});
   else if (mode == 0)
     k_lsq_gsv1<<<grid_for(m.nC, B), B, 0, s>>>(m, bc, phi, gx, gy, gz);
   else
