@@ -323,14 +323,39 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
   store_partials<1>(&bs, blockIdx.x, ra);
 }
 
+// k_finalize_nc<1> for scal[S_RR] that also publishes r.r and p.q to the
+// host (see pcg_update)
+__global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
+                                                       const double* __restrict__ partial,
+                                                       double* __restrict__ scal,
+                                                       double* __restrict__ pub, double stamp) {
+  __shared__ double sh[16];
+  double v[1] = {0.0};
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) v[0] += partial[i];
+  block_sums<1>(v, sh);
+  if (threadIdx.x == 0) {
+    scal[S_RR] = v[0];
+    pub[S_RR] = v[0];
+    pub[S_PQ] = scal[S_PQ];
+    __threadfence_system();
+    pub[S_PUB] = stamp;
+  }
+}
+
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
-                Reducer& red, hipStream_t s, const T* omd, T* y) {
+                Reducer& red, hipStream_t s, const T* omd, T* y, double* pub, double stamp) {
   const int nb = red_blocks(n);
   k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR), omd,
                                         y);
   AN_CHECK_LAUNCH();
-  finalize(red, nb, 1, scal + S_RR, s);
+  if (pub) {
+    k_finalize_pub<<<1, 1024, 0, s>>>(nb, red.partial.p, scal, pub, stamp);
+    AN_CHECK_LAUNCH();
+  } else {
+    finalize(red, nb, 1, scal + S_RR, s);
+  }
 }
 
 template <typename T>
@@ -394,9 +419,10 @@ template void dot2<double>(int, const double*, const double*, const double*, dou
 template void dot2<float>(int, const float*, const float*, const float*, double*, int, Reducer&,
                           hipStream_t);
 template void pcg_update<double>(int, double*, double*, const double*, const double*, double*, int,
-                                 int, Reducer&, hipStream_t, const double*, double*);
+                                 int, Reducer&, hipStream_t, const double*, double*, double*,
+                                 double);
 template void pcg_update<float>(int, float*, float*, const float*, const float*, double*, int, int,
-                                Reducer&, hipStream_t, const float*, float*);
+                                Reducer&, hipStream_t, const float*, float*, double*, double);
 template void pcg_direction<double>(int, double*, const double*, double*, int, int, bool,
                                     hipStream_t, double);
 template void pcg_direction<float>(int, float*, const float*, double*, int, int, bool,

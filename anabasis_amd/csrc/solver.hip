@@ -14,6 +14,7 @@
 // FP64.
 #include "solver.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 
@@ -194,6 +195,21 @@ static inline bool speculate(double rn, double rnPrev, double tol) {
   return !(rnPrev > 0.0 && rn * (rn / rnPrev) <= tol);
 }
 
+// A bounded spin on the stamp; past two seconds (a stalled or profiled
+// device) a stream synchronisation, after which the stamp must be there.
+void PcgSolver::wait_published(double stamp) {
+  volatile const double* h = pub.h;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned i = 1; h[S_PUB] != stamp; ++i) {
+    if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+      AN_HIP(hipStreamSynchronize(s));
+      if (h[S_PUB] != stamp) throw Error(7, "PCG scalars were not published");
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+}
+
 // FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.
 int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut) {
   (void)b;
@@ -216,11 +232,10 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                          e1 ? e1->b : nullptr);
-    // the update also writes the preconditioner's y = omdinv o r
+    // the update also writes the preconditioner's y = omdinv o r and
+    // publishes r.r and p.q to the host
     pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv.p,
-                       amg.lv[0].y.p);
-    AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    AN_HIP(hipEventRecord(ev, s));
+                       amg.lv[0].y.p, pub.d, ++pubStamp);
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
@@ -235,11 +250,11 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();  // queued while the host waits for ||r||
-    AN_HIP(hipEventSynchronize(ev));
+    wait_published(pubStamp);
     rnPrev = rn;
-    rn = std::sqrt(hs.h[S_RR]);
+    rn = std::sqrt(pub.h[S_RR]);
     *rnormOut = rn;
-    if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) return -it;
+    if (!(pub.h[S_PQ] > 0.0) || !std::isfinite(rn)) return -it;
     if (rn <= tol) break;
     if (!spec) next_direction();
   }
@@ -267,9 +282,7 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                        e1 ? e1->b : nullptr);
     pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s,
-                      amg.lv[0].omdinv32.p, amg.lv[0].y32.p);
-    AN_HIP(hipMemcpyAsync(hs.h + S_PQ, scal.p + S_PQ, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    AN_HIP(hipEventRecord(ev, s));
+                      amg.lv[0].omdinv32.p, amg.lv[0].y32.p, pub.d, ++pubStamp);
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
@@ -284,11 +297,11 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();
-    AN_HIP(hipEventSynchronize(ev));
+    wait_published(pubStamp);
     rnPrev = rn;
-    rn = std::sqrt(hs.h[S_RR]);
+    rn = std::sqrt(pub.h[S_RR]);
     *rnormOut = rn;
-    if (!(hs.h[S_PQ] > 0.0) || !std::isfinite(rn)) break;
+    if (!(pub.h[S_PQ] > 0.0) || !std::isfinite(rn)) break;
     if (rn <= tol) break;
     if (!spec) next_direction();
   }

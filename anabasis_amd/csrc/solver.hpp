@@ -10,7 +10,12 @@ namespace an {
 
 struct PinnedScalars {
   double* h = nullptr;
-  PinnedScalars() { AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * S_COUNT)); }
+  double* d = nullptr;  // the same memory as kernels address it
+  PinnedScalars() {
+    AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * S_COUNT));
+    for (int i = 0; i < S_COUNT; ++i) h[i] = 0.0;
+    AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
+  }
   ~PinnedScalars() {
     if (h) (void)hipHostFree(h);
   }
@@ -67,6 +72,15 @@ class PcgSolver {
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
   double read_scalar(int slot);
+  // r.r and p.q of an iteration come back through pcg_update's publication
+  // into pub (stamped): the host waits for the stamp.  pub is a pinned
+  // buffer of its own that only that kernel writes: with the publication in
+  // hs, whose first cache line the copies of read_scalar and solve() also
+  // write, repeated runs were not always bitwise identical (a stale value
+  // was read now and then); with its own buffer they are.
+  PinnedScalars pub;
+  double pubStamp = 0.0;
+  void wait_published(double stamp);
   EvPair* next_ev(std::vector<EvPair>& pool, size_t& used);
   void collect_profile(an_solve_info* inf);
 };

@@ -237,6 +237,26 @@ def test_pcg_graph_replay_matches_eager(meshes, omeshes):
                 assert np.array_equal(xe, xg), (precision, scale)
 
 
+def test_pcg_repeatable(meshes, omeshes):
+    """Two solvers fed the same sequence of solves (projection on, so the
+    host reads scalars through both the per-iteration publication and the
+    per-solve copies) return the same bits and iteration counts."""
+    m, om = meshes["cyl"], omeshes["cyl"]
+    vals = laplacian(om, m, ("patch_2_0",))
+    for precision in (0, 1):
+        kw = dict(relTol=1e-9, absTol=0.0, coarseSize=64, precision=precision, rhsProjection=4)
+        S = [lp.PCGSolver(om.rowOffsets, om.cols, **kw) for _ in range(2)]
+        for P in S:
+            P.set_values(vals, rebuild=True)
+        rng = np.random.default_rng(21)
+        b0 = rng.standard_normal(om.nCells)
+        for k in range(8):
+            b = b0 + 0.1 * k * rng.standard_normal(om.nCells)
+            (x1, i1), (x2, i2) = (P.solve(b.copy()) for P in S)
+            assert i1.converged and i1.iterations == i2.iterations, (precision, k)
+            assert np.array_equal(x1, x2), (precision, k)
+
+
 def test_pcg_rhs_projection(meshes, omeshes):
     """Successive right-hand sides with one matrix: every solve still meets
     the tolerance (true residual) and agrees with the plain solve; a repeated
