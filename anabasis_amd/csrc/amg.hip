@@ -35,10 +35,12 @@ static constexpr int kSpreadRows = 4096;
 static inline int spread_of(int n) { return n <= kSpreadRows ? 8 : 1; }
 
 // Logical block of a launch that needs 'need' blocks: a grid of exactly
-// 8 x need blocks is spread (blocks off XCD 0 return -1, they have no work)
+// 8 x need blocks is spread (blocks off XCD 0 return -1, they have no work);
+// otherwise each XCD takes a contiguous range of blocks (xcd_block), so
+// rows that share cache lines meet in one L2
 __device__ __forceinline__ int spread_block(int need) {
   if ((int)gridDim.x == 8 * need) return (blockIdx.x & 7) ? -1 : (int)(blockIdx.x >> 3);
-  return (int)blockIdx.x;
+  return xcd_block(gridDim.x);
 }
 static constexpr int kGalCap = 256;  // max entries of one coarse row
 
