@@ -85,9 +85,23 @@ __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A
                                              double* __restrict__ partial) {
   __shared__ double sh[3][kW];
   double v[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+  const int S = gridDim.x * kB;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * S) {
+    const int j = (i + S < n) ? i + S : i;  // second element of the pair (or i again)
+    double a[2][3], b[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] += A[3 * i + c] * B[3 * i + c];
+    for (int c = 0; c < 3; ++c) {
+      a[0][c] = A[3 * i + c];
+      b[0][c] = B[3 * i + c];
+      a[1][c] = A[3 * j + c];
+      b[1][c] = B[3 * j + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] += a[0][c] * b[0][c];
+    if (j != i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] += a[1][c] * b[1][c];
+  }
   block_sum3(v, sh);
   store3(v, blockIdx.x, partial);
 }
@@ -200,16 +214,32 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
   double alpha[3], acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < 3; ++c) alpha[c] = sc[3 * rhoNew + c] / sc[3 * S_R0V + c];
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
-    const double di = dinv[i];
+  const int St = gridDim.x * kB;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
+    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
+    const int el[2] = {i, j};
+    double di[2], r[2][3], v[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (!((mask >> c) & 1)) continue;
-      const int e = 3 * i + c;
-      const double si = R[e] - alpha[c] * V[e];
-      S[e] = si;
-      SH[e] = di * si;
-      acc[c] += si * si;
+    for (int u = 0; u < 2; ++u) {
+      di[u] = dinv[el[u]];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[u][c] = R[3 * el[u] + c];
+        v[u][c] = V[3 * el[u] + c];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1 && j == i) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (!((mask >> c) & 1)) continue;
+        const int e = 3 * el[u] + c;
+        const double si = r[u][c] - alpha[c] * v[u][c];
+        S[e] = si;
+        SH[e] = di[u] * si;
+        acc[c] += si * si;
+      }
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -226,13 +256,27 @@ __global__ __launch_bounds__(kB) void k3_dot2(int n, const double* __restrict__ 
                                               double* __restrict__ partial) {
   __shared__ double sh[3][kW];
   double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+  const int St = gridDim.x * kB;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
+    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
+    double t[2][3], sv[2][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double t = T[3 * i + c];
-      a[c] += t * S[3 * i + c];
-      b[c] += t * t;
+      t[0][c] = T[3 * i + c];
+      sv[0][c] = S[3 * i + c];
+      t[1][c] = T[3 * j + c];
+      sv[1][c] = S[3 * j + c];
     }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1 && j == i) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        a[c] += t[u][c] * sv[u][c];
+        b[c] += t[u][c] * t[u][c];
+      }
+    }
+  }
   block_sum3(a, sh);
   block_sum3(b, sh);
   if (threadIdx.x == 0)
@@ -266,21 +310,40 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
     omega[c] = (tt > 0.0) ? sc[3 * S_TS + c] / tt : 0.0;
     convS[c] = sqrt(sc[3 * S_SS + c]) <= tol.t[c];
   }
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+  const int St = gridDim.x * kB;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
+    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
+    const int el[2] = {i, j};
+    double xv[2][3], ph[2][3], shv[2][3], sv[2][3], tv[2][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (!((mask >> c) & 1)) continue;
-      const int e = 3 * i + c;
-      double ri;
-      if (convS[c]) {
-        X[e] += alpha[c] * PH[e];
-        ri = S[e];
-      } else {
-        X[e] += alpha[c] * PH[e] + omega[c] * SH[e];
-        ri = S[e] - omega[c] * T[e];
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int e = 3 * el[u] + c;
+        xv[u][c] = X[e];
+        ph[u][c] = PH[e];
+        shv[u][c] = SH[e];
+        sv[u][c] = S[e];
+        tv[u][c] = T[e];
       }
-      R[e] = ri;
-      acc[c] += ri * ri;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (u == 1 && j == i) break;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (!((mask >> c) & 1)) continue;
+        const int e = 3 * el[u] + c;
+        double ri;
+        if (convS[c]) {
+          X[e] = xv[u][c] + alpha[c] * ph[u][c];
+          ri = sv[u][c];
+        } else {
+          X[e] = xv[u][c] + (alpha[c] * ph[u][c] + omega[c] * shv[u][c]);
+          ri = sv[u][c] - omega[c] * tv[u][c];
+        }
+        R[e] = ri;
+        acc[c] += ri * ri;
+      }
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -373,8 +436,8 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
       const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
       k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, part);
       finalize(w.red, Gd, 3, sc + 3 * rhoNew, s);
-      k3_bicg_p<<<G, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
-                                 it == 1 ? 1 : 0, mask);
+      k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
+                                  it == 1 ? 1 : 0, mask);
       if (evStart && it == 1)
         hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
                               A.rb.count, rbs, (const int*)A.rowOff.p, (const int*)A.cols.p,

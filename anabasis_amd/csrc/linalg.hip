@@ -323,8 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
   store_partials<1>(&bs, blockIdx.x, ra);
 }
 
-// k_finalize_nc<1> for scal[S_RR] that also publishes r.r and p.q to the
-// host (see pcg_update)
+// k_finalize_nc<1> for scal[S_RR] that also publishes p.q and r.r to the
+// host as {p.q, stamp, r.r, stamp} (see pcg_update)
 __global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
                                                        const double* __restrict__ partial,
                                                        double* __restrict__ scal,
@@ -336,10 +336,12 @@ __global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
   block_sums<1>(v, sh);
   if (threadIdx.x == 0) {
     scal[S_RR] = v[0];
-    pub[S_RR] = v[0];
-    pub[S_PQ] = scal[S_PQ];
-    __threadfence_system();
-    pub[S_PUB] = stamp;
+    // each value travels with the stamp in one 16-byte store: two separate
+    // stores to host memory (value, then stamp after a system-scope fence)
+    // were now and then seen in the other order
+    double2* p2 = reinterpret_cast<double2*>(pub);
+    p2[0] = make_double2(scal[S_PQ], stamp);
+    p2[1] = make_double2(v[0], stamp);
   }
 }
 

@@ -116,7 +116,6 @@ enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 
                   S_FRZ = 13, S_FZQ = 14,  // flexible CG: r.z, z.q
                   S_PROJ = 16,             // 16 projection coefficients
                   S_NU = 32,               // ||A w||^2 of a new direction, then ||A x||^2
-                  S_PUB = 34,              // host copy only: stamp of the last publication
                   S_COUNT = 40 };
 
 // Deterministic reduction workspace: per-block partials, summed in index
@@ -167,10 +166,10 @@ void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Red
 
 // x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den];
 // with y: also y = omd o r (the preconditioner's first Jacobi sweep).
-// With pub (pinned host memory, mapped): the kernel that finishes r.r also
-// stores pub[S_RR] = r.r and pub[S_PQ] = scal[S_PQ], then, after a
-// system-scope fence, pub[S_PUB] = stamp; the host waits for the stamp
-// instead of a copy and an event.
+// With pub (pinned host memory, mapped, 16-byte aligned): the kernel that
+// finishes r.r also stores pub[0..3] = {p.q, stamp, r.r, stamp}, each value
+// and its stamp in one 16-byte store; the host waits until both stamps are
+// the expected one instead of a copy and an event.
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s, const T* omd = nullptr, T* y = nullptr,

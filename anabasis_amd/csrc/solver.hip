@@ -197,17 +197,20 @@ static inline bool speculate(double rn, double rnPrev, double tol) {
 
 // A bounded spin on the stamp; past two seconds (a stalled or profiled
 // device) a stream synchronisation, after which the stamp must be there.
-void PcgSolver::wait_published(double stamp) {
+void PcgSolver::wait_published(double stamp, double* pq, double* rr) {
   volatile const double* h = pub.h;
+  auto ready = [&]() { return h[1] == stamp && h[3] == stamp; };
   const auto t0 = std::chrono::steady_clock::now();
-  for (unsigned i = 1; h[S_PUB] != stamp; ++i) {
+  for (unsigned i = 1; !ready(); ++i) {
     if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
       AN_HIP(hipStreamSynchronize(s));
-      if (h[S_PUB] != stamp) throw Error(7, "PCG scalars were not published");
+      if (!ready()) throw Error(7, "PCG scalars were not published");
       break;
     }
   }
   std::atomic_thread_fence(std::memory_order_acquire);
+  *pq = h[0];
+  *rr = h[2];
 }
 
 // FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.
@@ -250,11 +253,12 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();  // queued while the host waits for ||r||
-    wait_published(pubStamp);
+    double pq = 0.0, rr = 0.0;
+    wait_published(pubStamp, &pq, &rr);
     rnPrev = rn;
-    rn = std::sqrt(pub.h[S_RR]);
+    rn = std::sqrt(rr);
     *rnormOut = rn;
-    if (!(pub.h[S_PQ] > 0.0) || !std::isfinite(rn)) return -it;
+    if (!(pq > 0.0) || !std::isfinite(rn)) return -it;
     if (rn <= tol) break;
     if (!spec) next_direction();
   }
@@ -297,11 +301,12 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();
-    wait_published(pubStamp);
+    double pq = 0.0, rr = 0.0;
+    wait_published(pubStamp, &pq, &rr);
     rnPrev = rn;
-    rn = std::sqrt(pub.h[S_RR]);
+    rn = std::sqrt(rr);
     *rnormOut = rn;
-    if (!(pub.h[S_PQ] > 0.0) || !std::isfinite(rn)) break;
+    if (!(pq > 0.0) || !std::isfinite(rn)) break;
     if (rn <= tol) break;
     if (!spec) next_direction();
   }

@@ -73,14 +73,11 @@ class PcgSolver {
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
   double read_scalar(int slot);
   // r.r and p.q of an iteration come back through pcg_update's publication
-  // into pub (stamped): the host waits for the stamp.  pub is a pinned
-  // buffer of its own that only that kernel writes: with the publication in
-  // hs, whose first cache line the copies of read_scalar and solve() also
-  // write, repeated runs were not always bitwise identical (a stale value
-  // was read now and then); with its own buffer they are.
+  // into pub ({p.q, stamp, r.r, stamp}): the host waits for both stamps.
+  // pub is a pinned buffer of its own that only that kernel writes.
   PinnedScalars pub;
   double pubStamp = 0.0;
-  void wait_published(double stamp);
+  void wait_published(double stamp, double* pq, double* rr);
   EvPair* next_ev(std::vector<EvPair>& pool, size_t& used);
   void collect_profile(an_solve_info* inf);
 };
