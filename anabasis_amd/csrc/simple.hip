@@ -203,32 +203,52 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
   double r[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) r[q] = ((a.active >> q) & 1) ? -a.gpc[q][c] * V : 0.0;
-  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
-    const int e = m.cfEnt[k];
-    const int f = e >> 1, side = e & 1;
-    const int slotInfo = m.cfSlot[k];
-    if (m.fDpn[f] <= 1e-14) {  // main.cu:1826 -- the face contributes nothing
-      if (FULL && (slotInfo & 1)) vals[slotInfo >> 1] = 0.0;
-      continue;
+  // faces four at a time: incidence entries, then the face values, then the
+  // terms in incidence order (as one face at a time)
+  constexpr int U = 4;
+  const int k1 = m.cfOff[c + 1];
+  for (int k0 = m.cfOff[c]; k0 < k1; k0 += U) {
+    int fu[U], side[U], su[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = m.cfEnt[min(k0 + u, k1 - 1)];
+      fu[u] = e >> 1;
+      side[u] = e & 1;
+      su[u] = m.cfSlot[min(k0 + u, k1 - 1)];
     }
-    if (FULL) {
-      const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
-      if (side == 0) {
-        diag += coef[f];
-        vals[slotInfo >> 1] = old + coef[n + f];
-      } else {
-        diag += coef[2 * n + f];
-        vals[slotInfo >> 1] = old + coef[3 * n + f];
+    double nd[U], cd[U], co[U], cr[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int f = fu[u];
+      nd[u] = m.fDpn[f];
+      if (FULL) {
+        cd[u] = coef[(size_t)(2 * side[u]) * n + f];      // diagonal share
+        co[u] = coef[(size_t)(2 * side[u] + 1) * n + f];  // off-diagonal entry
       }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cr[u][q] = ((a.active >> q) & 1) ? corr[(size_t)q * n + f] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (!((a.active >> q) & 1)) continue;
-      const double cr = corr[(size_t)q * n + f];
-      if (side == 0)
-        r[q] += cr;
-      else
-        r[q] += (-cr);
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u >= k1) break;
+      const int slotInfo = su[u];
+      if (nd[u] <= 1e-14) {  // main.cu:1826 -- the face contributes nothing
+        if (FULL && (slotInfo & 1)) vals[slotInfo >> 1] = 0.0;
+        continue;
+      }
+      if (FULL) {
+        const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+        diag += cd[u];
+        vals[slotInfo >> 1] = old + co[u];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        if (!((a.active >> q) & 1)) continue;
+        if (side[u] == 0)
+          r[q] += cr[u][q];
+        else
+          r[q] += (-cr[u][q]);
+      }
     }
   }
   for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
@@ -297,21 +317,47 @@ __global__ __launch_bounds__(kB) void k_p_asm(MeshView m, const int* __restrict_
   const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= m.nC) return;
   double diag = 0.0;
-  for (int k = m.cfOff[c]; k < m.cfOff[c + 1]; ++k) {
-    const int f = m.cfEnt[k] >> 1;
-    const int slotInfo = m.cfSlot[k];
-    const double dpn = m.fDpn[f];
-    if (dpn <= 1e-14) {
-      if (slotInfo & 1) vals[slotInfo >> 1] = 0.0;
-      continue;
+  // faces four at a time: incidence entries, then their face data, then the
+  // rAU gathers, then the terms in incidence order (as one face at a time)
+  constexpr int U = 4;
+  const int k1 = m.cfOff[c + 1];
+  for (int k0 = m.cfOff[c]; k0 < k1; k0 += U) {
+    int fu[U], su[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = min(k0 + u, k1 - 1);
+      fu[u] = m.cfEnt[k] >> 1;
+      su[u] = m.cfSlot[k];
     }
-    const double lam = m.fLam[f];
-    const int2 pn = m.fPN[f];
-    const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
-    const double coeff = m.fA[f] * rAUf / dpn;
-    diag += coeff;
-    const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
-    vals[slotInfo >> 1] = old + (-coeff);
+    double dpn[U], lam[U], area[U];
+    int2 pn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      dpn[u] = m.fDpn[fu[u]];
+      lam[u] = m.fLam[fu[u]];
+      pn[u] = m.fPN[fu[u]];
+      area[u] = m.fA[fu[u]];
+    }
+    double rP[U], rN[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      rP[u] = rAU[pn[u].x];
+      rN[u] = rAU[pn[u].y];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (k0 + u >= k1) break;
+      const int slotInfo = su[u];
+      if (dpn[u] <= 1e-14) {
+        if (slotInfo & 1) vals[slotInfo >> 1] = 0.0;
+        continue;
+      }
+      const double rAUf = (1.0 - lam[u]) * rP[u] + lam[u] * rN[u];
+      const double coeff = area[u] * rAUf / dpn[u];
+      diag += coeff;
+      const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+      vals[slotInfo >> 1] = old + (-coeff);
+    }
   }
   for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
     const int b = m.cbFace[j];
