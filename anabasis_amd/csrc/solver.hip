@@ -380,7 +380,11 @@ __global__ __launch_bounds__(kB) void k_pscale(int n, const double* __restrict__
 // c = B^T r into scal[S_PROJ ..]
 void PcgSolver::proj_coef(const double* rr) {
   const int n = A.n;
-  const int nb = std::min(grid_for(n, kB), 1024);
+  // at most 1024 blocks (more made the 16-component block sums and the
+  // finalize cost more than the loads they hide), and no more than the
+  // partial buffer (4 cap doubles) holds for projM components
+  const int nb =
+      std::min({grid_for(n, kB), 1024, 4 * red.cap / std::max(projM, 1)});
   k_mdot<<<nb, kB, 0, s>>>(n, projM, projB.p, rr, red.partial.p);
   AN_CHECK_LAUNCH();
   finalize(red, nb, projM, scal.p + S_PROJ, s);

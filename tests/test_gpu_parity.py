@@ -257,6 +257,24 @@ def test_pcg_repeatable(meshes, omeshes):
             assert np.array_equal(x1, x2), (precision, k)
 
 
+def test_pcg_rhs_projection_many_vectors(gpu):
+    """The most stored solutions (16) on a 110,592-row matrix: the projection's
+    block partials (blocks x 16 components) stay inside the reducer, and every
+    solve of a sequence still meets its tolerance on the true residual."""
+    m = lp.Mesh.hexbox(48, 48, 48, jitter=0.2, seed=5)
+    om = O.OMesh.from_product(m)
+    vals = laplacian(om, m, ("xmin",))
+    A = sp.csr_matrix((vals, om.cols, om.rowOffsets))
+    P = lp.PCGSolver(om.rowOffsets, om.cols, relTol=0.0, absTol=1e-9, rhsProjection=16)
+    P.set_values(vals)
+    rng = np.random.default_rng(31)
+    b0 = rng.standard_normal(om.nCells)
+    for k in range(18):
+        b = b0 + 0.05 * rng.standard_normal(om.nCells)
+        x, info = P.solve(b)
+        assert info.converged and np.linalg.norm(b - A @ x) <= 1.5e-9, k
+
+
 def test_pcg_rhs_projection(meshes, omeshes):
     """Successive right-hand sides with one matrix: every solve still meets
     the tolerance (true residual) and agrees with the plain solve; a repeated
