@@ -797,7 +797,7 @@ SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int*
   po.precision = par.precision;
   psolver.reset(new PcgSolver(nC, M.addr.rowOff.p, M.addr.cols.p, M.addr.nnz, po, true, s));
   red.init(1024, s);
-  scal.alloc(8);
+  scal.alloc(3 * kStatSlots);
   for (auto& e : ev) AN_HIP(hipEventCreate(&e));
   AN_HIP(hipStreamSynchronize(s));
 }
@@ -827,15 +827,23 @@ double SimpleSolver::relchg(const double* a, const double* b) {
   return std::sqrt(hs.h[0] / (1.0 + hs.h[1]));
 }
 
-double SimpleSolver::maxabs(const double* a) {
+void SimpleSolver::queue_stats(const double* a, const double* b, int k) {
+  if (k < 0 || k >= kStatSlots) throw Error(7, "statistics slot out of range");
   const int G = std::min(grid_for(M.nC, kB), 340);
-  k_stats<<<G, kB, 0, s>>>(M.nC, a, nullptr, red.partial.p);
-  k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p);
+  k_stats<<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p + 3 * k);
   AN_CHECK_LAUNCH();
-  AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
-  AN_HIP(hipStreamSynchronize(s));
-  return hs.h[2];
 }
+
+void SimpleSolver::fetch_stats(int nk) {
+  AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * (size_t)nk * sizeof(double), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+}
+
+double SimpleSolver::stat_relchg(int k) const {
+  return std::sqrt(hs.h[3 * k] / (1.0 + hs.h[3 * k + 1]));
+}
+double SimpleSolver::stat_maxabs(int k) const { return hs.h[3 * k + 2]; }
 
 // solve_scalar_component_with_nonorth (main.cu:3350-3384)
 MomArgs SimpleSolver::mom_args(int comp, const DevBuf<double>* grad) const {
@@ -969,12 +977,19 @@ void SimpleSolver::momentum_fused(an_simple_stats* st) {
     AN_HIP(hipStreamSynchronize(s));
     tAcc[0] += since(0, 1);
     tAcc[1] += since(1, 2);
+    // the components' relative changes (before q takes the new iterates),
+    // then one synchronisation
+    for (int c = 0; c < 3; ++c)
+      if ((active >> c) & 1) queue_stats(qNew3[c].p, q[c].p, c);
+    for (int c = 0; c < 3; ++c)
+      if ((active >> c) & 1)
+        AN_HIP(hipMemcpyAsync(q[c].p, qNew3[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+    fetch_stats(3);
+    const int was = active;
     for (int c = 0; c < 3; ++c) {
-      if (!((active >> c) & 1)) continue;
+      if (!((was >> c) & 1)) continue;
       st->bicgstabIts[c] = info[c].iterations;
-      const double rc = relchg(qNew3[c].p, q[c].p);
-      AN_HIP(hipMemcpyAsync(q[c].p, qNew3[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
-      if (rc < par.corrTol) active &= ~(1 << c);
+      if (stat_relchg(c) < par.corrTol) active &= ~(1 << c);
     }
   }
 }
@@ -1048,17 +1063,23 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
       }
     }
     tAcc[3] += tP;
-    st.massRes = maxabs(divCorr.p);
+    queue_stats(divCorr.p, nullptr, 0);
     k_p_update<<<grid_for(nC, kB), kB, 0, s>>>(nC, p.p, pCorr.p, par.pRelax);
     // g still holds grad(pCorr) of the last pass (pCorr has not changed
     // since), which is what the velocity correction takes
     k_vel_correct<<<grid_for(nC, kB), kB, 0, s>>>(nC, q[0].p, q[1].p, q[2].p, rAU.p, g[0].p,
                                                   g[1].p, g[2].p);
     AN_CHECK_LAUNCH();
-    st.duRel = relchg(q[0].p, qOld[0].p);
-    st.dvRel = relchg(q[1].p, qOld[1].p);
-    st.dwRel = relchg(q[2].p, qOld[2].p);
-    st.dpRel = relchg(p.p, pOld.p);
+    queue_stats(q[0].p, qOld[0].p, 1);
+    queue_stats(q[1].p, qOld[1].p, 2);
+    queue_stats(q[2].p, qOld[2].p, 3);
+    queue_stats(p.p, pOld.p, 4);
+    fetch_stats(5);
+    st.massRes = stat_maxabs(0);
+    st.duRel = stat_relchg(1);
+    st.dvRel = stat_relchg(2);
+    st.dwRel = stat_relchg(3);
+    st.dpRel = stat_relchg(4);
     st.pcgLast = pcgLast;
     st.pcgTotal = pcgTotal;
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tw0).count();

@@ -70,7 +70,14 @@ class SimpleSolver {
   void momentum_fused(an_simple_stats* st);
   MomArgs mom_args(int comp, const DevBuf<double>* grad) const;
   double relchg(const double* a, const double* b);  // device reduction
-  double maxabs(const double* a);
+  // several reductions with one copy and one synchronisation: queue_stats
+  // reduces a (and a - b) into slot k (k < kStatSlots); fetch_stats brings
+  // slots 0..nk-1 to the host, read with stat_relchg / stat_maxabs
+  static constexpr int kStatSlots = 5;
+  void queue_stats(const double* a, const double* b, int k);
+  void fetch_stats(int nk);
+  double stat_relchg(int k) const;
+  double stat_maxabs(int k) const;
   void mark(int i) { AN_HIP(hipEventRecord(ev[i], s)); }
   double since(int a, int b);
 };
