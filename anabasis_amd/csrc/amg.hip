@@ -190,11 +190,15 @@ __global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __rest
 // ---------------------------------------------------------- Galerkin ---
 // Aggregate sum C = P^T A P, one 64-lane wave per coarse row I.  The fine
 // entries of I's members are visited in member order and CSR order within
-// a row, 64 at a time; the distinct coarse columns are kept in an LDS list
-// in first-seen order, and a column's value is its first entry plus the
-// later ones in visiting order (the serial definition's order, so the sums
-// are bit-identical).  FILL 0 counts the distinct columns (more than
-// kGalCap: overflow); FILL 1 writes them sorted by column.
+// a row; the members' rows are laid end to end (offsets from a wave scan of
+// their lengths), so the wave takes 64 entries at a time across members
+// (a level-0 row has ~7 entries: one member per step left most lanes idle
+// and paid the dependent loads once per member).  The distinct coarse
+// columns are kept in an LDS list in first-seen order, and a column's value
+// is its first entry plus the later ones in visiting order, whatever the
+// chunking (the serial definition's order, so the sums are bit-identical).
+// FILL 0 counts the distinct columns (more than kGalCap: overflow); FILL 1
+// writes them sorted by column.
 static constexpr int kGalWaves = 4;
 template <bool FILL>
 __global__ __launch_bounds__(64 * kGalWaves) void k_gal_wave(
@@ -206,6 +210,7 @@ __global__ __launch_bounds__(64 * kGalWaves) void k_gal_wave(
   __shared__ double sV[kGalWaves][kGalCap];
   __shared__ int eJ[kGalWaves][64], eT[kGalWaves][64];
   __shared__ double eV[kGalWaves][64];
+  __shared__ int mS[kGalWaves][64], mO[kGalWaves][64], mL[kGalWaves][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int I = blockIdx.x * kGalWaves + w;
   if (I >= nc) return;  // the whole wave: no block barriers below
@@ -214,15 +219,35 @@ __global__ __launch_bounds__(64 * kGalWaves) void k_gal_wave(
   int nd = 0;
   bool over = false;
   const int m1 = moff[I + 1];
-  for (int m = moff[I]; m < m1 && !over; ++m) {
-    const int i = mem[m];
-    const int ke = ro[i + 1];
-    for (int k0 = ro[i]; k0 < ke; k0 += 64) {
-      const int k = k0 + lane;
-      const bool has = k < ke;
+  for (int mb = moff[I]; mb < m1 && !over; mb += 64) {  // members, 64 at a time
+    const int nm = min(64, m1 - mb);
+    int rs = 0, rl = 0;
+    if (lane < nm) {
+      const int i = mem[mb + lane];
+      rs = ro[i];
+      rl = ro[i + 1] - rs;
+    }
+    int inc = rl;  // inclusive scan of the row lengths over the lanes
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    const int total = __shfl(inc, 63, 64);
+    mS[w][lane] = rs;
+    mO[w][lane] = inc - rl;
+    mL[w][lane] = rl;
+    wave_sync();
+    for (int k0 = 0; k0 < total; k0 += 64) {
+      const int e = k0 + lane;  // position in the members' entries laid end to end
+      const bool has = e < total;
+      const int ke = total;     // chunk bound for the lead-lane sums below
       int J = -1;
       double v = 0.0;
       if (has) {
+        int q = 0;
+        for (int p = 0; p < nm; ++p)
+          if (mO[w][p] <= e && e < mO[w][p] + mL[w][p]) q = p;
+        const int k = mS[w][q] + (e - mO[w][q]);
         J = agg[ci[k]];
         if (FILL) v = va[k];
       }
