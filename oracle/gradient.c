@@ -74,6 +74,7 @@ void or_lsq_gradient_gsv1(int nCells, int nFaces, int nInternalFaces,
                           const double *phi, double *grad) {
   int *bo, *bl;
   cell_bfaces(nCells, nFaces, nInternalFaces, owner, &bo, &bl);
+#pragma omp parallel for schedule(static)
   for (int P = 0; P < nCells; ++P) {
     const double *xP = cc + 3 * (size_t)P;
     double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -149,6 +150,7 @@ void or_lsq_gradient_lp(int nCells, int nFaces, int nInternalFaces,
                         double *grad) {
   int *bo, *bl;
   cell_bfaces(nCells, nFaces, nInternalFaces, owner, &bo, &bl);
+#pragma omp parallel for schedule(static)
   for (int P = 0; P < nCells; ++P) {
     const double *xP = cc + 3 * (size_t)P;
     const double phiP = phi[P];

@@ -265,7 +265,11 @@ int or_bicgstab_jacobi(int n, const int *rowOffsets, const int *cols,
                        int maxIter, double relTol, double absTol,
                        double *finalRelRes);
 
+/* threads the oracle's parallel loops use */
+int or_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -29,6 +29,11 @@ def lib():
     return _lib
 
 
+def num_threads() -> int:
+    """Threads of the oracle's parallel loops (OMP_NUM_THREADS or all cores)."""
+    return int(lib().or_num_threads())
+
+
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 

@@ -24,6 +24,8 @@
 
 #include "oracle.h"
 
+#include <omp.h>
+
 typedef struct {
   double rho, mu;
   int nVelNonOrthCorr, nNonOrthCorr;
@@ -233,3 +235,6 @@ int or_simple_run(int nCells, int nFaces, int nIF, const int *owner,
   free(divNo);
   return diverged ? -done : done;
 }
+
+/* threads the parallel loops use (OMP_NUM_THREADS or all cores) */
+int or_num_threads(void) { return omp_get_max_threads(); }

@@ -36,6 +36,7 @@ void or_rhiechow_flux(int nFaces, int nInternalFaces, const int *owner,
                       const double *uFaceBC, const int *bcVType,
                       const double *vFaceBC, const int *bcWType,
                       const double *wFaceBC, double rho, double *phiStar) {
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < nFaces; ++f) {
     const int P = owner[f];
     if (f < nInternalFaces) {
@@ -143,6 +144,7 @@ void or_correct_flux(int nFaces, int nInternalFaces, const int *owner,
                      const double *pFaceBC, const double *phiStar,
                      const double *pCorr, const double *phiNonOrth,
                      double *phi) {
+#pragma omp parallel for schedule(static)
   for (int f = 0; f < nFaces; ++f) {
     if (f < nInternalFaces) {
       int P = owner[f], N = neigh[f];

@@ -45,8 +45,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Loops whose iterations write disjoint outputs from inputs the loop does not
+ * modify run on all threads (static schedule); every reduction, scan and
+ * scatter stays serial, so the results are the same bits on any thread count. */
+#define OR_PFOR _Pragma("omp parallel for schedule(static)")
+
 void or_spmv(int n, const int *ro, const int *ci, const double *va,
              const double *x, double *y) {
+  OR_PFOR
   for (int i = 0; i < n; ++i) {
     double s = 0.0;
     for (int k = ro[i]; k < ro[i + 1]; ++k) s += va[k] * x[ci[k]];
@@ -96,6 +102,7 @@ static int pair_match(int n, const int *ro, const int *ci, const double *va,
   int *match = (int *)malloc(sizeof(int) * (size_t)n);
   int *cand = (int *)malloc(sizeof(int) * (size_t)n);
   double *smax = (double *)malloc(sizeof(double) * (size_t)n);
+  OR_PFOR
   for (int i = 0; i < n; ++i) {
     match[i] = -1;
     double m = 0.0;
@@ -107,6 +114,7 @@ static int pair_match(int n, const int *ro, const int *ci, const double *va,
     smax[i] = m;
   }
   for (int r = 0; r < OR_MATCH_ROUNDS; ++r) {
+    OR_PFOR
     for (int i = 0; i < n; ++i) {
       cand[i] = -1;
       if (match[i] >= 0) continue;
@@ -128,6 +136,7 @@ static int pair_match(int n, const int *ro, const int *ci, const double *va,
       }
       cand[i] = best;
     }
+    OR_PFOR
     for (int i = 0; i < n; ++i) {
       int j = cand[i];
       if (j >= 0 && cand[j] == i) match[i] = j;
@@ -135,6 +144,7 @@ static int pair_match(int n, const int *ro, const int *ci, const double *va,
   }
   /* an unmatched row joins the pair of its strongest matched strong
    * neighbour (ties by edge hash); otherwise it stays a singleton */
+  OR_PFOR
   for (int i = 0; i < n; ++i) {
     cand[i] = -1;
     if (match[i] >= 0) continue;
@@ -190,81 +200,86 @@ static void build_members(int n, int nc, const int *agg, int **moff_out,
   *mem_out = mem;
 }
 
-/* coarse = P^T A P for piecewise-constant P (aggregate sum) */
+/* coarse = P^T A P for piecewise-constant P (aggregate sum); the coarse
+ * rows are independent (per-thread scratch), the row offsets a serial scan */
+static int gal_row(const int *ro, const int *ci, const double *va, const int *agg,
+                   const int *moff, const int *mem, int I, int fill, int **kJp,
+                   double **kVp, int *capp) {
+  int cnt = 0;
+  for (int m = moff[I]; m < moff[I + 1]; ++m) {
+    int i = mem[m];
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      int J = agg[ci[k]];
+      int t = 0;
+      for (; t < cnt; ++t)
+        if ((*kJp)[t] == J) break;
+      if (t == cnt) {
+        if (cnt == *capp) {
+          *capp *= 2;
+          *kJp = (int *)realloc(*kJp, sizeof(int) * (size_t)*capp);
+          *kVp = (double *)realloc(*kVp, sizeof(double) * (size_t)*capp);
+        }
+        (*kJp)[cnt] = J;
+        if (fill) (*kVp)[cnt] = va[k];
+        ++cnt;
+      } else if (fill) {
+        (*kVp)[t] += va[k];
+      }
+    }
+  }
+  return cnt;
+}
+
 static void galerkin(const int *ro, const int *ci, const double *va,
                      int nc, const int *agg, const int *moff, const int *mem,
                      int **cro_out, int **cci_out, double **cva_out) {
   int *cro = (int *)malloc(sizeof(int) * ((size_t)nc + 1));
-  int cap = 64;
-  int *kJ = (int *)malloc(sizeof(int) * (size_t)cap);
-  double *kV = (double *)malloc(sizeof(double) * (size_t)cap);
   /* pass 1: count */
   cro[0] = 0;
-  for (int I = 0; I < nc; ++I) {
-    int cnt = 0;
-    for (int m = moff[I]; m < moff[I + 1]; ++m) {
-      int i = mem[m];
-      for (int k = ro[i]; k < ro[i + 1]; ++k) {
-        int J = agg[ci[k]];
-        int found = 0;
-        for (int t = 0; t < cnt; ++t)
-          if (kJ[t] == J) {
-            found = 1;
-            break;
-          }
-        if (!found) {
-          if (cnt == cap) {
-            cap *= 2;
-            kJ = (int *)realloc(kJ, sizeof(int) * (size_t)cap);
-            kV = (double *)realloc(kV, sizeof(double) * (size_t)cap);
-          }
-          kJ[cnt++] = J;
-        }
-      }
-    }
-    cro[I + 1] = cro[I] + cnt;
+#pragma omp parallel
+  {
+    int cap = 64;
+    int *kJ = (int *)malloc(sizeof(int) * (size_t)cap);
+    double *kV = (double *)malloc(sizeof(double) * (size_t)cap);
+#pragma omp for schedule(static)
+    for (int I = 0; I < nc; ++I)
+      cro[I + 1] = gal_row(ro, ci, va, agg, moff, mem, I, 0, &kJ, &kV, &cap);
+    free(kJ);
+    free(kV);
   }
+  for (int I = 0; I < nc; ++I) cro[I + 1] += cro[I];
   int *cci = (int *)malloc(sizeof(int) * ((size_t)cro[nc] + 1));
   double *cva = (double *)malloc(sizeof(double) * ((size_t)cro[nc] + 1));
   /* pass 2: accumulate in member/CSR order, then sort by column */
-  for (int I = 0; I < nc; ++I) {
-    int cnt = 0;
-    for (int m = moff[I]; m < moff[I + 1]; ++m) {
-      int i = mem[m];
-      for (int k = ro[i]; k < ro[i + 1]; ++k) {
-        int J = agg[ci[k]];
-        int t = 0;
-        for (; t < cnt; ++t)
-          if (kJ[t] == J) break;
-        if (t == cnt) {
-          kJ[cnt] = J;
-          kV[cnt] = va[k];
-          ++cnt;
-        } else {
-          kV[t] += va[k];
+#pragma omp parallel
+  {
+    int cap = 64;
+    int *kJ = (int *)malloc(sizeof(int) * (size_t)cap);
+    double *kV = (double *)malloc(sizeof(double) * (size_t)cap);
+#pragma omp for schedule(static)
+    for (int I = 0; I < nc; ++I) {
+      const int cnt = gal_row(ro, ci, va, agg, moff, mem, I, 1, &kJ, &kV, &cap);
+      /* insertion sort by column */
+      for (int a = 1; a < cnt; ++a) {
+        int j = kJ[a];
+        double v = kV[a];
+        int b = a - 1;
+        while (b >= 0 && kJ[b] > j) {
+          kJ[b + 1] = kJ[b];
+          kV[b + 1] = kV[b];
+          --b;
         }
+        kJ[b + 1] = j;
+        kV[b + 1] = v;
+      }
+      for (int t = 0; t < cnt; ++t) {
+        cci[cro[I] + t] = kJ[t];
+        cva[cro[I] + t] = kV[t];
       }
     }
-    /* insertion sort by column */
-    for (int a = 1; a < cnt; ++a) {
-      int j = kJ[a];
-      double v = kV[a];
-      int b = a - 1;
-      while (b >= 0 && kJ[b] > j) {
-        kJ[b + 1] = kJ[b];
-        kV[b + 1] = kV[b];
-        --b;
-      }
-      kJ[b + 1] = j;
-      kV[b + 1] = v;
-    }
-    for (int t = 0; t < cnt; ++t) {
-      cci[cro[I] + t] = kJ[t];
-      cva[cro[I] + t] = kV[t];
-    }
+    free(kJ);
+    free(kV);
   }
-  free(kJ);
-  free(kV);
   *cro_out = cro;
   *cci_out = cci;
   *cva_out = cva;
@@ -406,6 +421,7 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
          this level (the GPU aggregate-sum kernel's row capacity) */
       for (int I = 0; I < nc1 && !wide; ++I)
         if (nro[I + 1] - nro[I] > OR_GAL_CAP) wide = 1;
+      OR_PFOR
       for (int i = 0; i < L->n; ++i) agg[i] = a1[agg[i]];
       free(a1);
       if (owned) {
@@ -466,6 +482,7 @@ static int amg_flexible(const or_amg *H) {
 /* one damped Jacobi sweep x += omega D^-1 (b - A x) (Jacobi: uses old x) */
 static void jacobi_sweep(const or_level *L, const double *b, double *x,
                          double *tmp) {
+  OR_PFOR
   for (int i = 0; i < L->n; ++i) {
     double s = 0.0;
     for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) s += L->va[k] * x[L->ci[k]];
@@ -493,6 +510,7 @@ static void kcorrect(or_amg *H, int c, double *b, double *x) {
   or_spmv(n, L->ro, L->ci, L->va, c1, v1);
   const double rho1 = dotv(n, c1, v1), g1 = dotv(n, c1, b);
   const double a1 = rho1 > 0.0 ? g1 / rho1 : 0.0;
+  OR_PFOR
   for (int i = 0; i < n; ++i) b[i] = b[i] - a1 * v1[i];
   vcycle(H, c, b, x);
   or_spmv(n, L->ro, L->ci, L->va, x, w);
@@ -501,6 +519,7 @@ static void kcorrect(or_amg *H, int c, double *b, double *x) {
   const double beta = d - g * gr;
   const double a2 = beta > 0.0 ? e / beta : 0.0;
   const double al1 = a1 - a2 * gr;
+  OR_PFOR
   for (int i = 0; i < n; ++i) x[i] = al1 * c1[i] + a2 * x[i];
 }
 
@@ -513,6 +532,7 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
          chunk's partial sum in column order, then the chunk partials in
          chunk order (the blocked order the GPU coarse solve uses) */
       const int ch = L->n <= 1024 ? 16 : 64;
+      OR_PFOR
       for (int i = 0; i < L->n; ++i) {
         double s = 0.0;
         const double *row = L->coarseInv + (size_t)i * L->n;
@@ -531,15 +551,18 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
     return;
   }
   /* pre-smoothing from zero: first sweep is x = omega D^-1 b */
+  OR_PFOR
   for (int i = 0; i < L->n; ++i) x[i] = L->omdinv[i] * b[i];
   for (int s = 1; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
   /* residual and restriction (member order) */
+  OR_PFOR
   for (int i = 0; i < L->n; ++i) {
     double s = 0.0;
     for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) s += L->va[k] * x[L->ci[k]];
     r[i] = b[i] - s;
   }
   double *bc = H->bb[l + 1];
+  OR_PFOR
   for (int I = 0; I < L->nc; ++I) {
     double s = 0.0;
     for (int m = L->moff[I]; m < L->moff[I + 1]; ++m) s += r[L->mem[m]];
@@ -550,6 +573,7 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
     kcorrect(H, l + 1, bc, xc);
   else
     vcycle(H, l + 1, bc, xc);
+  OR_PFOR
   for (int i = 0; i < L->n; ++i) x[i] = x[i] + xc[L->agg[i]];
   for (int s = 0; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
 }
@@ -594,6 +618,7 @@ int or_pcg_amg(int n, const int *ro, const int *ci, const double *va,
     goto done;
   }
   or_spmv(n, ro, ci, va, x, q);
+  OR_PFOR
   for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
   rnorm = sqrt(dotv(n, r, r));
   tol = fmax(opt->relTol * bnorm, opt->absTol);
@@ -611,6 +636,7 @@ int or_pcg_amg(int n, const int *ro, const int *ci, const double *va,
     double pq = dotv(n, p, q);
     if (!(pq > 0.0)) break;
     double alpha = rho / pq;
+    OR_PFOR
     for (int i = 0; i < n; ++i) {
       x[i] += alpha * p[i];
       r[i] -= alpha * q[i];
@@ -627,6 +653,7 @@ int or_pcg_amg(int n, const int *ro, const int *ci, const double *va,
        fixed preconditioner: -(z.(r_new - r_old))/rho with r_new - r_old =
        -alpha q, alpha = rho/pq */
     double beta = flex ? -dotv(n, z, q) / pq : rhoNew / rho;
+    OR_PFOR
     for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     rho = rhoNew;
   }
@@ -646,6 +673,7 @@ int or_bicgstab_jacobi(int n, const int *ro, const int *ci, const double *va,
                        const double *b, double *x, int maxIter, double relTol,
                        double absTol, double *finalRelRes) {
   double *dinv = (double *)malloc(sizeof(double) * (size_t)n);
+  OR_PFOR
   for (int i = 0; i < n; ++i) {
     double d = 0.0;
     for (int k = ro[i]; k < ro[i + 1]; ++k)
@@ -671,6 +699,7 @@ int or_bicgstab_jacobi(int n, const int *ro, const int *ci, const double *va,
     goto done;
   }
   or_spmv(n, ro, ci, va, x, t);
+  OR_PFOR
   for (int i = 0; i < n; ++i) {
     r[i] = b[i] - t[i];
     r0[i] = r[i];
@@ -685,24 +714,30 @@ int or_bicgstab_jacobi(int n, const int *ro, const int *ci, const double *va,
     double rhoNew = dotv(n, r0, r);
     if (rhoNew == 0.0) break;
     double beta = (rhoNew / rho) * (alpha / omega);
+    OR_PFOR
     for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    OR_PFOR
     for (int i = 0; i < n; ++i) ph[i] = dinv[i] * p[i];
     or_spmv(n, ro, ci, va, ph, v);
     double r0v = dotv(n, r0, v);
     if (r0v == 0.0) break;
     alpha = rhoNew / r0v;
+    OR_PFOR
     for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
     double snorm = sqrt(dotv(n, s, s));
     if (snorm <= tol) {
+      OR_PFOR
       for (int i = 0; i < n; ++i) x[i] += alpha * ph[i];
       rnorm = snorm;
       conv = 1;
       break;
     }
+    OR_PFOR
     for (int i = 0; i < n; ++i) sh[i] = dinv[i] * s[i];
     or_spmv(n, ro, ci, va, sh, t);
     double tt = dotv(n, t, t);
     omega = (tt > 0.0) ? dotv(n, t, s) / tt : 0.0;
+    OR_PFOR
     for (int i = 0; i < n; ++i) {
       x[i] += alpha * ph[i] + omega * sh[i];
       r[i] = s[i] - omega * t[i];
@@ -823,6 +858,7 @@ int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
     goto done;
   }
   or_spmv(n, ro, ci, va, x, q);
+  OR_PFOR
   for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
   rnorm = sqrt(dotv(n, r, r));
   tol = fmax(relTol * bnorm, absTol);
@@ -839,6 +875,7 @@ int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
     double pq = dotv(n, p, q);
     if (!(pq > 0.0)) break;
     double alpha = rho / pq;
+    OR_PFOR
     for (int i = 0; i < n; ++i) {
       x[i] += alpha * p[i];
       r[i] -= alpha * q[i];
@@ -851,6 +888,7 @@ int or_pcg_solve(void *hv, const double *b, double *x, double relTol,
     vcycle(&h->H, 0, r, z);
     double rhoNew = dotv(n, r, z);
     double beta = flex ? -dotv(n, z, q) / pq : rhoNew / rho;
+    OR_PFOR
     for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     rho = rhoNew;
   }
