@@ -394,7 +394,9 @@ def main():
     ap.add_argument("--simple-params", type=str, default="",
                     help="SIMPLE parameter overrides k=v,... (e.g. pAmgRebuildEvery=1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=1)
+    # the first three outer iterations (the first with the AMG setup): about
+    # 10-15 s of the oracle on 16 threads
+    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--only-psolve", action="store_true",
                     help="run only the pressure-solve section (profiling / counter runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
