@@ -66,14 +66,12 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
   amg.opt.omega = o.jacobiOmega;
   amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
   amg.opt.kcycle = o.kcycle > 0 ? o.kcycle : 0;
-  AN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   AN_HIP(hipStreamSynchronize(s));
 }
 
 PcgSolver::~PcgSolver() {
   drop_graphs();
   if (cs) (void)hipStreamDestroy(cs);
-  if (ev) (void)hipEventDestroy(ev);
   for (auto& e : spmvEv) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);

@@ -34,7 +34,6 @@ class PcgSolver {
   DevBuf<double> r, z, p, q;
   DevBuf<float> r32, z32, p32, q32, e32;
   hipStream_t s = 0;
-  hipEvent_t ev = nullptr;
   bool valuesSet = false;
   double lastSetupSeconds = 0.0;
   // profiling (opt.profile): event pairs around SpMV / V-cycle launches
