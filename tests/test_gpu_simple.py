@@ -49,6 +49,30 @@ def test_simple_matches_oracle(case):
     assert np.array_equal(f, fo)  # same fields -> identical serial sums
 
 
+def test_simple_converged_state_matches_oracle(gpu):
+    """Run to the case's own stop test (massRes < tolMass and the largest
+    velocity change < tolVel) on both sides: the same iteration count, the
+    converged fields within 1e-8 and the cylinder force within 1e-8."""
+    m = lp.Mesh.cylinder(20000)
+    vel, prs = sm.cylinder_case_bcs()
+    par = sm.default_params(velTol=1e-13, pTol=1e-13, pressure=dict(coarseSize=64))
+    S = sm.Simple(m, par, vel, prs)
+    om = O.OMesh.from_product(m)
+    tU, tP, ub, vb, wb, pb = S.bc
+    u0 = [S.field(k) for k in "uvwp"]
+    st, conv = S.iterate(300)
+    ref = O.simple_run(om, O.simple_params_from(par), tU, tP, ub, vb, wb, pb, 300, *u0)
+    assert conv and st.step == ref["done"] and st.step < 300
+    for k in "uvwp":
+        assert rel(S.field(k), ref[k]) < 1e-8, k
+    k = m.patch_index("patch_3_0")
+    f = S.patch_forces("patch_3_0", -1)
+    fo = O.patch_forces(om, m.patchStartFace[k], m.patchStartFace[k] + m.patchNFaces[k],
+                        ref["u"], ref["v"], ref["w"], ref["p"], par.mu, -1)
+    fc, foc = np.asarray(f)[6:9], np.asarray(fo)[6:9]  # total force on the cylinder
+    assert np.abs(fc - foc).max() <= 1e-8 * np.abs(foc).max()
+
+
 def test_simple_flux_is_conservative(case):
     m, vel, prs, par = case
     S = sm.Simple(m, par, vel, prs)
