@@ -454,6 +454,16 @@ an_status an_pcg_solve(an_solver_t s, const double* b, double* x, an_memspace me
   AN_CATCH
 }
 
+an_status an_pcg_residual_history(an_solver_t s, double* out, int cap, int* count) {
+  AN_TRY
+  if (!s || !count || (cap > 0 && !out)) throw Error(AN_EINVAL, "NULL argument");
+  const std::vector<double>& h = s->pcg->history;
+  *count = (int)h.size();
+  for (int i = 0; i < cap && i < (int)h.size(); ++i) out[i] = h[(size_t)i];
+  return AN_OK;
+  AN_CATCH
+}
+
 an_status an_spmv(an_solver_t s, const double* x, double* y, an_memspace mem) {
   AN_TRY
   if (!s || !x || !y) throw Error(AN_EINVAL, "NULL argument");

@@ -256,6 +256,7 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     rnPrev = rn;
     rn = std::sqrt(rr);
     *rnormOut = rn;
+    history.push_back(rn);
     if (!(pq > 0.0) || !std::isfinite(rn)) return -it;
     if (rn <= tol) break;
     if (!spec) next_direction();
@@ -440,6 +441,8 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
                         hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));
   const double bnorm = std::sqrt(hs.h[S_BB]);
+  history.clear();
+  history.push_back(std::sqrt(hs.h[S_RR]));
   an_solve_info inf{};
   inf.levels = amg.levels();
   inf.setupSeconds = lastSetupSeconds;
@@ -472,6 +475,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
         residual<double, double>(A, b, x, r.p, scal.p, red, s);
         read_scalar(S_RR);
         double rnNew = std::sqrt(hs.h[S_RR]);
+        history.push_back(rnNew);
         ++steps;
         if (!std::isfinite(rnNew) || (steps > 3 && rnNew > 0.9 * rn)) {
           rn = rnNew;

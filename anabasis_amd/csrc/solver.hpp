@@ -51,6 +51,9 @@ class PcgSolver {
   // b, x device pointers; x holds x0.
   an_status solve(const double* b, double* x, an_solve_info* info);
   void apply_prec(const double* rr, double* zz) { amg.apply(rr, zz, s); }
+  // residual 2-norms of the last solve: ||r0|| and one per PCG iteration
+  // (mixed precision: ||r0|| and the FP64 residual after each refinement)
+  std::vector<double> history;
 
  private:
   // z = M^-1 r on the solver's own r/z (FP64) or r32/z32 (FP32) vectors,

@@ -308,6 +308,15 @@ class PCGSolver:
         check(st, allow=(L.AN_ENOCONV,) if allow_noconv else ())
         return x, info
 
+    def residual_history(self):
+        """||r0|| and the residual norm after each iteration of the last solve
+        (mixed precision: after each refinement step)."""
+        cnt = C.c_int()
+        check(lib().an_pcg_residual_history(self._h, None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value)
+        check(lib().an_pcg_residual_history(self._h, ptr(out), cnt.value, C.byref(cnt)))
+        return out
+
     def spmv(self, x):
         y = np.empty(self.n)
         check(lib().an_spmv(self._h, ptr(f64(x)), ptr(y), L.AN_HOST))

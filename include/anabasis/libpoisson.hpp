@@ -37,6 +37,8 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
+#include <cstdio>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -411,6 +413,26 @@ struct Solver {
 };
 }  // namespace anabasis
 
+namespace anabasis {
+// opt.monitor: the residual norm of every iteration of the last solve, one
+// line each (the reference's monitor sets HYPRE's PCG print level 2,
+// hypre_backend.cu:63-64, 129-130)
+inline void print_residual_history(an_solver_t s, const std::vector<double>& rhs) {
+  int n = 0;
+  check(an_pcg_residual_history(s, nullptr, 0, &n));
+  std::vector<double> h((size_t)n);
+  check(an_pcg_residual_history(s, h.data(), n, &n));
+  double bb = 0.0;
+  for (double v : rhs) bb += v * v;
+  const double bn = std::sqrt(bb);
+  std::printf("  PCG iteration   ||r||_2        rate       ||r||_2/||b||_2\n");
+  for (int i = 0; i < n; ++i)
+    std::printf("  %13d   %.6e   %.6f   %.6e\n", i, h[(size_t)i],
+                i > 0 && h[(size_t)i - 1] > 0.0 ? h[(size_t)i] / h[(size_t)i - 1] : 0.0,
+                bn > 0.0 ? h[(size_t)i] / bn : 0.0);
+}
+}  // namespace anabasis
+
 // hypre_backend.cu:3-114: one-shot PCG solve; x is the initial guess and
 // the result.  Non-convergence is reported in the info, not thrown.
 inline HypreSolveInfo solve_system_hypre_gpu(const CSRPattern& pat,
@@ -426,6 +448,7 @@ inline HypreSolveInfo solve_system_hypre_gpu(const CSRPattern& pat,
   an_solve_info info{};
   an_status st = an_pcg_solve(s.h, rhs.data(), x.data(), AN_HOST, &info);
   if (st != AN_OK && st != AN_ENOCONV) anabasis::check(st);
+  if (opt.monitor) anabasis::print_residual_history(s.h, rhs);
   return HypreSolveInfo{info.iterations, info.finalRelResNorm};
 }
 
@@ -484,6 +507,7 @@ inline HypreSolveInfo solve_reusable_hypre_system_gpu(HypreReusableSystem& sys,
   an_solve_info info{};
   an_status st = an_pcg_solve(sys.solver->h, sys.rhs.data(), x.data(), AN_HOST, &info);
   if (st != AN_OK && st != AN_ENOCONV) anabasis::check(st);
+  if (sys.opt.monitor) anabasis::print_residual_history(sys.solver->h, sys.rhs);
   return HypreSolveInfo{info.iterations, info.finalRelResNorm};
 }
 

@@ -208,6 +208,13 @@ an_status an_pcg_set_values(an_solver_t s, const double *values,
 /* Solves A x = b; x holds the initial guess on entry. */
 an_status an_pcg_solve(an_solver_t s, const double *b, double *x,
                        an_memspace mem, an_solve_info *info);
+/* Residual 2-norms of the last an_pcg_solve: ||r0|| and one per PCG
+ * iteration (precision 1: ||r0|| and the FP64 residual after each
+ * refinement step), the values HYPRE prints at print level 2 when the
+ * reference's `monitor` option is set (LP/src/hypre_backend.cu:63-64,
+ * 129-130).  Copies min(cap, count) values into out (host memory); *count
+ * receives the full length. */
+an_status an_pcg_residual_history(an_solver_t s, double *out, int cap, int *count);
 /* y = A x with the solver's current values (test hook for the SpMV). */
 an_status an_spmv(an_solver_t s, const double *x, double *y, an_memspace mem);
 /* One preconditioner application z = M^-1 r (test hook for the V-cycle). */

@@ -237,6 +237,26 @@ def test_pcg_graph_replay_matches_eager(meshes, omeshes):
                 assert np.array_equal(xe, xg), (precision, scale)
 
 
+@pytest.mark.parametrize("kind", ["tet", "cyl"])
+def test_pcg_residual_history_matches_oracle(meshes, omeshes, kind):
+    """The per-iteration residual norms of a solve (what the reference's
+    `monitor` option prints) follow the oracle's PCG + AMG iteration by
+    iteration: same count, each norm within 1e-8 relative (only the dot
+    products' summation order differs)."""
+    m, om = meshes[kind], omeshes[kind]
+    vals = laplacian(om, m, ("xmin",) if kind == "tet" else ("patch_2_0",))
+    b = np.random.default_rng(41).standard_normal(om.nCells)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, coarseSize=64)
+    S.set_values(vals)
+    x, info = S.solve(b)
+    h = S.residual_history()
+    xo, it, rr, ho = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                               coarseSize=64, hist_cap=500)
+    assert info.converged and info.iterations == it and len(h) == it + 1
+    ho = ho[: it + 1]
+    assert np.all(np.abs(h - ho) <= 1e-8 * ho), np.abs(h - ho) / ho
+
+
 def test_pcg_repeatable(meshes, omeshes):
     """Two solvers fed the same sequence of solves (projection on, so the
     host reads scalars through both the per-iteration publication and the
