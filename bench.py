@@ -154,6 +154,53 @@ def run_momentum3(n):
                 note="host-array API: both timings include the PCIe upload of matrix and vectors")
 
 
+# -------------------------------------------------- MMS on tet cubes ----
+def run_mms_tet(nb):
+    """apps/poisson_mms_lib on the unit cube (nb^3 hex blocks x 6 tets):
+    u = sin(pi x) sin(pi y) sin(pi z), Dirichlet on every face, f = 3 pi^2 u.
+    Reports the L2 error (and the observed order against nb/2), and, for the
+    first pass's system (orthogonal part, x0 = 0), the PCG + AMG iterations
+    and residual history to 1e-10 relative."""
+    from anabasis_amd import _lib as L
+    from anabasis_amd import libpoisson as lp
+
+    def exact(x):
+        return np.prod(np.sin(np.pi * x), axis=1)
+
+    def solve(n):
+        m = lp.Mesh.tetcube(n)
+        bc = lp.ScalarBCSet([lp.ScalarPatchBC(p, L.AN_BC_DIRICHLET, lambda x, nn: exact(x))
+                             for p in m.patchNames])
+        src = 3 * np.pi ** 2 * exact(m.cc)
+        t0 = time.time()
+        phi, info = lp.solve_poisson(m, src, bc, lp.EllipticOptions(
+            nNonOrthCorr=10, solver=dict(absTol=1e-12)))
+        dt = time.time() - t0
+        err = phi - exact(m.cc)
+        return m, bc, src, np.sqrt(np.sum(m.vol * err ** 2) / m.vol.sum()), dt
+
+    _, _, _, e_half, _ = solve(nb // 2)
+    m, bc, src, e, dt = solve(nb)
+    types, vals = lp.build_boundary_face_data(m, bc)
+    pat = lp.build_scalar_pattern(m)
+    A, rhs = lp.assemble_scalar_elliptic_system(m, np.ones(m.nFaces), src, types, vals,
+                                                np.zeros((m.nCells, 3)), include_non_orth=False,
+                                                nnz=pat.nnz)
+    S = lp.PCGSolver(pat.rowOffsets, pat.cols, relTol=1e-10, absTol=0.0)
+    S.set_values(A)
+    x, info = S.solve(rhs)
+    hist = S.residual_history()
+    log(f"MMS tets {m.nCells}: L2 {e:.3e} (order {np.log2(e_half / e):.2f}), PCG {info.iterations} "
+        f"its to 1e-10, {1e3 * info.solveSeconds:.1f} ms")
+    return {"cells": m.nCells, "l2_error": e, "l2_error_half_resolution": e_half,
+            "observed_order": float(np.log2(e_half / e)), "poisson_solve_s": dt,
+            "pcg_iterations": info.iterations, "pcg_solve_s": info.solveSeconds,
+            "residual_history": hist.tolist(),
+            "note": "solve_poisson with 11 passes (nNonOrthCorr 10, each PCG from x0 = 0, "
+                    "absTol 1e-12) for the error; the history is one PCG solve of the first "
+                    "pass's system to 1e-10 relative"}
+
+
 # ------------------------------------------------------- batched MMS ----
 def run_batch_mms(nprob, precision):
     """nprob Dirichlet Poisson MMS problems on unit-cube tet meshes of n^3
@@ -385,6 +432,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-psolve", action="store_true")
     ap.add_argument("--mom3-n", type=int, default=100, help="3-RHS momentum box size (0: skip)")
+    ap.add_argument("--mms-n", type=int, default=16,
+                    help="MMS tet cube blocks per side (16: 24,576 tets; 0: skip)")
     ap.add_argument("--batch-problems", type=int, default=48, help="batched MMS problems (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=128, help="adjoint box size (0: skip)")
     ap.add_argument("--adjoint-ncorr", type=int, default=12)
@@ -451,6 +500,7 @@ def main():
                 "alg_bytes_per_launch": alg, "avg_launch_s": t,
                 "launches_timed": ps["fp64"]["spmv_calls"]}
     mom3 = run_momentum3(args.mom3_n) if args.mom3_n > 0 else None
+    mms = run_mms_tet(args.mms_n) if args.mms_n > 0 else None
     batch = None
     if args.batch_problems > 0:
         batch = {"fp64": run_batch_mms(args.batch_problems, 0),
@@ -479,6 +529,7 @@ def main():
         "simple_mixed": mixed,
         "psolve": ps,
         "momentum3": mom3,
+        "mms_tet": mms,
         "batch_mms": batch,
         "adjoint": adj,
         "roofline": roof,
