@@ -86,6 +86,25 @@ def run_simple(m, vel, prs, precision, warmup, steps):
                 fields={k: S.field(k) for k in "uvwp"})
 
 
+def run_cylinder_case(cells, max_iters):
+    """Configuration 2: the reference's cylinder channel at ~200k hex cells
+    (cases/cylinder.case controls), FP64, up to nsteps = 500 outer iterations
+    with the case's own stop test; reports the cylinder force vector
+    (pressure, viscous, total; forceNormalSign -1)."""
+    from anabasis_amd import simple as sm
+    m, vel, prs = simple_case(cells)
+    S = sm.Simple(m, sm.default_params(), vel, prs)
+    t0 = time.time()
+    st, conv = S.iterate(max_iters)
+    dt = time.time() - t0
+    f = S.patch_forces("patch_3_0", -1)
+    log(f"cylinder {m.nCells} cells: {st.step} its ({'converged' if conv else 'not converged'}) "
+        f"{dt:.2f}s, force {f[6]:.6e} {f[7]:.6e} {f[8]:.6e}")
+    return {"cells": m.nCells, "iterations": st.step, "converged": conv, "seconds": dt,
+            "massRes": st.massRes, "force_pressure": f[0:3].tolist(),
+            "force_viscous": f[3:6].tolist(), "force_total": f[6:9].tolist()}
+
+
 def cpu_baseline_simple(m, vel, prs, iters):
     from anabasis_amd import simple as sm
     from oracle import pyoracle as O
@@ -432,6 +451,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-psolve", action="store_true")
     ap.add_argument("--mom3-n", type=int, default=100, help="3-RHS momentum box size (0: skip)")
+    ap.add_argument("--case-cells", type=int, default=200_000,
+                    help="configuration 2 cylinder cells (0: skip)")
     ap.add_argument("--mms-n", type=int, default=16,
                     help="MMS tet cube blocks per side (16: 24,576 tets; 0: skip)")
     ap.add_argument("--batch-problems", type=int, default=48, help="batched MMS problems (0: skip)")
@@ -501,6 +522,7 @@ def main():
                 "launches_timed": ps["fp64"]["spmv_calls"]}
     mom3 = run_momentum3(args.mom3_n) if args.mom3_n > 0 else None
     mms = run_mms_tet(args.mms_n) if args.mms_n > 0 else None
+    case2 = run_cylinder_case(args.case_cells, 500) if args.case_cells > 0 else None
     batch = None
     if args.batch_problems > 0:
         batch = {"fp64": run_batch_mms(args.batch_problems, 0),
@@ -530,6 +552,7 @@ def main():
         "psolve": ps,
         "momentum3": mom3,
         "mms_tet": mms,
+        "cylinder_case": case2,
         "batch_mms": batch,
         "adjoint": adj,
         "roofline": roof,
