@@ -1,0 +1,96 @@
+"""Small and ragged systems on the GPU against the oracle.
+
+The other parity tests use meshes of hundreds to thousands of cells.  Here
+the fine level itself is smaller than one wavefront or one CSR row block,
+the row count is not a multiple of 64, and the AMG hierarchy is either the
+dense coarsest level alone (n <= coarseSize) or a chain of levels of a few
+rows each (coarseSize 4).  Addressing, gradients and assembly must match the
+oracle bit for bit; the hierarchy's application too; PCG follows the
+oracle's iteration count and residual history (dot-product order aside;
+below 1e-14 of the initial residual only roundoff remains).
+"""
+import numpy as np
+import pytest
+
+from anabasis_amd import _lib as L
+from anabasis_amd import libpoisson as lp
+from oracle import pyoracle as O
+
+from meshes import bc_arrays
+
+pytestmark = pytest.mark.gpu
+
+# 8, 60, 504 and 1287 cells
+SIZES = [(2, 2, 2), (5, 4, 3), (9, 8, 7), (13, 11, 9)]
+
+
+@pytest.fixture(scope="module", params=SIZES, ids=lambda s: "x".join(map(str, s)))
+def box(gpu, request):
+    nx, ny, nz = request.param
+    m = lp.Mesh.hexbox(nx, ny, nz, 1.0, 0.8, 0.6, jitter=0.2, seed=nx + 7 * ny)
+    return m, O.OMesh.from_product(m)
+
+
+def laplacian(om, m):
+    t, v = bc_arrays(m, dirichlet=("xmin",))
+    vals, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                  np.zeros((om.nCells, 3)))
+    return vals
+
+
+def test_small_addressing_gradient_assembly_bitexact(box):
+    m, om = box
+    pat = lp.csr_from_faces(m.nCells, om.owner, om.neigh)
+    ref = O.pattern(m.nCells, om.owner, om.neigh)
+    for a, b in zip((pat.rowOffsets, pat.cols, pat.diagPos, pat.facePP, pat.facePN, pat.faceNP,
+                     pat.faceNN), ref):
+        assert np.array_equal(a, b)
+    rng = np.random.default_rng(m.nCells)
+    nb = m.nBoundaryFaces
+    phi = rng.standard_normal(m.nCells)
+    t = (rng.random(nb) < 0.5).astype(np.int32)
+    v = rng.standard_normal(nb)
+    for mode, name in ((L.AN_GRAD_GSV1, "gsv1"), (L.AN_GRAD_LP, "lp")):
+        assert np.array_equal(lp.compute_lsq_gradient(m, phi, t, v, mode),
+                              O.lsq_gradient(om, phi, t, v, name)), name
+    gam = 0.5 + rng.random(m.nFaces)
+    grad = rng.standard_normal((m.nCells, 3))
+    vals, rhs = lp.assemble_scalar_elliptic_system(m, gam, phi, t, v, grad, True, False, 3, 1.25,
+                                                   nnz=om.nnz)
+    rv, rr = O.elliptic_assemble(om, gam, phi, t, v, grad, True, False, 3, 1.25)
+    assert np.array_equal(vals, rv) and np.array_equal(rhs, rr)
+
+
+@pytest.mark.parametrize("coarse", [64, 4])
+def test_small_spmv_and_hierarchy_bitexact(box, coarse):
+    m, om = box
+    vals = laplacian(om, m)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, coarseSize=coarse, kcycle=0)
+    S.set_values(vals)
+    x = np.random.default_rng(3).standard_normal(om.nCells)
+    assert np.array_equal(S.spmv(x), O.spmv(om.rowOffsets, om.cols, vals, x))
+    z = S.amg_apply(x)
+    zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=coarse, kcycle=0)
+    if om.nCells <= coarse:
+        assert nl == 1  # the dense coarsest level alone
+    assert np.array_equal(z, zo)
+
+
+@pytest.mark.parametrize("coarse", [64, 4])
+def test_small_pcg_follows_oracle(box, coarse):
+    m, om = box
+    vals = laplacian(om, m)
+    b = np.random.default_rng(17).standard_normal(om.nCells)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, coarseSize=coarse)
+    S.set_values(vals)
+    x, info = S.solve(b)
+    h = S.residual_history()
+    xo, it, rr, ho = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                               coarseSize=coarse, hist_cap=500)
+    assert info.converged and info.iterations == it and len(h) == it + 1
+    ho = ho[: it + 1]
+    # an exact coarsest solve (one level) or an exhausted Krylov space ends
+    # at roundoff: there the floor 1e-14 ||r0|| applies, elsewhere 1e-8
+    # relative
+    assert np.all(np.abs(h - ho) <= 1e-8 * ho + 1e-14 * ho[0]), np.abs(h - ho) / ho
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
