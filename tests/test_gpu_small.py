@@ -129,3 +129,38 @@ def test_small_bicgstab_and_fused_columns(box):
             assert infos[c].converged, c
         if infos[c].converged and ito >= 0:
             assert np.linalg.norm(xs[c] - xo) <= 1e-9 * max(np.linalg.norm(xo), 1e-300), c
+
+
+@pytest.mark.parametrize("dims", [(5, 4, 3), (9, 8, 7)], ids=["60", "504"])
+def test_small_simple_matches_oracle(gpu, dims):
+    """Four SIMPLE outer iterations of a small box channel (uniform inflow at
+    xmin, fixed pressure at xmax, no-slip elsewhere; a single-level or
+    two-level pressure hierarchy) against the oracle: fields within 1e-8,
+    wall forces on the same fields bit for bit."""
+    from anabasis_amd import simple as sm
+    m = lp.Mesh.hexbox(*dims, 1.0, 0.8, 0.6, jitter=0.2, seed=dims[0])
+    vel = [sm.VelocityPatchBC("xmin", "fixed_uniform_vector", vector=(0.3, 0.0, 0.0)),
+           sm.VelocityPatchBC("xmax", "zero_gradient")]
+    vel += [sm.VelocityPatchBC(p, "wall_noslip") for p in ("ymin", "ymax", "zmin", "zmax")]
+    prs = [sm.PressurePatchBC(p, "zero_gradient") for p in ("xmin", "ymin", "ymax", "zmin",
+                                                             "zmax")]
+    prs.append(sm.PressurePatchBC("xmax", "fixed_value", 0.0))
+    # mu = 0.01 keeps the cell Reynolds number of these coarse boxes low
+    # enough for the iteration to converge
+    par = sm.default_params(nNonOrthCorr=2, velTol=1e-14, pTol=1e-14, mu=0.01,
+                            pressure=dict(coarseSize=64))
+    S = sm.Simple(m, par, vel, prs)
+    om = O.OMesh.from_product(m)
+    tU, tP, ub, vb, wb, pb = S.bc
+    u0 = [S.field(k) for k in "uvwp"]
+    st, _ = S.iterate(4)
+    ref = O.simple_run(om, O.simple_params_from(par), tU, tP, ub, vb, wb, pb, 4, *u0)
+    assert ref["done"] == 4 and st.step == 4
+    for k in ("u", "v", "w", "p", "phi"):
+        a, b = S.field(k), ref[k]
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), k
+    k = m.patch_index("ymin")
+    f = S.patch_forces("ymin", -1)
+    fo = O.patch_forces(om, m.patchStartFace[k], m.patchStartFace[k] + m.patchNFaces[k],
+                        S.field("u"), S.field("v"), S.field("w"), S.field("p"), par.mu, -1)
+    assert np.array_equal(f, fo)
