@@ -96,6 +96,24 @@ def test_small_pcg_follows_oracle(box, coarse):
     assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("coarse", [64, 4])
+def test_small_mixed_precision_pcg(box, coarse):
+    """FP32 inner PCG/AMG with FP64 refinement on the same small systems
+    (the FP32 copy of a dense-only or few-row hierarchy): converged to the
+    FP64 tolerance and at the oracle's FP64 solution."""
+    m, om = box
+    vals = laplacian(om, m)
+    b = np.random.default_rng(17).standard_normal(om.nCells)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, coarseSize=coarse,
+                     precision=1)
+    S.set_values(vals)
+    x, info = S.solve(b)
+    xo, it, rr, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                              coarseSize=coarse)
+    assert info.converged and info.finalRelResNorm <= 1e-10
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
 def test_small_bicgstab_and_fused_columns(box):
     """Momentum-type (non-symmetric) systems: BiCGSTAB reaches the oracle's
     solution, and the fused 3-RHS solve equals three single solves bit for
