@@ -199,10 +199,13 @@ def default_params(**kw) -> L.SimpleParams:
 class Simple:
     """an_simple_t: the SIMPLE state on the GPU."""
 
-    def __init__(self, mesh, params: L.SimpleParams, vel, prs, inlet_patch="patch_1_0"):
+    def __init__(self, mesh, params: L.SimpleParams, vel, prs, inlet_patch="patch_1_0",
+                 bc=None):
+        """vel/prs: BC specs; or bc = (tU, tP, u, v, w, p) per-patch types and
+        per-face values given directly (vel and prs are then ignored)."""
         self.mesh = mesh  # keeps the mesh handle alive
         self.params = params
-        self.bc = apply_bc_specs(mesh, vel, prs)
+        self.bc = tuple(bc) if bc is not None else apply_bc_specs(mesh, vel, prs)
         tU, tP, ub, vb, wb, pb = self.bc
         inlet = mesh.patch_index(inlet_patch) if inlet_patch in mesh.patchNames else -1
         h = C.c_void_p()

@@ -31,7 +31,15 @@
  * orthogonal meshes; a Dirichlet MMS solve converges at second order in L2;
  * relaxation keeps the solution of the unrelaxed system and rAU = V/aP;
  * the rhs-only path plus its relaxation (aP rebuilt from the relaxed matrix,
- * main.cu:1790) equals the full assembly's relaxed rhs to round-off.
+ * main.cu:1790) equals the full assembly's relaxed rhs to round-off.  And by
+ * tests/test_oracle_pins.py: the momentum assembly with rho = 0 equals the
+ * LP elliptic assembly with gamma = mu on a jittered box (viscous
+ * coefficients, Dirichlet diagonal and rhs, non-orthogonal rhs and source);
+ * central convection on a uniform orthogonal box has the closed form
+ * +-U_k A_k / 2 off the diagonal, a zero interior diagonal, F_b on an
+ * outflow boundary diagonal and -F_b q_B on a Dirichlet rhs (transport and
+ * momentum alike); the plane-Poiseuille and driver-restatement checks of the
+ * SIMPLE loop exercise the momentum and pressure assemblies end to end.
  */
 #include "oracle.h"
 

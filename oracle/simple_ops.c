@@ -20,7 +20,12 @@
  * non-orthogonal flux vanishes on orthogonal meshes; the wall force of a
  * uniform pressure over a closed surface is zero, on a flat patch it is
  * p A n, the viscous force of a uniform tangential velocity is mu U A/dn,
- * and the pressure force is linear in p.
+ * and the pressure force is linear in p.  And by tests/test_oracle_pins.py:
+ * the Rhie-Chow flux has the closed forms rho A U.n - rho A rAU dp / h (zero
+ * cell gradients) and rho A U.n + rho A rAU a h^2 / 2 (cubic pressure with
+ * exact cell gradients, so the sign of the term is fixed); the correction
+ * and continuity operators run inside the plane-Poiseuille and
+ * driver-restatement checks of the SIMPLE loop.
  */
 #include "oracle.h"
 
