@@ -18,6 +18,7 @@ AN_OK, AN_EINVAL, AN_ENOCONV, AN_EHIP, AN_EIO, AN_ENOMEM, AN_EDIVERGED, AN_EINTE
 AN_HOST, AN_DEVICE = 0, 1
 AN_BC_NEUMANN, AN_BC_DIRICHLET, AN_BC_ZERO_GRADIENT = 0, 1, 2
 AN_GRAD_GSV1, AN_GRAD_LP = 0, 1
+AN_NOALLOC_MAX_ROWS = 400000  # include/anabasis_hip.h
 
 _STATUS = {
     AN_EINVAL: "AN_EINVAL", AN_ENOCONV: "AN_ENOCONV", AN_EHIP: "AN_EHIP", AN_EIO: "AN_EIO",
@@ -124,6 +125,8 @@ SIGNATURES = {
                                C.POINTER(SolveInfo)]),
     "an_bicgstab_solve3": (_I, [_I, _P, _P, C.c_int64, _P, _P, _P, _I, _D, _D, _I,
                                 C.POINTER(SolveInfo)]),
+    "an_bicgstab_solve_noalloc": (_I, [_I, _P, _P, C.c_int64, _P, _P, _P, _I, _D, _D,
+                                       C.POINTER(SolveInfo)]),
     "an_elliptic_default_options": (None, [C.POINTER(EllipticOptions)]),
     "an_solve_scalar_elliptic": (_I, [_P, _P, _P, _P, _P, C.POINTER(EllipticOptions), _P, _I,
                                       C.POINTER(SolveInfo)]),
@@ -156,6 +159,20 @@ SIGNATURES = {
 _lib = None
 
 
+def _load_torch_runtime_first():
+    """PyTorch's ROCm wheel carries its own HIP and HSA runtimes under
+    unversioned file names (torch/lib/libamdhip64.so, libhsa-runtime64.so).
+    Loaded after this library (which links /opt/rocm's libhsa-runtime64.so.1),
+    torch brings a second HSA runtime into the process, and whichever of the
+    two initialises second finds no GPU.  Loaded first, torch's HSA runtime
+    carries the soname this library asks for and is shared by both.  So when
+    torch is installed it is imported before the library is opened."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Loads the library once; raises if it has not been built."""
     global _lib
@@ -163,6 +180,7 @@ def lib():
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
                               "(make -C anabasis_amd/csrc)")
+        _load_torch_runtime_first()
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)

@@ -49,15 +49,43 @@ void set_last_error(const std::string& msg);
 
 #define AN_CHECK_LAUNCH() AN_HIP(hipGetLastError())
 
+// Entry-point wrappers of the C ABI: every C++ exception becomes an
+// an_status plus the thread-local message of an_last_error().
+#define AN_TRY try {
+#define AN_CATCH                                                     \
+  }                                                                  \
+  catch (const an::Error& e) {                                       \
+    ::an::set_last_error(e.what());                                        \
+    return (an_status)e.code;                                        \
+  }                                                                  \
+  catch (const std::bad_alloc&) {                                    \
+    ::an::set_last_error("out of host memory");                            \
+    return AN_ENOMEM;                                                \
+  }                                                                  \
+  catch (const std::exception& e) {                                  \
+    ::an::set_last_error(e.what());                                        \
+    return AN_EINVAL;                                                \
+  }                                                                  \
+  catch (...) {                                                      \
+    ::an::set_last_error("unknown error");                                 \
+    return AN_EINTERNAL;                                             \
+  }
+
 // Cache of device blocks (sizes rounded up to powers of two) for code that
 // allocates and frees many buffers, all used on one stream: a freed block
 // goes back to the cache instead of hipFree (which waits for the device)
 // and is handed out again in stream order.  A DevBuf allocated while a
 // PoolScope is active on its thread takes its block from that pool and
 // returns it there; the pool must outlive such buffers.
+//
+// Arena mode (constructed over a caller-owned device range): blocks are
+// consecutive 256-byte-aligned pieces of that range, never returned; the
+// pool neither allocates nor frees device memory, and a request beyond the
+// range throws AN_ENOMEM.
 class DevPool {
  public:
   DevPool() = default;
+  DevPool(void* arenaBase, size_t arenaBytes) : base_((char*)arenaBase), cap_(arenaBytes) {}
   DevPool(const DevPool&) = delete;
   DevPool& operator=(const DevPool&) = delete;
   ~DevPool() {
@@ -68,7 +96,19 @@ class DevPool {
     while (b < bytes) b <<= 1;
     return b;
   }
+  // block size handed out for a request of `bytes`
+  size_t round(size_t bytes) const { return base_ ? ((bytes + 255) & ~size_t(255)) : bucket(bytes); }
+  size_t used() const { return used_; }
   void* get(size_t bytes) {
+    if (base_) {
+      if (bytes > cap_ - used_)
+        throw Error(5, "scratch of " + std::to_string(cap_) + " bytes exhausted (" +
+                           std::to_string(used_) + " in use, " + std::to_string(bytes) +
+                           " more requested)");
+      void* p = base_ + used_;
+      used_ += bytes;
+      return p;
+    }
     auto it = free_.find(bytes);
     if (it != free_.end()) {
       void* p = it->second;
@@ -79,10 +119,14 @@ class DevPool {
     AN_HIP(hipMalloc(&p, bytes));
     return p;
   }
-  void put(void* p, size_t bytes) { free_.emplace(bytes, p); }
+  void put(void* p, size_t bytes) {
+    if (!base_) free_.emplace(bytes, p);
+  }
 
  private:
   std::multimap<size_t, void*> free_;
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
 };
 inline thread_local DevPool* tlPool = nullptr;
 struct PoolScope {
@@ -130,7 +174,7 @@ struct DevBuf {
     if (!count) return;
     if (tlPool) {
       pool = tlPool;
-      bytes = DevPool::bucket(count * sizeof(T));
+      bytes = pool->round(count * sizeof(T));
       p = (T*)pool->get(bytes);
     } else {
       AN_HIP(hipMalloc((void**)&p, count * sizeof(T)));

@@ -29,26 +29,6 @@ struct an_simple_s {
   std::unique_ptr<SimpleSolver> sim;
 };
 
-#define AN_TRY try {
-#define AN_CATCH                                                     \
-  }                                                                  \
-  catch (const an::Error& e) {                                       \
-    set_last_error(e.what());                                        \
-    return (an_status)e.code;                                        \
-  }                                                                  \
-  catch (const std::bad_alloc&) {                                    \
-    set_last_error("out of host memory");                            \
-    return AN_ENOMEM;                                                \
-  }                                                                  \
-  catch (const std::exception& e) {                                  \
-    set_last_error(e.what());                                        \
-    return AN_EINVAL;                                                \
-  }                                                                  \
-  catch (...) {                                                      \
-    set_last_error("unknown error");                                 \
-    return AN_EINTERNAL;                                             \
-  }
-
 static hipMemcpyKind kind_in(an_memspace m) {
   return m == AN_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
 }

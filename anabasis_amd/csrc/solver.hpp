@@ -10,17 +10,28 @@ namespace an {
 
 struct PinnedScalars {
   double* h = nullptr;
-  double* d = nullptr;  // the same memory as kernels address it
-  PinnedScalars() {
-    AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * S_COUNT));
+  double* d = nullptr;  // the same memory as kernels address it (pinned only)
+  // pinned == false: h is a plain host array (no allocation at all; copies
+  // into it are staged by the runtime)
+  explicit PinnedScalars(bool pinned = true) {
+    if (pinned) {
+      AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * S_COUNT));
+      AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
+      owned_ = true;
+    } else {
+      h = local_;
+    }
     for (int i = 0; i < S_COUNT; ++i) h[i] = 0.0;
-    AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
   }
   ~PinnedScalars() {
-    if (h) (void)hipHostFree(h);
+    if (owned_) (void)hipHostFree(h);
   }
   PinnedScalars(const PinnedScalars&) = delete;
   PinnedScalars& operator=(const PinnedScalars&) = delete;
+
+ private:
+  bool owned_ = false;
+  double local_[S_COUNT];
 };
 
 class PcgSolver {
@@ -90,6 +101,7 @@ struct BicgWork {
   DevBuf<double> scal, dinv, r, r0, p, v, ph, s_, sh, t;
   PinnedScalars hs;
   int n = 0;
+  explicit BicgWork(bool pinnedScalars = true) : hs(pinnedScalars) {}
   void ensure(int n, int nbSpmv, hipStream_t s);
 };
 an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,

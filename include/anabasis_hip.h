@@ -22,6 +22,7 @@
 #ifndef ANABASIS_HIP_H
 #define ANABASIS_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -231,6 +232,25 @@ an_status an_bicgstab_solve(int n, const int *rowOffsets, const int *cols,
                             const double *b, double *x, int maxIter,
                             double relTol, double absTol, an_memspace mem,
                             an_solve_info *info);
+
+/* an_bicgstab_solve without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, hosts that forbid hipMalloc).
+ * rowOffsets[n+1], cols[nnz], values[nnz], b[n] and x[n] (x0 on entry) are
+ * DEVICE pointers; the result is the same as an_bicgstab_solve with
+ * AN_DEVICE on the same inputs, bit for bit.  Every work buffer comes from
+ * one AN_NOALLOC_SCRATCH_BYTES device array per GPU that the library holds
+ * until the process ends, so n is limited to AN_NOALLOC_MAX_ROWS (AN_EINVAL
+ * beyond it, before any device work).  Calls on one GPU share that array:
+ * calls that overlap on the same GPU race.  Runs on the null stream and
+ * returns when the solve has finished. */
+#define AN_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+#define AN_NOALLOC_MAX_ROWS 400000
+an_status an_bicgstab_solve_noalloc(int n, const int *rowOffsets, const int *cols,
+                                    int64_t nnz, const double *values,
+                                    const double *b, double *x, int maxIter,
+                                    double relTol, double absTol,
+                                    an_solve_info *info);
 
 /* Three right-hand sides sharing one matrix -- the u/v/w momentum systems,
  * which the reference solves one after the other with the x-momentum matrix
