@@ -310,7 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
                                                        RedArgs ra, const T* __restrict__ omd,
                                                        T* __restrict__ y) {
   __shared__ double sh[kWaves];
-  const double alpha = scal[num] / scal[den];
+  const double d = scal[den], a = scal[num] / d;
+  const double alpha = (d > 0.0 && isfinite(a)) ? a : 0.0;
   double v = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = (T)((double)x[i] + alpha * (double)p[i]);
@@ -323,12 +324,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
   store_partials<1>(&bs, blockIdx.x, ra);
 }
 
-// k_finalize_nc<1> for scal[S_RR] that also publishes p.q and r.r to the
-// host as {p.q, stamp, r.r, stamp} (see pcg_update)
+// k_finalize_nc<1> for scal[S_RR] that also decides the iteration's outcome
+// and publishes it (PcgPub, pcg_word)
 __global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
                                                        const double* __restrict__ partial,
-                                                       double* __restrict__ scal,
-                                                       double* __restrict__ pub, double stamp) {
+                                                       double* __restrict__ scal, PcgPub pub) {
   __shared__ double sh[16];
   double v[1] = {0.0};
 #pragma unroll 4
@@ -336,24 +336,27 @@ __global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
   block_sums<1>(v, sh);
   if (threadIdx.x == 0) {
     scal[S_RR] = v[0];
-    // each value travels with the stamp in one 16-byte store: two separate
-    // stores to host memory (value, then stamp after a system-scope fence)
-    // were now and then seen in the other order
-    double2* p2 = reinterpret_cast<double2*>(pub);
-    p2[0] = make_double2(scal[S_PQ], stamp);
-    p2[1] = make_double2(v[0], stamp);
+    const double rn = sqrt(v[0]), pq = scal[S_PQ];
+    const int code = (!(pq > 0.0) || !isfinite(rn)) ? PCG_BREAKDOWN
+                     : (rn <= pub.tol)             ? PCG_CONVERGED
+                                                   : PCG_CONTINUE;
+    if (pub.hist) pub.hist[pub.idx] = rn;
+    // release at system scope: the history and scalars written above are
+    // visible before the word that announces them
+    __hip_atomic_store(pub.word, pcg_word(pub.stamp, code, (float)rn), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
-                Reducer& red, hipStream_t s, const T* omd, T* y, double* pub, double stamp) {
+                Reducer& red, hipStream_t s, const T* omd, T* y, const PcgPub* pub) {
   const int nb = red_blocks(n);
   k_pcg_update<T><<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR), omd,
                                         y);
   AN_CHECK_LAUNCH();
-  if (pub) {
-    k_finalize_pub<<<1, 1024, 0, s>>>(nb, red.partial.p, scal, pub, stamp);
+  if (pub && pub->word) {
+    k_finalize_pub<<<1, 1024, 0, s>>>(nb, red.partial.p, scal, *pub);
     AN_CHECK_LAUNCH();
   } else {
     finalize(red, nb, 1, scal + S_RR, s);
@@ -421,10 +424,10 @@ template void dot2<double>(int, const double*, const double*, const double*, dou
 template void dot2<float>(int, const float*, const float*, const float*, double*, int, Reducer&,
                           hipStream_t);
 template void pcg_update<double>(int, double*, double*, const double*, const double*, double*, int,
-                                 int, Reducer&, hipStream_t, const double*, double*, double*,
-                                 double);
+                                 int, Reducer&, hipStream_t, const double*, double*,
+                                 const PcgPub*);
 template void pcg_update<float>(int, float*, float*, const float*, const float*, double*, int, int,
-                                Reducer&, hipStream_t, const float*, float*, double*, double);
+                                Reducer&, hipStream_t, const float*, float*, const PcgPub*);
 template void pcg_direction<double>(int, double*, const double*, double*, int, int, bool,
                                     hipStream_t, double);
 template void pcg_direction<float>(int, float*, const float*, double*, int, int, bool,

@@ -164,16 +164,41 @@ template <typename T>
 void dot2(int n, const T* a, const T* b, const T* c, double* scal, int slot, Reducer& red,
           hipStream_t s);
 
-// x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den];
+// Outcome of one PCG iteration for the host, decided on the device by the
+// kernel that finishes r.r: CONTINUE, CONVERGED (||r|| <= tol) or BREAKDOWN
+// (p.q <= 0 or ||r|| not finite), checked in that order as HYPRE's PCG does.
+enum PcgDecision { PCG_CONTINUE = 0, PCG_CONVERGED = 1, PCG_BREAKDOWN = 2 };
+struct PcgPub {
+  unsigned long long* word = nullptr;  // device view of coherent host memory
+  unsigned stamp = 0;                  // iteration stamp (low 30 bits used)
+  double tol = 0.0;
+  double* hist = nullptr;              // device: hist[idx] = ||r|| (FP64)
+  int idx = 0;
+};
+// The published word: stamp in bits 63..34, decision in 33..32 and the
+// float value of ||r|| in 31..0 (a hint for the host's speculation only;
+// stop decisions and the residual history use the FP64 value).  One
+// naturally aligned 64-bit store, so the host never sees a stamp with
+// another iteration's decision.
+__host__ __device__ inline unsigned long long pcg_word(unsigned stamp, int code, float rn) {
+  unsigned bits;
+  __builtin_memcpy(&bits, &rn, sizeof bits);
+  return ((unsigned long long)(stamp & 0x3fffffffu) << 34) |
+         ((unsigned long long)(code & 3) << 32) | bits;
+}
+
+// x += alpha p ; r -= alpha q ; scal[S_RR] = r.r, alpha = scal[num]/scal[den]
+// (alpha = 0 when scal[den] <= 0 or the quotient is not finite: a breakdown
+// leaves x at the last good iterate, as HYPRE stops before the update);
 // with y: also y = omd o r (the preconditioner's first Jacobi sweep).
-// With pub (pinned host memory, mapped, 16-byte aligned): the kernel that
-// finishes r.r also stores pub[0..3] = {p.q, stamp, r.r, stamp}, each value
-// and its stamp in one 16-byte store; the host waits until both stamps are
-// the expected one instead of a copy and an event.
+// With pub.word: the kernel that finishes r.r also decides the iteration's
+// outcome, stores ||r|| into pub.hist[pub.idx] and publishes
+// pcg_word(stamp, decision, ||r||) with a system-scope release store; the
+// host waits for the stamp instead of a copy and a synchronisation.
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s, const T* omd = nullptr, T* y = nullptr,
-                double* pub = nullptr, double stamp = 0.0);
+                const PcgPub* pub = nullptr);
 // p = z + beta p, beta = sign * scal[num]/scal[den]  (first: p = z)
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,

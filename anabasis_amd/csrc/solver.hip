@@ -17,6 +17,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 namespace an {
 
@@ -193,31 +194,46 @@ static inline bool speculate(double rn, double rnPrev, double tol) {
   return !(rnPrev > 0.0 && rn * (rn / rnPrev) <= tol);
 }
 
-// A bounded spin on the stamp; past two seconds (a stalled or profiled
-// device) a stream synchronisation, after which the stamp must be there.
-void PcgSolver::wait_published(double stamp, double* pq, double* rr) {
-  volatile const double* h = pub.h;
-  auto ready = [&]() { return h[1] == stamp && h[3] == stamp; };
-  const auto t0 = std::chrono::steady_clock::now();
-  for (unsigned i = 1; !ready(); ++i) {
-    if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
-      AN_HIP(hipStreamSynchronize(s));
-      if (!ready()) throw Error(7, "PCG scalars were not published");
-      break;
-    }
-  }
-  std::atomic_thread_fence(std::memory_order_acquire);
-  *pq = h[0];
-  *rr = h[2];
+PcgSolver::PubWord::PubWord() {
+  AN_HIP(hipHostMalloc((void**)&h, 64, hipHostMallocCoherent));
+  __atomic_store_n(h, 0ull, __ATOMIC_RELEASE);
+  AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
+}
+PcgSolver::PubWord::~PubWord() {
+  if (h) (void)hipHostFree(h);
 }
 
-// FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.
+// A bounded spin on the stamp (acquire loads of the one published word);
+// past two seconds (a stalled or profiled device) a stream synchronisation,
+// after which the word must carry the stamp.
+PcgSolver::Published PcgSolver::wait_published(unsigned stamp) {
+  const unsigned want = stamp & 0x3fffffffu;
+  unsigned long long w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned i = 1; (unsigned)(w >> 34) != want; ++i) {
+    if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+      AN_HIP(hipStreamSynchronize(s));
+      w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
+      if ((unsigned)(w >> 34) != want) throw Error(7, "PCG iteration was not published");
+      break;
+    }
+    w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
+  }
+  float rn;
+  const unsigned bits = (unsigned)(w & 0xffffffffu);
+  std::memcpy(&rn, &bits, sizeof rn);
+  return Published{(int)((w >> 32) & 3), (double)rn};
+}
+
+// FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.  The residual
+// norm of iteration k lands in hist[k] (hist[0] = ||r0||, written here).
 int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut) {
   (void)b;
   const int n = A.n;
   double rn = std::sqrt(hs.h[S_RR]);
   *rnormOut = rn;
   if (rn <= tol) return 0;
+  hist.ensure((size_t)maxIter + 2);
   // flexible CG (K-cycle preconditioner): alpha = r.z / p.q with r.z in
   // S_FRZ, beta = -(z.q)/(p.q) (Polak-Ribiere: z.(r_new - r_old) / r_old.z_old
   // with r_new - r_old = -alpha q); otherwise beta = rho_new / rho_old
@@ -225,7 +241,7 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
   const ApplyDots d0{nullptr, red.args(scal.p + (flex ? S_FRZ : S_RHO0))};
   precondition(false, d0);  // r.z from the cycle's last kernel
   pcg_direction<double>(n, p.p, z.p, scal.p, 0, 0, true, s);
-  int it = 1;
+  int it = 1, ret = 0;
   double rnPrev = 0.0;
   for (; it <= maxIter; ++it) {
     const int rhoOld = flex ? S_FRZ : (it & 1) ? S_RHO0 : S_RHO1;
@@ -234,9 +250,10 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     spmv<double, double>(A, p.p, q.p, p.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                          e1 ? e1->b : nullptr);
     // the update also writes the preconditioner's y = omdinv o r and
-    // publishes r.r and p.q to the host
+    // publishes the iteration's outcome to the host
+    const PcgPub pa = pub_args(tol, it);
     pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv.p,
-                       amg.lv[0].y.p, pub.d, ++pubStamp);
+                       amg.lv[0].y.p, &pa);
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
@@ -250,19 +267,34 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
         pcg_direction<double>(n, p.p, z.p, scal.p, rhoNew, rhoOld, false, s);
     };
     const bool spec = speculate(rn, rnPrev, tol);
-    if (spec) next_direction();  // queued while the host waits for ||r||
-    double pq = 0.0, rr = 0.0;
-    wait_published(pubStamp, &pq, &rr);
+    if (spec) next_direction();  // queued while the host waits for the outcome
+    const Published pb = wait_published(pa.stamp);
     rnPrev = rn;
-    rn = std::sqrt(rr);
-    *rnormOut = rn;
-    history.push_back(rn);
-    if (!(pq > 0.0) || !std::isfinite(rn)) return -it;
-    if (rn <= tol) break;
+    rn = pb.rnHint;
+    if (pb.code == PCG_BREAKDOWN) {
+      ret = -it;
+      break;
+    }
+    if (pb.code == PCG_CONVERGED) {
+      ret = it;
+      break;
+    }
     if (!spec) next_direction();
   }
+  if (it > maxIter) {
+    it = maxIter;
+    ret = -maxIter;
+  }
   AN_HIP(hipStreamSynchronize(s));
-  return it > maxIter ? -maxIter : it;
+  // the FP64 residual norms of iterations 1..it, copied once
+  const size_t old = history.size();
+  history.resize(old + (size_t)it);
+  if (it > 0) {
+    AN_HIP(hipMemcpy(history.data() + old, hist.p + 1, sizeof(double) * (size_t)it,
+                     hipMemcpyDeviceToHost));
+    *rnormOut = history.back();
+  }
+  return ret;
 }
 
 // FP32 inner PCG from x = 0 on A32 with FP32 AMG.
@@ -270,6 +302,7 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
                      double* rnormOut) {
   const int n = A.n;
   const DevCSR<float>& A32 = amg.lv[0].A32;
+  hist.ensure((size_t)maxIter + 2);
   fill_zero(x, sizeof(float) * (size_t)n, s);
   AN_HIP(hipMemcpyAsync(r32.p, b, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
   const bool flex = amg.flexible();
@@ -284,8 +317,9 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     EvPair* e1 = next_ev(spmvEv, spmvUsed);
     spmv<float, float>(A32, p32.p, q32.p, p32.p, scal.p, S_PQ, red, s, e1 ? e1->a : nullptr,
                        e1 ? e1->b : nullptr);
+    const PcgPub pa = pub_args(tol, it);
     pcg_update<float>(n, x, r32.p, p32.p, q32.p, scal.p, rhoOld, S_PQ, red, s,
-                      amg.lv[0].omdinv32.p, amg.lv[0].y32.p, pub.d, ++pubStamp);
+                      amg.lv[0].omdinv32.p, amg.lv[0].y32.p, &pa);
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
@@ -300,17 +334,16 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     };
     const bool spec = speculate(rn, rnPrev, tol);
     if (spec) next_direction();
-    double pq = 0.0, rr = 0.0;
-    wait_published(pubStamp, &pq, &rr);
+    const Published pb = wait_published(pa.stamp);
     rnPrev = rn;
-    rn = std::sqrt(rr);
-    *rnormOut = rn;
-    if (!(pq > 0.0) || !std::isfinite(rn)) break;
-    if (rn <= tol) break;
+    rn = pb.rnHint;
+    if (pb.code != PCG_CONTINUE) break;
     if (!spec) next_direction();
   }
+  if (it > maxIter) it = maxIter;
   AN_HIP(hipStreamSynchronize(s));
-  return it > maxIter ? maxIter : it;
+  if (it > 0) AN_HIP(hipMemcpy(rnormOut, hist.p + it, sizeof(double), hipMemcpyDeviceToHost));
+  return it;
 }
 
 // ------------------------------------------- projection onto previous solutions

@@ -85,12 +85,37 @@ class PcgSolver {
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
   double read_scalar(int slot);
-  // r.r and p.q of an iteration come back through pcg_update's publication
-  // into pub ({p.q, stamp, r.r, stamp}): the host waits for both stamps.
-  // pub is a pinned buffer of its own that only that kernel writes.
-  PinnedScalars pub;
-  double pubStamp = 0.0;
-  void wait_published(double stamp, double* pq, double* rr);
+  // Each iteration's outcome comes back as one 64-bit word (pcg_word:
+  // stamp, device-side stop decision, float ||r|| hint) that the kernel
+  // finishing r.r stores with a system-scope release into coherent host
+  // memory; the host reads it with acquire loads until the stamp is the
+  // iteration's.  The FP64 residual norms stay in hist (device) and are
+  // copied once per solve.
+  struct PubWord {
+    unsigned long long* h = nullptr;
+    unsigned long long* d = nullptr;
+    PubWord();
+    ~PubWord();
+    PubWord(const PubWord&) = delete;
+    PubWord& operator=(const PubWord&) = delete;
+  };
+  PubWord pub;
+  unsigned pubStamp = 0;
+  DevBuf<double> hist;
+  struct Published {
+    int code;
+    double rnHint;
+  };
+  Published wait_published(unsigned stamp);
+  PcgPub pub_args(double tol, int idx) {
+    PcgPub a;
+    a.word = pub.d;
+    a.stamp = ++pubStamp;
+    a.tol = tol;
+    a.hist = hist.p;
+    a.idx = idx;
+    return a;
+  }
   EvPair* next_ev(std::vector<EvPair>& pool, size_t& used);
   void collect_profile(an_solve_info* inf);
 };

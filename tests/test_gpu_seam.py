@@ -1,0 +1,147 @@
+"""The solver seam a generic_simple_v1 caller would use (GSV1
+init_pressure_system / update_pressure_matrix_from_rAU /
+solve_pressure_gpu_device_rhs_device_x0, main.cu:2224-2676, and
+solve_momentum_gpu, main.cu:2553-2576) with DEVICE pointers: an_pcg_create,
+an_pcg_set_values, an_pcg_solve, an_bicgstab_solve and an_bicgstab_solve3
+given AN_DEVICE buffers must return the bits of the same calls on host
+arrays.  And PCG breakdown on an indefinite matrix: the solve stops before
+the update, as HYPRE's PCG and the oracle do, so x is the last good iterate
+(x0 itself when the first p.q is negative)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from anabasis_amd import _lib as L
+from anabasis_amd import libpoisson as lp
+from oracle import pyoracle as O
+
+from meshes import bc_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
+
+
+def _pressure_system(seed=2):
+    m = lp.Mesh.hexbox(20, 18, 16, jitter=0.2, seed=seed)
+    om = O.OMesh.from_product(m)
+    t, v = bc_arrays(m, dirichlet=("xmax",))
+    rng = np.random.default_rng(seed)
+    rau = rng.uniform(0.5, 2.0, om.nCells)
+    tP = np.array([0, 1, 0, 0, 0, 0], np.int32)
+    vals = O.pressure_assemble(om, rau, tP)
+    b = rng.standard_normal(om.nCells)
+    return om, vals, b
+
+
+def test_pcg_device_pointers_match_host(gpu):
+    import torch
+    om, vals, b = _pressure_system()
+    n, nnz = om.nCells, om.nnz
+    x0 = np.random.default_rng(7).standard_normal(n) * 1e-3
+    # host path
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0)
+    S.set_values(vals)
+    xh, ih = S.solve(b, x0)
+    hh = S.residual_history()
+    # device path: pattern, values, rhs and x0 all device pointers
+    ro, ci = _dev(torch, om.rowOffsets, torch.int32), _dev(torch, om.cols, torch.int32)
+    va, bd = _dev(torch, vals, torch.float64), _dev(torch, b, torch.float64)
+    xd = _dev(torch, x0, torch.float64)
+    opt = lp.pcg_options(relTol=1e-10, absTol=0.0)
+    h = C.c_void_p()
+    torch.cuda.synchronize()
+    L.check(L.lib().an_pcg_create(n, ro.data_ptr(), ci.data_ptr(), nnz, C.byref(opt),
+                                  L.AN_DEVICE, C.byref(h)))
+    try:
+        L.check(L.lib().an_pcg_set_values(h, va.data_ptr(), 1, L.AN_DEVICE))
+        info = L.SolveInfo()
+        L.check(L.lib().an_pcg_solve(h, bd.data_ptr(), xd.data_ptr(), L.AN_DEVICE,
+                                     C.byref(info)))
+        cnt = C.c_int()
+        L.check(L.lib().an_pcg_residual_history(h, None, 0, C.byref(cnt)))
+        hd = np.empty(cnt.value)
+        L.check(L.lib().an_pcg_residual_history(h, L.ptr(hd), cnt.value, C.byref(cnt)))
+    finally:
+        L.lib().an_solver_destroy(h)
+    torch.cuda.synchronize()
+    assert info.converged and info.iterations == ih.iterations > 3
+    assert np.array_equal(xd.cpu().numpy(), xh)
+    assert np.array_equal(hd, hh)
+    assert info.finalResNorm == ih.finalResNorm
+
+
+def test_bicgstab_device_pointers_match_host(gpu):
+    import torch
+    m = lp.Mesh.hexbox(18, 16, 14, jitter=0.2, seed=5)
+    pat = lp.build_scalar_pattern(m)
+    nb = m.nFaces - m.nInternalFaces
+    flux = m.array("Sf") @ np.array([0.8, -0.3, 0.2])
+    vals, _ = lp.assemble_scalar_transport_system(m, flux, np.full(m.nFaces, 0.04),
+                                                  np.zeros(m.nCells), np.zeros(m.nCells),
+                                                  np.ones(nb, np.int32), np.zeros(nb),
+                                                  np.zeros((m.nCells, 3)), False, False)
+    rng = np.random.default_rng(9)
+    bs = [rng.standard_normal(m.nCells) for _ in range(3)]
+    n, nnz = pat.nRows, pat.nnz
+    xh, ih = lp.bicgstab(pat.rowOffsets, pat.cols, vals, bs[0], absTol=1e-12)
+    xh3, ih3 = lp.bicgstab3(pat.rowOffsets, pat.cols, vals, bs, absTol=1e-12)
+    ro, ci = _dev(torch, pat.rowOffsets, torch.int32), _dev(torch, pat.cols, torch.int32)
+    va = _dev(torch, vals, torch.float64)
+    bd = [_dev(torch, b, torch.float64) for b in bs]
+    xd = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    torch.cuda.synchronize()
+    info = L.SolveInfo()
+    L.check(L.lib().an_bicgstab_solve(n, ro.data_ptr(), ci.data_ptr(), nnz, va.data_ptr(),
+                                      bd[0].data_ptr(), xd[0].data_ptr(), 1000, 0.0, 1e-12,
+                                      L.AN_DEVICE, C.byref(info)))
+    torch.cuda.synchronize()
+    assert info.iterations == ih.iterations
+    assert np.array_equal(xd[0].cpu().numpy(), xh)
+    xd = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    bp = (C.c_void_p * 3)(*[b.data_ptr() for b in bd])
+    xp = (C.c_void_p * 3)(*[x.data_ptr() for x in xd])
+    info3 = (L.SolveInfo * 3)()
+    torch.cuda.synchronize()
+    L.check(L.lib().an_bicgstab_solve3(n, ro.data_ptr(), ci.data_ptr(), nnz, va.data_ptr(), bp,
+                                       xp, 1000, 0.0, 1e-12, L.AN_DEVICE, info3))
+    torch.cuda.synchronize()
+    for c in range(3):
+        assert info3[c].iterations == ih3[c].iterations
+        assert np.array_equal(xd[c].cpu().numpy(), xh3[c]), c
+
+
+def _indefinite(k):
+    """Laplacian on a jittered box shifted between its k-th and (k+1)-th
+    eigenvalues: symmetric and indefinite."""
+    m = lp.Mesh.hexbox(10, 9, 8, jitter=0.2, seed=3)
+    om = O.OMesh.from_product(m)
+    t, v = bc_arrays(m, dirichlet=("xmin",))
+    vals, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                  np.zeros((om.nCells, 3)))
+    ev = np.linalg.eigvalsh(sp.csr_matrix((vals, om.cols, om.rowOffsets)).toarray())
+    vals = vals.copy()
+    vals[om.diagPos] -= 0.5 * (ev[k] + ev[k + 1])
+    b = np.random.default_rng(1).standard_normal(om.nCells)
+    return om, vals, b
+
+
+@pytest.mark.parametrize("k", [3, 100])
+def test_pcg_breakdown_keeps_last_good_iterate(gpu, k):
+    om, vals, b = _indefinite(k)
+    xo, it_o, _, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                               coarseSize=32)
+    assert it_o < 0  # the oracle broke down (returns -it - 1)
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, coarseSize=32)
+    S.set_values(vals)
+    x, info = S.solve(b, allow_noconv=True)
+    assert not info.converged and np.all(np.isfinite(x))
+    assert info.iterations == -it_o - 1
+    if k == 100:  # p.q < 0 in the first iteration: x never moves
+        assert np.all(x == 0.0) and np.all(xo == 0.0)
+    else:
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)

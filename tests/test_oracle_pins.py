@@ -337,3 +337,25 @@ def test_simple_outer_iteration_matches_independent_restatement():
                                                 pb, *state)
     r1 = O.simple_run(om, par, tU, tP, *fb, pb, 1, *state)
     assert np.abs(p1 - r1["p"]).max() > 1e-3 * np.abs(r1["p"]).max()
+
+
+# --------------------------------------------------------- PCG breakdown
+def test_pcg_breakdown_stops_before_the_update():
+    """A symmetric indefinite matrix (Laplacian shifted between two of its
+    eigenvalues): the PCG reports a breakdown (p.q <= 0) and returns the last
+    good iterate, finite; when the very first p.q is negative x stays x0
+    (HYPRE's PCG checks gamma = p.Ap before updating x)."""
+    m, om = box(10, 9, 8, jitter=0.2, seed=3)
+    t, v = bc_arrays(m, dirichlet=("xmin",))
+    vals, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, v,
+                                  np.zeros((om.nCells, 3)))
+    ev = np.linalg.eigvalsh(csr(om, vals).toarray())
+    b = np.random.default_rng(1).standard_normal(om.nCells)
+    for k, first in ((3, False), (100, True)):
+        shifted = vals.copy()
+        shifted[om.diagPos] -= 0.5 * (ev[k] + ev[k + 1])
+        x0 = np.zeros(om.nCells)
+        x, it, _, _ = O.pcg_amg(om.rowOffsets, om.cols, shifted, b, x0=x0, relTol=1e-10,
+                                absTol=0.0, coarseSize=32)
+        assert it < 0 and np.all(np.isfinite(x))
+        assert np.all(x == 0.0) == first
