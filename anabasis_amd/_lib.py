@@ -148,6 +148,12 @@ SIGNATURES = {
     "an_simple_default_params": (None, [C.POINTER(SimpleParams)]),
     "an_simple_create": (_I, [_P, C.POINTER(SimpleParams), _P, _P, _P, _P, _P, _P, _I,
                               C.POINTER(_P)]),
+    "an_bc_load": (_I, [C.c_char_p, C.POINTER(_P)]),
+    "an_bc_parse": (_I, [C.c_char_p, C.c_char_p, C.POINTER(_P)]),
+    "an_bc_destroy": (_I, [_P]),
+    "an_patch_geometry": (_I, [_P, _I, _P]),
+    "an_bc_coverage": (_I, [_P, _P, _I, C.c_char_p, _I, _pI, _pI]),
+    "an_bc_apply": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "an_simple_iterate": (_I, [_P, _I, C.POINTER(SimpleStats), _pI]),
     "an_simple_get_field": (_I, [_P, C.c_char_p, _P, _I]),
     "an_simple_set_field": (_I, [_P, C.c_char_p, _P, _I]),

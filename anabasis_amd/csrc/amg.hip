@@ -629,46 +629,160 @@ __global__ void k_kres(int n, const double* __restrict__ ks, const T* __restrict
 }
 
 
-// ------------------------------------------------------------ host -----
-std::vector<double> dense_inverse_host(int n, const std::vector<int>& ro,
-                                       const std::vector<int>& ci, const std::vector<double>& va) {
-  std::vector<double> A((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i)
-    for (int k = ro[i]; k < ro[i + 1]; ++k) A[(size_t)i * n + ci[k]] += va[k];
-  double dmax = 0.0;
-  for (int i = 0; i < n; ++i)
-    if (std::fabs(A[(size_t)i * n + i]) > dmax) dmax = std::fabs(A[(size_t)i * n + i]);
+// --------------------------------------------------- dense coarsest ----
+// A[i][:] = sum of row i's CSR entries in CSR order (duplicates added in
+// order), |A[i][i]| into dabs[i]; one thread per row.
+__global__ void k_dense_scatter(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                                const double* __restrict__ va, double* __restrict__ A,
+                                double* __restrict__ dabs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* row = A + (size_t)i * n;
+  for (int j = 0; j < n; ++j) row[j] = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k) row[ci[k]] += va[k];
+  dabs[i] = fabs(row[i]);
+}
+
+// Left-looking Cholesky in one workgroup, L in the lower triangle of A:
+// column k needs row k's finished prefix (staged in LDS) and every row
+// below it; a pivot d <= 1e-13 max|diag| zeroes the column instead.  Each
+// sum runs over j in increasing order, as the oracle's loops do.
+static constexpr int kCholThreads = 1024;
+static constexpr int kDenseChunk = 16;  // loads in flight per thread
+__global__ __launch_bounds__(kCholThreads) void k_dense_cholesky(int n, double* __restrict__ A,
+                                                                 const double* __restrict__ dabs) {
+  __shared__ double rowk[4096];
+  __shared__ double red[kCholThreads / 64];
+  __shared__ double lk;
+  __shared__ int skip;
+  const int t = threadIdx.x, nt = blockDim.x;
+  // max |diag|: order-free
+  double m = 0.0;
+  for (int i = t; i < n; i += nt) m = fmax(m, dabs[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((t & 63) == 0) red[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    double dm = 0.0;
+    for (int w = 0; w < nt / 64; ++w) dm = fmax(dm, red[w]);
+    red[0] = dm;
+  }
+  __syncthreads();
+  const double thr = 1e-13 * red[0];
   for (int k = 0; k < n; ++k) {
-    double d = A[(size_t)k * n + k];
-    for (int j = 0; j < k; ++j) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
-    if (d <= 1e-13 * dmax) {
-      for (int i = k; i < n; ++i) A[(size_t)i * n + k] = 0.0;
-      continue;
+    const double* Ak = A + (size_t)k * n;
+    for (int j = t; j < k; j += nt) rowk[j] = Ak[j];
+    __syncthreads();
+    if (t == 0) {
+      double d = Ak[k];
+      int j = 0;
+      for (; j + kDenseChunk <= k; j += kDenseChunk) {
+        double a[kDenseChunk];
+#pragma unroll
+        for (int u = 0; u < kDenseChunk; ++u) a[u] = rowk[j + u];
+#pragma unroll
+        for (int u = 0; u < kDenseChunk; ++u) d -= a[u] * a[u];
+      }
+      for (; j < k; ++j) d -= rowk[j] * rowk[j];
+      skip = d <= thr;
+      if (!skip) {
+        const double l = sqrt(d);
+        A[(size_t)k * n + k] = l;
+        lk = l;
+      }
     }
-    double l = std::sqrt(d);
-    A[(size_t)k * n + k] = l;
-    for (int i = k + 1; i < n; ++i) {
-      double s = A[(size_t)i * n + k];
-      for (int j = 0; j < k; ++j) s -= A[(size_t)i * n + j] * A[(size_t)k * n + j];
-      A[(size_t)i * n + k] = s / l;
+    __syncthreads();
+    if (skip) {
+      for (int i = k + t; i < n; i += nt) A[(size_t)i * n + k] = 0.0;
+    } else {
+      const double l = lk;
+      for (int i = k + 1 + t; i < n; i += nt) {
+        double* Ai = A + (size_t)i * n;
+        double s = Ai[k];
+        // s -= A[i][j] * L[k][j] for j = 0 .. k-1 in order; each chunk's
+        // loads are issued before its (ordered) updates
+        int j = 0;
+        for (; j + kDenseChunk <= k; j += kDenseChunk) {
+          double a[kDenseChunk];
+#pragma unroll
+          for (int u = 0; u < kDenseChunk; ++u) a[u] = Ai[j + u];
+#pragma unroll
+          for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * rowk[j + u];
+        }
+        for (; j < k; ++j) s -= Ai[j] * rowk[j];
+        Ai[k] = s / l;
+      }
     }
+    __syncthreads();
   }
-  std::vector<double> inv((size_t)n * n, 0.0), y(n);
-  for (int col = 0; col < n; ++col) {
-    for (int i = 0; i < n; ++i) {
-      double s = (i == col) ? 1.0 : 0.0;
-      for (int j = 0; j < i; ++j) s -= A[(size_t)i * n + j] * y[j];
-      double l = A[(size_t)i * n + i];
-      y[i] = (l != 0.0) ? s / l : 0.0;
+}
+
+// Column col of the inverse: forward substitution L y = e_col, then
+// backward L^T x = y, both in the oracle's order; one thread per column,
+// W[i*n + col] holds y and is overwritten by x (coalesced across a wave).
+__global__ void k_dense_columns(int n, const double* __restrict__ L, double* __restrict__ W) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
+  double* Wc = W + col;  // Wc[j * n] = entry j of this column
+  for (int i = 0; i < n; ++i) {
+    const double* Li = L + (size_t)i * n;
+    double s = (i == col) ? 1.0 : 0.0;
+    int j = 0;
+    for (; j + kDenseChunk <= i; j += kDenseChunk) {
+      double a[kDenseChunk], w[kDenseChunk];
+#pragma unroll
+      for (int u = 0; u < kDenseChunk; ++u) {
+        a[u] = Li[j + u];
+        w[u] = Wc[(size_t)(j + u) * n];
+      }
+#pragma unroll
+      for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * w[u];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = y[i];
-      for (int j = i + 1; j < n; ++j) s -= A[(size_t)j * n + i] * inv[(size_t)j * n + col];
-      double l = A[(size_t)i * n + i];
-      inv[(size_t)i * n + col] = (l != 0.0) ? s / l : 0.0;
-    }
+    for (; j < i; ++j) s -= Li[j] * Wc[(size_t)j * n];
+    const double l = Li[i];
+    Wc[(size_t)i * n] = (l != 0.0) ? s / l : 0.0;
   }
-  return inv;
+  for (int i = n - 1; i >= 0; --i) {
+    double s = Wc[(size_t)i * n];
+    int j = i + 1;
+    for (; j + kDenseChunk <= n; j += kDenseChunk) {
+      double a[kDenseChunk], w[kDenseChunk];
+#pragma unroll
+      for (int u = 0; u < kDenseChunk; ++u) {
+        a[u] = L[(size_t)(j + u) * n + i];
+        w[u] = Wc[(size_t)(j + u) * n];
+      }
+#pragma unroll
+      for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * w[u];
+    }
+    for (; j < n; ++j) s -= L[(size_t)j * n + i] * Wc[(size_t)j * n];
+    const double l = L[(size_t)i * n + i];
+    Wc[(size_t)i * n] = (l != 0.0) ? s / l : 0.0;
+  }
+}
+
+// invT[j*n + i] = W[i*n + j]
+__global__ void k_dense_transpose(int n, const double* __restrict__ W, double* __restrict__ invT) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const size_t j = e / (size_t)n, i = e % (size_t)n;
+  invT[e] = W[i * (size_t)n + j];
+}
+
+void dense_inverse_device(const DevCSR<double>& A, DevBuf<double>& invT, hipStream_t s) {
+  const int n = A.n;
+  if (n <= 0 || n > 4096) throw Error(7, "dense coarsest level must have 1..4096 rows");
+  const size_t nn = (size_t)n * n;
+  DevBuf<double> M(nn), W(nn), dabs((size_t)n);
+  k_dense_scatter<<<grid_for(n, 64), 64, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, M.p, dabs.p);
+  AN_CHECK_LAUNCH();
+  k_dense_cholesky<<<1, kCholThreads, 0, s>>>(n, M.p, dabs.p);
+  AN_CHECK_LAUNCH();
+  k_dense_columns<<<grid_for(n, 64), 64, 0, s>>>(n, M.p, W.p);
+  AN_CHECK_LAUNCH();
+  invT.alloc(nn);
+  k_dense_transpose<<<grid_for((long long)nn, 256), 256, 0, s>>>(n, W.p, invT.p);
+  AN_CHECK_LAUNCH();
 }
 
 static int read_int(const int* d, hipStream_t s) {
@@ -806,20 +920,7 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
   // lv may have reallocated while growing: re-point owned matrices
   for (size_t l = 1; l < lv.size(); ++l) lv[l].A = &lv[l].own;
   AmgLevel& Lc = lv.back();
-  if (Lc.n <= 4096) {
-    const DevCSR<double>& A = *Lc.A;
-    std::vector<int> ro(Lc.n + 1), ci((size_t)A.nnz);
-    std::vector<double> va((size_t)A.nnz);
-    AN_HIP(hipMemcpyAsync(ro.data(), A.rowOff.p, sizeof(int) * (Lc.n + 1), hipMemcpyDeviceToHost, s));
-    AN_HIP(hipMemcpyAsync(ci.data(), A.cols.p, sizeof(int) * A.nnz, hipMemcpyDeviceToHost, s));
-    AN_HIP(hipMemcpyAsync(va.data(), A.vals.p, sizeof(double) * A.nnz, hipMemcpyDeviceToHost, s));
-    AN_HIP(hipStreamSynchronize(s));
-    std::vector<double> inv = dense_inverse_host(Lc.n, ro, ci, va);
-    std::vector<double> invT((size_t)Lc.n * Lc.n);
-    for (int i = 0; i < Lc.n; ++i)
-      for (int j = 0; j < Lc.n; ++j) invT[(size_t)j * Lc.n + i] = inv[(size_t)i * Lc.n + j];
-    Lc.coarseInvT.upload(invT, s);
-  }
+  if (Lc.n <= 4096) dense_inverse_device(*Lc.A, Lc.coarseInvT, s);
   int maxNb = 0;
   for (size_t l = 1; l < lv.size(); ++l) {
     AmgLevel& L = lv[l];

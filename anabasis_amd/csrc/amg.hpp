@@ -70,9 +70,12 @@ class AmgHierarchy {
   DevBuf<double> kscal;  // 8 scalars per level: rho1, c1.b, c2.v1, c2.Ac2, c2.b
 };
 
-// Host dense inverse used for the coarsest level (Cholesky; pivots below
-// 1e-13*max|diag| treated as zero), shared with the oracle's algorithm.
-std::vector<double> dense_inverse_host(int n, const std::vector<int>& ro,
-                                       const std::vector<int>& ci, const std::vector<double>& va);
+// Dense inverse of the coarsest level (n <= 4096), built on the GPU:
+// Cholesky with pivots below 1e-13*max|diag| treated as zero, then one
+// forward and one backward substitution per column of the identity.  Every
+// entry is formed by the same sequence of operations as the oracle's
+// dense_inverse (oracle/solvers.c), so the result is the same bits.
+// invT[j*n + i] = inverse[i][j] (column-major).
+void dense_inverse_device(const DevCSR<double>& A, DevBuf<double>& invT, hipStream_t s);
 
 }  // namespace an

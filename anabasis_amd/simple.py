@@ -1,16 +1,19 @@
 """generic_simple_v1 driver over the C ABI, plus the runtime boundary
 conditions it consumes.
 
-Boundary data follow apps/generic_simple_v1/src:
-  build_patch_geometry_table   patch_geometry.cu:83-140
-  apply_bc_specs_to_legacy_face_arrays   velocity_bc_eval.cu:197-290
-  coverage table               main.cu:3178-3233
-The outer iteration itself runs in libanabasis_hip.so (an_simple_*).
+The boundary-condition layer runs in the library (an_bc_*, an_patch_geometry,
+following apps/generic_simple_v1/src):
+  load_runtime_bc_config                  bc_runtime_config.cu:83-210
+  build_patch_geometry_table              patch_geometry.cu:83-140
+  apply_bc_specs_to_legacy_face_arrays    velocity_bc_eval.cu:197-290
+  coverage table                          main.cu:3178-3233
+This module turns spec objects into the .bc grammar and calls it.  The outer
+iteration itself runs in libanabasis_hip.so (an_simple_*).
 """
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Sequence, Tuple
 
 import numpy as np
@@ -38,6 +41,21 @@ class VelocityPatchBC:
     coordB: int = 2
     normalMode: str = "average_patch_normal"  # or local_face_normal
 
+    def line(self) -> str:
+        """The spec as a .bc line."""
+        t = self.type
+        head = f"velocity {self.patchName} {t}"
+        if t == "fixed_uniform_vector":
+            return head + " " + " ".join(repr(float(v)) for v in self.vector)
+        if t == "fixed_normal_speed":
+            return f"{head} {float(self.normalSpeed)!r} {self.normalMode}"
+        if t == "fixed_flow_rate":
+            return f"{head} {float(self.flowRate)!r} {self.normalMode}"
+        if t == "parabolic_box_inlet":
+            return (f"{head} {float(self.H)!r} {float(self.Umax)!r} {self.coordA} {self.coordB} "
+                    f"{self.normalMode}")
+        return head
+
 
 @dataclass
 class PressurePatchBC:
@@ -45,126 +63,97 @@ class PressurePatchBC:
     type: str
     value: float = 0.0
 
+    def line(self) -> str:
+        head = f"pressure {self.patchName} {self.type}"
+        return f"{head} {float(self.value)!r}" if self.type == "fixed_value" else head
+
+
+def bc_text(vel: Sequence[VelocityPatchBC], prs: Sequence[PressurePatchBC]) -> str:
+    return "".join(s.line() + "\n" for s in list(vel) + list(prs))
+
+
+class BCConfig:
+    """an_bc_t: a parsed runtime BC configuration."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def load(path: str) -> "BCConfig":
+        h = C.c_void_p()
+        check(lib().an_bc_load(str(path).encode(), C.byref(h)))
+        return BCConfig(h)
+
+    @staticmethod
+    def parse(text: str, source: str = "<string>") -> "BCConfig":
+        h = C.c_void_p()
+        check(lib().an_bc_parse(text.encode(), source.encode(), C.byref(h)))
+        return BCConfig(h)
+
+    @staticmethod
+    def from_specs(vel, prs) -> "BCConfig":
+        return BCConfig.parse(bc_text(vel, prs), "<specs>")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().an_bc_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def _config(vel, prs) -> BCConfig:
+    return vel if isinstance(vel, BCConfig) else BCConfig.from_specs(vel, prs)
+
 
 def patch_geometry_table(mesh) -> List[dict]:
-    """patch_geometry.cu:83-140: area, area-weighted centroid, sum S,
-    average normal sum S/|sum S|, planarity |sum S|/area."""
+    """patch_geometry.cu:83-140 (an_patch_geometry): area, area-weighted
+    centroid, sum S, average normal, planarity, tangents."""
     out = []
     for k, name in enumerate(mesh.patchNames):
-        s, n = mesh.patchStartFace[k], mesh.patchNFaces[k]
-        if n < 0 or s < mesh.nInternalFaces or s + n > mesh.nFaces:
-            raise RuntimeError(f"Invalid patch face range for patch '{name}'")
-        Af = mesh.Af[s:s + n]
-        area = 0.0
-        cacc = np.zeros(3)
-        ssum = np.zeros(3)
-        for f in range(s, s + n):  # serial, face order
-            area += mesh.Af[f]
-            cacc = cacc + mesh.Af[f] * mesh.xf[f]
-            ssum = ssum + mesh.Sf[f]
-        nrm = float(np.sqrt(ssum @ ssum))
-        out.append(dict(patchIndex=k, patchName=name, nFaces=n, totalArea=area,
-                        centroid=cacc / area if area > 1e-30 else np.zeros(3), sumSf=ssum,
-                        averageNormal=ssum / nrm if nrm > 1e-30 else np.zeros(3),
-                        planarity=nrm / area if area > 1e-30 else 0.0))
-        del Af
+        g = np.empty(18)
+        check(lib().an_patch_geometry(mesh._h, k, ptr(g)))
+        out.append(dict(patchIndex=k, patchName=name, nFaces=int(g[0]), totalArea=g[1],
+                        centroid=g[2:5], sumSf=g[5:8], averageNormal=g[8:11], planarity=g[11],
+                        tangent1=g[12:15], tangent2=g[15:18]))
     return out
 
 
-def _parabolic(spec: VelocityPatchBC, x, nrm):
-    H, Um = spec.H, spec.Umax
-    if H <= 0.0:
-        raise RuntimeError(f"ParabolicBoxInlet has non-positive H on patch '{spec.patchName}'")
-    a = np.clip(x[:, spec.coordA], 0.0, H)
-    b = np.clip(x[:, spec.coordB], 0.0, H)
-    H2 = H * H
-    H4 = H2 * H2
-    mag = 16.0 * Um * a * b * (H - a) * (H - b) / H4
-    return -mag[:, None] * nrm  # inflow along the inward normal (velocity_bc_eval.cu:97-99)
-
-
-def apply_bc_specs(mesh, vel: Sequence[VelocityPatchBC], prs: Sequence[PressurePatchBC]):
-    """velocity_bc_eval.cu:197-290 -> per-patch types (1 = Dirichlet) and
-    per-face values (nFaces arrays, internal entries 0)."""
+def apply_bc_specs(mesh, vel, prs=None):
+    """an_bc_apply (velocity_bc_eval.cu:197-290): per-patch types (1 =
+    Dirichlet) and per-face values (nFaces arrays, internal entries 0).
+    vel/prs: spec lists, or vel = a BCConfig."""
+    cfg = _config(vel, prs)
     nP, nF = len(mesh.patchNames), mesh.nFaces
-    tU = np.zeros(nP, np.int32)  # Neumann unless set (main.cu:3262-3265)
-    tP = np.zeros(nP, np.int32)
-    uv = [np.zeros(nF) for _ in range(3)]
-    pv = np.zeros(nF)
-    geo = patch_geometry_table(mesh)
-    for spec in vel:
-        if spec.type not in VELOCITY_TYPES:
-            raise RuntimeError(f"unknown velocity BC type '{spec.type}'")
-        k = mesh.patch_index(spec.patchName)
-        s, n = mesh.patchStartFace[k], mesh.patchNFaces[k]
-        if spec.type == "zero_gradient":
-            tU[k] = 0
-            continue
-        tU[k] = 1
-        xf, nf = mesh.xf[s:s + n], mesh.nf[s:s + n]
-        g = geo[k]
-        if spec.normalMode == "average_patch_normal":
-            an = g["averageNormal"]
-            if np.linalg.norm(an) <= 1e-30:
-                raise RuntimeError(f"Cannot normalize vector for AveragePatchNormal on patch "
-                                   f"'{spec.patchName}'")
-            nrm = np.broadcast_to(an / np.linalg.norm(an), (n, 3))
-        else:
-            nrm = nf / np.linalg.norm(nf, axis=1)[:, None]
-        if spec.type == "wall_noslip":
-            U = np.zeros((n, 3))
-        elif spec.type == "fixed_uniform_vector":
-            U = np.broadcast_to(np.array(spec.vector, float), (n, 3))
-        elif spec.type == "parabolic_box_inlet":
-            U = _parabolic(spec, xf, nrm)
-        else:
-            if spec.type == "fixed_flow_rate":
-                if spec.normalMode == "average_patch_normal" and g["planarity"] < 0.98:
-                    raise RuntimeError("FixedFlowRate with AveragePatchNormal requires an "
-                                       f"approximately planar patch. Patch '{spec.patchName}'")
-                if g["totalArea"] <= 1e-30:
-                    raise RuntimeError(f"Patch '{spec.patchName}' has near-zero area")
-                un = spec.flowRate / g["totalArea"]
-            else:
-                un = spec.normalSpeed
-            U = un * nrm
-        for d in range(3):
-            uv[d][s:s + n] = U[:, d]
-    for spec in prs:
-        if spec.type not in PRESSURE_TYPES:
-            raise RuntimeError(f"unknown pressure BC type '{spec.type}'")
-        k = mesh.patch_index(spec.patchName)
-        s, n = mesh.patchStartFace[k], mesh.patchNFaces[k]
-        if spec.type == "fixed_value":
-            tP[k] = 1
-            pv[s:s + n] = spec.value
-        else:
-            tP[k] = 0
-    return tU, tP, uv[0], uv[1], uv[2], pv
+    tU, tP = np.zeros(nP, np.int32), np.zeros(nP, np.int32)
+    uv = [np.zeros(nF) for _ in range(4)]
+    check(lib().an_bc_apply(mesh._h, cfg._h, ptr(tU), ptr(tP), *[ptr(a) for a in uv]))
+    return tU, tP, uv[0], uv[1], uv[2], uv[3]
 
 
-def coverage_table(mesh, vel, prs) -> List[str]:
-    """main.cu:3178-3233: one line per patch; raises when a patch lacks a
-    velocity or pressure BC, or has two."""
-    for specs, what in ((vel, "velocity"), (prs, "pressure")):
-        names = [s.patchName for s in specs]
-        dup = sorted({n for n in names if names.count(n) > 1})
-        if dup:
-            raise RuntimeError(f"Duplicate {what} BC specs found: {dup}")
-    lines, missV, missP = [], [], []
-    for name in mesh.patchNames:
-        hv = any(s.patchName == name for s in vel)
-        hp = any(s.patchName == name for s in prs)
-        lines.append(f"  {name:<32s} velocity={'yes' if hv else 'NO'} "
-                     f"pressure={'yes' if hp else 'NO'}")
-        if not hv:
-            missV.append(name)
-        if not hp:
-            missP.append(name)
-    if missV or missP:
-        raise RuntimeError("incomplete runtime BC coverage: missing velocity "
-                           f"{missV}, pressure {missP}")
-    return lines
+def coverage(mesh, vel, prs=None):
+    """(table, report, complete) of main.cu:3178-3233 (an_bc_coverage)."""
+    cfg = _config(vel, prs)
+    out = []
+    for which in (0, 1):
+        n, comp = C.c_int(), C.c_int()
+        check(lib().an_bc_coverage(mesh._h, cfg._h, which, None, 0, C.byref(n), C.byref(comp)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().an_bc_coverage(mesh._h, cfg._h, which, buf, n.value + 1, C.byref(n),
+                                   C.byref(comp)))
+        out.append(buf.value.decode())
+    return out[0], out[1], bool(comp.value)
+
+
+def coverage_table(mesh, vel, prs=None) -> List[str]:
+    """The coverage lines; raises with the reference's report when a patch
+    lacks a velocity or pressure BC (or has two)."""
+    table, report, complete = coverage(mesh, vel, prs)
+    if not complete:
+        raise RuntimeError(report.strip())
+    return table.splitlines()
 
 
 def cylinder_case_bcs():

@@ -372,6 +372,43 @@ an_status an_lsq_gradient_adjoint(an_mesh_t mesh, const int *bcTypeBF,
                                   const double *bcValBF, const double *w,
                                   double *phibar, double *phibarBF,
                                   an_memspace mem);
+/* ------------------------------------------------- runtime boundary conditions */
+/* The runtime BC configuration of generic_simple_v1: `velocity <patch>
+ * <type> ...` and `pressure <patch> <type> ...` lines, '#' comments.
+ * Velocity types wall_noslip, zero_gradient, fixed_uniform_vector ux uy uz,
+ * fixed_normal_speed v [mode], fixed_flow_rate Q [mode],
+ * parabolic_box_inlet H Umax coordA coordB [mode] (mode
+ * average_patch_normal | local_face_normal); pressure types zero_gradient,
+ * open, fixed_value v.  Parse errors return AN_EINVAL with the reference's
+ * message ("BC config parse error in '<source>' at line <n>: ..."). */
+typedef struct an_bc_s *an_bc_t;
+/* load_runtime_bc_config (GSV1 bc_runtime_config.cu:83-210). */
+an_status an_bc_load(const char *path, an_bc_t *out);
+/* The same grammar from a string; sourceName stands for the path in
+ * messages. */
+an_status an_bc_parse(const char *text, const char *sourceName, an_bc_t *out);
+an_status an_bc_destroy(an_bc_t bc);
+/* build_patch_geometry_table (GSV1 patch_geometry.cu:83-140) for patch k:
+ * out[18] = {nFaces, area, centroid[3], sumSf[3], averageNormal[3],
+ * planarity, tangent1[3], tangent2[3]} (serial face-order sums). */
+an_status an_patch_geometry(an_mesh_t mesh, int k, double *out);
+/* The patch coverage table (GSV1 main.cu:3178-3233).  which = 0: the table,
+ * one "  %-32s velocity=yes|NO pressure=yes|NO" line per patch in mesh
+ * order; which = 1: the error report printed when a patch lacks a BC (empty
+ * when complete).  text receives min(cap-1, length) characters plus a nul;
+ * *length the full length; *complete 1 or 0.  Duplicate specs return
+ * AN_EINVAL ("Duplicate velocity BC specs found:\n  <patch>\n"). */
+an_status an_bc_coverage(an_mesh_t mesh, an_bc_t bc, int which, char *text, int cap,
+                         int *length, int *complete);
+/* validate_runtime_bc_config_against_patches + duplicate checks +
+ * apply_bc_specs_to_legacy_face_arrays (GSV1 bc_runtime_config.cu:212-230,
+ * main.cu:3115-3141, velocity_bc_eval.cu:197-290): per-patch types
+ * patchUType/patchPType[nPatches] (AN_BC_DIRICHLET or AN_BC_NEUMANN, the
+ * reference's bc_to_int) and per-face values [nFaces] (host arrays), the
+ * inputs of an_simple_create. */
+an_status an_bc_apply(an_mesh_t mesh, an_bc_t bc, int *patchUType, int *patchPType,
+                      double *uFaceBC, double *vFaceBC, double *wFaceBC, double *pFaceBC);
+
 /* ------------------------------------------------------------------ SIMPLE */
 typedef struct {
   double rho, mu;
