@@ -48,8 +48,14 @@ static std::string strip_comments(const std::string& txt) {
   while (a <= s.size()) {
     size_t e = s.find('\n', a);
     if (e == std::string::npos) e = s.size();
-    size_t c = s.find("//", a);
-    size_t stop = (c != std::string::npos && c < e) ? c : e;
+    // the first "//" of this line (searching only the line keeps the pass
+    // linear in the file size)
+    size_t stop = e;
+    for (size_t k = a; k + 1 < e; ++k)
+      if (s[k] == '/' && s[k + 1] == '/') {
+        stop = k;
+        break;
+      }
     out.append(s, a, stop - a);
     out.push_back('\n');
     if (e == s.size()) break;
