@@ -245,8 +245,194 @@ void finish_topology(HostMesh& m) {
   }
 }
 
+// ------------------------------------------------------- binary cache ---
+// <dir>/.anabasis_mesh.bin: the parsed mesh and its geometry, written after
+// an ASCII parse and used instead of the ASCII files while their sizes and
+// modification times are the ones recorded in it.  Best effort both ways:
+// an unwritable directory or a stale, truncated or foreign cache file just
+// means the ASCII files are parsed.  ANABASIS_MESH_CACHE=0 switches it off.
+namespace {
+constexpr char kCacheMagic[8] = {'A', 'N', 'M', 'E', 'S', 'H', '0', '1'};
+const char* const kMeshFiles[5] = {"boundary", "points", "faces", "owner", "neighbour"};
+
+bool cache_enabled() {
+  const char* e = std::getenv("ANABASIS_MESH_CACHE");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
+// FNV-1a of the first and last 4 KiB of a file (with its size and mtime:
+// a cheap fingerprint for files rewritten within one mtime tick)
+int64_t edge_hash(const std::string& fn, int64_t size) {
+  std::FILE* f = std::fopen(fn.c_str(), "rb");
+  if (!f) return 0;
+  uint64_t h = 1469598103934665603ull;
+  unsigned char buf[4096];
+  for (int part = 0; part < 2; ++part) {
+    if (part == 1 && std::fseek(f, (long)std::max<int64_t>(0, size - 4096), SEEK_SET) != 0) break;
+    const size_t n = std::fread(buf, 1, sizeof buf, f);
+    for (size_t i = 0; i < n; ++i) h = (h ^ buf[i]) * 1099511628211ull;
+  }
+  std::fclose(f);
+  return (int64_t)h;
+}
+
+// size, mtime (ns) and edge hash of each ASCII file; false if one is missing
+constexpr int kStamps = 15;
+bool file_stamps(const std::string& dir, int64_t out[kStamps]) {
+  for (int i = 0; i < 5; ++i) {
+    const std::string fn = dir + "/" + kMeshFiles[i];
+    struct stat st;
+    if (::stat(fn.c_str(), &st) != 0) return false;
+    out[3 * i] = (int64_t)st.st_size;
+    out[3 * i + 1] = (int64_t)st.st_mtim.tv_sec * 1000000000LL + (int64_t)st.st_mtim.tv_nsec;
+    out[3 * i + 2] = edge_hash(fn, (int64_t)st.st_size);
+  }
+  return true;
+}
+
+struct CacheWriter {
+  std::FILE* f;
+  bool ok = true;
+  void bytes(const void* p, size_t n) { ok = ok && (n == 0 || std::fwrite(p, 1, n, f) == n); }
+  void i64(int64_t v) { bytes(&v, sizeof v); }
+  template <typename T>
+  void vec(const std::vector<T>& v) {
+    i64((int64_t)v.size());
+    bytes(v.data(), v.size() * sizeof(T));
+  }
+  void str(const std::string& s) {
+    i64((int64_t)s.size());
+    bytes(s.data(), s.size());
+  }
+};
+
+struct CacheReader {
+  std::FILE* f;
+  bool ok = true;
+  void bytes(void* p, size_t n) { ok = ok && (n == 0 || std::fread(p, 1, n, f) == n); }
+  int64_t i64() {
+    int64_t v = 0;
+    bytes(&v, sizeof v);
+    return v;
+  }
+  template <typename T>
+  void vec(std::vector<T>& v, int64_t expect) {
+    const int64_t n = i64();
+    if (!ok || n != expect || n < 0) {
+      ok = false;
+      return;
+    }
+    v.resize((size_t)n);
+    bytes(v.data(), (size_t)n * sizeof(T));
+  }
+  std::string str() {
+    const int64_t n = i64();
+    if (!ok || n < 0 || n > 4096) {
+      ok = false;
+      return {};
+    }
+    std::string s((size_t)n, ' ');
+    bytes(&s[0], (size_t)n);
+    return s;
+  }
+};
+
+bool load_cache(const std::string& dir, HostMesh& m) {
+  int64_t want[kStamps];
+  if (!cache_enabled() || !file_stamps(dir, want)) return false;
+  std::FILE* f = std::fopen((dir + "/.anabasis_mesh.bin").c_str(), "rb");
+  if (!f) return false;
+  CacheReader r{f};
+  char magic[8];
+  r.bytes(magic, sizeof magic);
+  bool ok = r.ok && std::memcmp(magic, kCacheMagic, sizeof magic) == 0;
+  for (int i = 0; ok && i < kStamps; ++i) ok = r.i64() == want[i];
+  if (ok) {
+    m.nPoints = (int)r.i64();
+    m.nFaces = (int)r.i64();
+    m.nInternalFaces = (int)r.i64();
+    m.nCells = (int)r.i64();
+    const int64_t nPatches = r.i64(), nFaceVerts = r.i64();
+    ok = r.ok && m.nPoints >= 0 && m.nFaces >= 0 && m.nInternalFaces >= 0 && m.nCells >= 0 &&
+         nPatches >= 0 && nPatches < 100000 && nFaceVerts >= 0;
+    for (int64_t k = 0; ok && k < nPatches; ++k) {
+      m.patchNames.push_back(r.str());
+      m.patchTypes.push_back(r.str());
+      m.patchStart.push_back((int)r.i64());
+      m.patchNFaces.push_back((int)r.i64());
+      ok = r.ok;
+    }
+    const int64_t nP = m.nPoints, nF = m.nFaces, nC = m.nCells;
+    r.vec(m.P, 3 * nP);
+    r.vec(m.faceOff, nF + 1);
+    r.vec(m.faceVerts, nFaceVerts);
+    r.vec(m.owner, nF);
+    r.vec(m.neigh, (int64_t)m.nInternalFaces);
+    r.vec(m.bPatch, nF);
+    r.vec(m.cc, 3 * nC);
+    r.vec(m.vol, nC);
+    r.vec(m.xf, 3 * nF);
+    r.vec(m.nf, 3 * nF);
+    r.vec(m.Sf, 3 * nF);
+    r.vec(m.Af, nF);
+    r.bytes(&m.maxNonOrthDeg, sizeof m.maxNonOrthDeg);
+    char tail = 0;
+    ok = ok && r.ok && std::fread(&tail, 1, 1, f) == 0;  // nothing after the record
+  }
+  std::fclose(f);
+  if (!ok) {
+    m = HostMesh();
+    return false;
+  }
+  m.hasPoints = m.hasGeometry = true;
+  return true;
+}
+
+void save_cache(const std::string& dir, const HostMesh& m) {
+  int64_t stamps[kStamps];
+  if (!cache_enabled() || !file_stamps(dir, stamps)) return;
+  const std::string tmp = dir + "/.anabasis_mesh.bin.tmp";
+  std::FILE* f = std::fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  CacheWriter w{f};
+  w.bytes(kCacheMagic, sizeof kCacheMagic);
+  for (int64_t v : stamps) w.i64(v);
+  w.i64(m.nPoints);
+  w.i64(m.nFaces);
+  w.i64(m.nInternalFaces);
+  w.i64(m.nCells);
+  w.i64((int64_t)m.patchNames.size());
+  w.i64((int64_t)m.faceVerts.size());
+  for (size_t k = 0; k < m.patchNames.size(); ++k) {
+    w.str(m.patchNames[k]);
+    w.str(k < m.patchTypes.size() ? m.patchTypes[k] : std::string());
+    w.i64(m.patchStart[k]);
+    w.i64(m.patchNFaces[k]);
+  }
+  w.vec(m.P);
+  w.vec(m.faceOff);
+  w.vec(m.faceVerts);
+  w.vec(m.owner);
+  w.vec(m.neigh);
+  w.vec(m.bPatch);
+  w.vec(m.cc);
+  w.vec(m.vol);
+  w.vec(m.xf);
+  w.vec(m.nf);
+  w.vec(m.Sf);
+  w.vec(m.Af);
+  w.bytes(&m.maxNonOrthDeg, sizeof m.maxNonOrthDeg);
+  const bool ok = w.ok && std::fclose(f) == 0;
+  if (ok)
+    std::rename(tmp.c_str(), (dir + "/.anabasis_mesh.bin").c_str());
+  else
+    std::remove(tmp.c_str());
+}
+}  // namespace
+
 HostMesh read_polymesh(const std::string& dir) {
   HostMesh m;
+  if (load_cache(dir, m)) return m;
   read_boundary(dir + "/boundary", m);
   m.P = read_points(dir + "/points");
   m.nPoints = (int)(m.P.size() / 3);
@@ -264,6 +450,7 @@ HostMesh read_polymesh(const std::string& dir) {
   m.hasPoints = true;
   finish_topology(m);
   compute_geometry(m);
+  save_cache(dir, m);
   return m;
 }
 
@@ -277,6 +464,7 @@ static void write_header(std::FILE* f, const char* cls, const char* obj) {
 
 void write_polymesh(const HostMesh& m, const std::string& dir) {
   ::mkdir(dir.c_str(), 0755);
+  std::remove((dir + "/.anabasis_mesh.bin").c_str());  // describes the old files
   auto open = [&](const char* name) {
     std::FILE* f = std::fopen((dir + "/" + name).c_str(), "w");
     if (!f) throw std::runtime_error("Could not write " + dir + "/" + name);

@@ -77,3 +77,51 @@ def test_topology_only_mesh(cylinder_fixture):
     assert m.patchNames == ["patch_1_0", "patch_0_0", "patch_2_0", "patch_5_0", "patch_3_0"]
     with pytest.raises(L.AnError):
         m.array("cc")
+
+
+def _mesh_arrays(m):
+    return {k: m.array(k) for k in ("points", "faceOffsets", "faceVerts", "owner", "neigh",
+                                    "bPatch", "cc", "vol", "xf", "nf", "Sf", "Af")}
+
+
+def test_binary_mesh_cache(tmp_path, monkeypatch):
+    """read_polymesh writes <dir>/.anabasis_mesh.bin after parsing and reads it
+    back while the ASCII files are unchanged: the same arrays bit for bit; a
+    rewritten, edited or corrupted mesh directory falls back to the parse."""
+    import os
+    monkeypatch.delenv("ANABASIS_MESH_CACHE", raising=False)
+    d = tmp_path / "polyMesh"
+    m0 = Mesh.hexbox(5, 4, 3, 1.0, 0.8, 0.6, jitter=0.2, seed=1)
+    m0.write_polymesh(d)
+    cache = d / ".anabasis_mesh.bin"
+    assert not cache.exists()
+    a = _mesh_arrays(Mesh.read_polymesh(d))
+    assert cache.exists()
+    b = _mesh_arrays(Mesh.read_polymesh(d))
+    ref = _mesh_arrays(m0)
+    for k in ref:
+        assert np.array_equal(a[k], ref[k]) and np.array_equal(b[k], ref[k]), k
+    # rewriting the directory drops the cache; the new mesh is read
+    m1 = Mesh.hexbox(5, 4, 3, 1.0, 0.8, 0.6, jitter=0.2, seed=2)
+    m1.write_polymesh(d)
+    assert not cache.exists()
+    c = _mesh_arrays(Mesh.read_polymesh(d))
+    assert np.array_equal(c["cc"], m1.array("cc")) and not np.array_equal(c["cc"], a["cc"])
+    # an edit of the same size (points moved by hand): the stamps differ
+    pts = (d / "points").read_text()
+    first = pts.index("(", pts.index("(") + 1)
+    edited = pts[:first] + pts[first:].replace("0.8", "0.9", 1)
+    assert len(edited) == len(pts)
+    (d / "points").write_text(edited)
+    e = _mesh_arrays(Mesh.read_polymesh(d))
+    assert not np.array_equal(e["points"], c["points"])
+    # a truncated cache is ignored
+    blob = cache.read_bytes()
+    cache.write_bytes(blob[: len(blob) // 2])
+    f = _mesh_arrays(Mesh.read_polymesh(d))
+    assert np.array_equal(f["points"], e["points"]) and np.array_equal(f["cc"], e["cc"])
+    # ANABASIS_MESH_CACHE=0: neither read nor written
+    os.remove(cache)
+    monkeypatch.setenv("ANABASIS_MESH_CACHE", "0")
+    Mesh.read_polymesh(d)
+    assert not cache.exists()
