@@ -74,16 +74,18 @@ __global__ void k3_unpack(int n, const double* __restrict__ X, double* __restric
   a2[i] = X[3 * i + 2];
 }
 
-__global__ void k3_zero_col(int n, double* __restrict__ X, int c) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) X[3 * i + c] = 0.0;
-}
+// The live column mask of a solve; without a state (the set-up kernels
+// before the first stop test) all three columns.
+__device__ __forceinline__ int live_mask(const Bicg3State* st) { return st ? st->mask : 7; }
 
-// partial[3 blk + c] = block's share of sum_i A[3i+c] B[3i+c] (k_dot's map)
+// partial[3 blk + c] = block's share of sum_i A[3i+c] B[3i+c] (k_dot's map);
+// nothing once no column is live
 __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A,
                                              const double* __restrict__ B,
+                                             const Bicg3State* st,
                                              double* __restrict__ partial) {
   __shared__ double sh[3][kW];
+  if (!live_mask(st)) return;  // grid-uniform
   double v[3] = {0.0, 0.0, 0.0};
   const int S = gridDim.x * kB;
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * S) {
@@ -107,8 +109,8 @@ __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A
 }
 
 // MODE 0: Y = A X; MODE 1: Y = B - A X.  DOT 0: none; 1: dot(DW, Y); 2: dot(Y, Y)
-// (per column, k_spmv's row blocks and trees).  Columns outside 'mask' are
-// neither written nor reduced.
+// (per column, k_spmv's row blocks and trees).  Columns outside the live
+// mask are neither written nor reduced.
 template <int MODE, int DOT>
 __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ rbs,
                                               const int* __restrict__ rowOff,
@@ -117,10 +119,13 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
                                               const double* __restrict__ X,
                                               const double* __restrict__ B,
                                               double* __restrict__ Y,
-                                              const double* __restrict__ DW, int mask,
+                                              const double* __restrict__ DW,
+                                              const Bicg3State* st,
                                               double* __restrict__ partial) {
   __shared__ double prod[kW][kCap];
   __shared__ double sh[3][kW];
+  const int mask = live_mask(st);
+  if (!mask) return;  // grid-uniform
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
@@ -182,7 +187,10 @@ __global__ __launch_bounds__(kB) void k3_bicg_p(int n, const double* __restrict_
                                                 const double* __restrict__ dinv,
                                                 double* __restrict__ PH,
                                                 const double* __restrict__ sc, int rhoNew,
-                                                int rhoOld, int first, int mask) {
+                                                int rhoOld, int first,
+                                                const Bicg3State* __restrict__ st) {
+  const int mask = st->mask;
+  if (!mask) return;
   double beta[3], om[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -208,9 +216,12 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
                                                 const double* __restrict__ V,
                                                 const double* __restrict__ dinv,
                                                 double* __restrict__ S, double* __restrict__ SH,
-                                                double* __restrict__ sc, int rhoNew, int mask,
+                                                double* __restrict__ sc, int rhoNew,
+                                                const Bicg3State* __restrict__ st,
                                                 double* __restrict__ partial) {
   __shared__ double sh[3][kW];
+  const int mask = st->mask;
+  if (!mask) return;
   double alpha[3], acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < 3; ++c) alpha[c] = sc[3 * rhoNew + c] / sc[3 * S_R0V + c];
@@ -253,8 +264,10 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
 // partial[6 blk + c] = t.s, partial[6 blk + 3 + c] = t.t   (k_dot2 per column)
 __global__ __launch_bounds__(kB) void k3_dot2(int n, const double* __restrict__ T,
                                               const double* __restrict__ S,
+                                              const Bicg3State* __restrict__ st,
                                               double* __restrict__ partial) {
   __shared__ double sh[3][kW];
+  if (!st->mask) return;
   double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
   const int St = gridDim.x * kB;
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
@@ -287,10 +300,6 @@ __global__ __launch_bounds__(kB) void k3_dot2(int n, const double* __restrict__ 
     }
 }
 
-struct Tol3 {
-  double t[3];
-};
-
 // omega = ts/tt; x += alpha ph (+ omega sh); r = s (- omega t); rr = r.r
 __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
                                                 double* __restrict__ R,
@@ -298,9 +307,12 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
                                                 const double* __restrict__ SH,
                                                 const double* __restrict__ S,
                                                 const double* __restrict__ T,
-                                                double* __restrict__ sc, Tol3 tol, int mask,
+                                                double* __restrict__ sc,
+                                                const Bicg3State* __restrict__ st,
                                                 double* __restrict__ partial) {
   __shared__ double sh[3][kW];
+  const int mask = st->mask;
+  if (!mask) return;
   double alpha[3], omega[3], acc[3] = {0.0, 0.0, 0.0};
   bool convS[3];
 #pragma unroll
@@ -308,7 +320,7 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
     alpha[c] = sc[3 * S_AL + c];
     const double tt = sc[3 * S_TT + c];
     omega[c] = (tt > 0.0) ? sc[3 * S_TS + c] / tt : 0.0;
-    convS[c] = sqrt(sc[3 * S_SS + c]) <= tol.t[c];
+    convS[c] = sqrt(sc[3 * S_SS + c]) <= st->tol[c];
   }
   const int St = gridDim.x * kB;
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
@@ -364,19 +376,137 @@ __global__ void k3_dinv(int n, const int* __restrict__ ro, const int* __restrict
   dinv[i] = (d != 0.0) ? 1.0 / d : 1.0;
 }
 
+// k_finalize_nc<NC> (the same loads and tree, so the same bits) that does
+// nothing once no column is live
+template <int NC>
+__global__ __launch_bounds__(1024) void k3_fin(int nparts, const double* __restrict__ partial,
+                                               double* __restrict__ out,
+                                               const Bicg3State* st) {
+  __shared__ double sh[NC * 16];
+  if (!live_mask(st)) return;
+  double v[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[c] = 0.0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += partial[(size_t)i * NC + c];
+  block_sums<NC>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = v[c];
+}
+
+// The stop test before the first iteration: tolerances, b = 0 columns
+// (x = 0, HYPRE's convention) and columns whose r0 already meets the tolerance.
+__global__ void k3_init(const double* __restrict__ sc, Bicg3State* __restrict__ st, int activeIn,
+                        double relTol, double absTol) {
+  if (threadIdx.x != 0) return;
+  int m = activeIn & 7, z = 0;
+  for (int c = 0; c < 3; ++c) {
+    const double bnorm = sqrt(sc[3 * S_BB + c]), rn = sqrt(sc[3 * S_RR + c]);
+    const double tol = fmax(relTol * bnorm, absTol);
+    st->bnorm[c] = bnorm;
+    st->rn[c] = rn;
+    st->tol[c] = tol;
+    st->its[c] = 0;
+    st->conv[c] = 0;
+    if (!((m >> c) & 1)) continue;
+    if (bnorm == 0.0) {
+      z |= 1 << c;
+      st->conv[c] = 1;
+      m &= ~(1 << c);
+    } else if (rn <= tol) {
+      st->conv[c] = 1;
+      m &= ~(1 << c);
+    }
+  }
+  st->mask = m;
+  st->zero = z;
+}
+
+__global__ void k3_zero_cols(int n, double* __restrict__ X, const Bicg3State* __restrict__ st) {
+  const int z = st->zero, i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!z || i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    if ((z >> c) & 1) X[3 * i + c] = 0.0;
+}
+
+// k3_fin<3> for r.r, then iteration 'it''s stop test per live column
+// (converged: ||s|| or ||r|| <= tol; dropped: rho, r0.v or omega = 0 or ||r||
+// not finite) and the publication of the outcome: pcg_word(stamp,
+// CONTINUE while a column is live else CONVERGED, max ||r||/tol over the
+// live columns as the host's speculation hint).
+__global__ __launch_bounds__(1024) void k3_fin_decide(int nparts,
+                                                      const double* __restrict__ partial,
+                                                      double* __restrict__ sc,
+                                                      Bicg3State* __restrict__ st, int it,
+                                                      int rhoNew, unsigned long long* word,
+                                                      unsigned stamp) {
+  __shared__ double sh[3 * 16];
+  const int mask = st->mask;
+  if (!mask) {  // nothing was live: announce the stop again
+    if (threadIdx.x == 0)
+      __hip_atomic_store(word, pcg_word(stamp, PCG_CONVERGED, 0.0f), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] += partial[(size_t)i * 3 + c];
+  block_sums<3>(v, sh);
+  if (threadIdx.x != 0) return;
+  int m = mask;
+  float worst = 0.0f;
+  for (int c = 0; c < 3; ++c) {
+    sc[3 * S_RR + c] = v[c];
+    if (!((mask >> c) & 1)) continue;
+    const double rn = sqrt(v[c]), tol = st->tol[c];
+    st->rn[c] = rn;
+    st->its[c] = it;
+    if (sqrt(sc[3 * S_SS + c]) <= tol || rn <= tol) {
+      st->conv[c] = 1;
+      m &= ~(1 << c);
+    } else if (sc[3 * rhoNew + c] == 0.0 || sc[3 * S_R0V + c] == 0.0 ||
+               sc[3 * S_OM + c] == 0.0 || !isfinite(rn)) {
+      m &= ~(1 << c);
+    } else {
+      worst = fmaxf(worst, tol > 0.0 ? (float)(rn / tol) : __builtin_huge_valf());
+    }
+  }
+  st->mask = m;
+  // release at system scope: the state written above is visible before
+  // the word that announces it
+  __hip_atomic_store(word, pcg_word(stamp, m ? PCG_CONTINUE : PCG_CONVERGED, worst),
+                     __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 void Bicg3Work::ensure(int nn, int nbSpmv, hipStream_t st) {
   const int need = std::max(nbSpmv, 1024) * 2 + 16;  // 6 doubles per block fit in 4 * need
   if (n == nn && scal.p && red.cap >= need) return;
   n = nn;
   red.init(need, st);
   scal.alloc(3 * S_COUNT);
+  state.alloc((sizeof(Bicg3State) + sizeof(double) - 1) / sizeof(double));
   dinv.alloc(n);
   for (DevBuf<double>* b : {&x, &b3, &r, &r0, &p, &v, &ph, &s_, &sh, &t}) b->alloc(3 * (size_t)n);
-  if (!hs) AN_HIP(hipHostMalloc((void**)&hs, sizeof(double) * 3 * S_COUNT));
+  if (!hs) AN_HIP(hipHostMalloc((void**)&hs, sizeof(Bicg3State)));
 }
 
 Bicg3Work::~Bicg3Work() {
   if (hs) (void)hipHostFree(hs);
+}
+
+// The host queues iteration k+1 before it waits for the published outcome
+// of iteration k unless the last two hints predict that k ends the solve
+// (the PCG rule with max ||r||/tol against 1); an iteration queued after
+// the last live column stopped finds the mask empty and every kernel
+// returns at once.
+static inline bool speculate3(double q, double qPrev) {
+  return !(qPrev > 0.0 && q * (q / qPrev) <= 1.0);
 }
 
 an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], double* const x[3],
@@ -394,104 +524,85 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   const int2* rbs = A.rb.start.p;
   double* sc = w.scal.p;
   double* part = w.red.partial.p;
-  an_solve_info inf[3] = {};
+  Bicg3State* st = reinterpret_cast<Bicg3State*>(w.state.p);
   k3_dinv<<<gN, kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
   k3_pack<<<gN, kB, 0, s>>>(n, b[0], b[1], b[2], w.b3.p);
   k3_pack<<<gN, kB, 0, s>>>(n, x[0], x[1], x[2], w.x.p);
   AN_CHECK_LAUNCH();
-  int mask = activeIn & 7;
-  // ||b||, r = b - A x, ||r||
-  k3_dot<<<Gd, kB, 0, s>>>(n, w.b3.p, w.b3.p, part);
-  finalize(w.red, Gd, 3, sc + 3 * S_BB, s);
+  // ||b||, r = b - A x, ||r||, the first stop test
+  k3_dot<<<Gd, kB, 0, s>>>(n, w.b3.p, w.b3.p, nullptr, part);
+  k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * S_BB, nullptr);
   k3_spmv<1, 2><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.x.p,
-                                   w.b3.p, w.r.p, nullptr, 7, part);
-  finalize(w.red, nbS, 3, sc + 3 * S_RR, s);
+                                   w.b3.p, w.r.p, nullptr, nullptr, part);
+  k3_fin<3><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_RR, nullptr);
+  k3_init<<<1, 64, 0, s>>>(sc, st, activeIn, relTol, absTol);
+  k3_zero_cols<<<gN, kB, 0, s>>>(n, w.x.p, st);
   AN_CHECK_LAUNCH();
-  AN_HIP(hipMemcpyAsync(w.hs, sc, sizeof(double) * 3 * S_COUNT, hipMemcpyDeviceToHost, s));
-  AN_HIP(hipStreamSynchronize(s));
-  double bnorm[3], rn[3];
-  Tol3 tol;
-  bool conv[3] = {false, false, false};
-  for (int c = 0; c < 3; ++c) {
-    bnorm[c] = std::sqrt(w.hs[3 * S_BB + c]);
-    rn[c] = std::sqrt(w.hs[3 * S_RR + c]);
-    tol.t[c] = std::fmax(relTol * bnorm[c], absTol);
-    if (!((mask >> c) & 1)) continue;
-    if (bnorm[c] == 0.0) {  // HYPRE convention: b = 0 -> x = 0
-      k3_zero_col<<<gN, kB, 0, s>>>(n, w.x.p, c);
-      conv[c] = true;
-      mask &= ~(1 << c);
-    } else if (rn[c] <= tol.t[c]) {
-      conv[c] = true;
-      mask &= ~(1 << c);
+  AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, s));
+  const unsigned base = w.stamp;
+  auto queue_iteration = [&](int it) {
+    const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
+    const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
+    k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, st, part);
+    k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * rhoNew, st);
+    k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
+                                it == 1 ? 1 : 0, st);
+    if (evStart && it == 1)
+      hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
+                            A.rb.count, rbs, (const int*)A.rowOff.p, (const int*)A.cols.p,
+                            (const double*)A.vals.p, (const double*)w.ph.p,
+                            (const double*)nullptr, w.v.p, (const double*)w.r0.p,
+                            (const Bicg3State*)st, part);
+    else
+      k3_spmv<0, 1><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.ph.p,
+                                       nullptr, w.v.p, w.r0.p, st, part);
+    k3_fin<3><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_R0V, st);
+    k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, sc, rhoNew, st, part);
+    k3_fin<3><<<1, 1024, 0, s>>>(G, part, sc + 3 * S_SS, st);
+    k3_spmv<0, 0><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
+                                     nullptr, w.t.p, nullptr, st, nullptr);
+    k3_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, st, part);
+    k3_fin<6><<<1, 1024, 0, s>>>(G, part, sc + 3 * S_TS, st);  // S_TS[0..2], S_TT[0..2]
+    k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, sc, st, part);
+    k3_fin_decide<<<1, 1024, 0, s>>>(G, part, sc, st, it, rhoNew, w.pub.d, base + (unsigned)it);
+    AN_CHECK_LAUNCH();
+  };
+  if (maxIter >= 1) {
+    queue_iteration(1);
+    double q = 0.0, qPrev = 0.0;
+    for (int it = 1;; ++it) {
+      const bool spec = it < maxIter && speculate3(q, qPrev);
+      if (spec) queue_iteration(it + 1);  // runs while the host waits
+      const unsigned long long word =
+          w.pub.wait(base + (unsigned)it, s, "BiCGSTAB iteration was not published");
+      qPrev = q;
+      q = PubWord::hint(word);
+      if (PubWord::code(word) != PCG_CONTINUE || it == maxIter) break;
+      if (!spec) queue_iteration(it + 1);
     }
-  }
-  const int solving = mask;
-  int it = 0;
-  if (mask) {
-    AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * 3 * (size_t)n,
-                          hipMemcpyDeviceToDevice, s));
-    for (it = 1; it <= maxIter && mask; ++it) {
-      const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
-      const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
-      k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, part);
-      finalize(w.red, Gd, 3, sc + 3 * rhoNew, s);
-      k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
-                                  it == 1 ? 1 : 0, mask);
-      if (evStart && it == 1)
-        hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
-                              A.rb.count, rbs, (const int*)A.rowOff.p, (const int*)A.cols.p,
-                              (const double*)A.vals.p, (const double*)w.ph.p,
-                              (const double*)nullptr, w.v.p, (const double*)w.r0.p, mask, part);
-      else
-        k3_spmv<0, 1><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.ph.p,
-                                         nullptr, w.v.p, w.r0.p, mask, part);
-      finalize(w.red, nbS, 3, sc + 3 * S_R0V, s);
-      k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, sc, rhoNew, mask,
-                                 part);
-      finalize(w.red, G, 3, sc + 3 * S_SS, s);
-      k3_spmv<0, 0><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
-                                       nullptr, w.t.p, nullptr, mask, nullptr);
-      k3_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, part);
-      finalize(w.red, G, 6, sc + 3 * S_TS, s);  // S_TS[0..2], S_TT[0..2]
-      k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, sc, tol, mask,
-                                 part);
-      finalize(w.red, G, 3, sc + 3 * S_RR, s);
-      AN_CHECK_LAUNCH();
-      AN_HIP(hipMemcpyAsync(w.hs, sc, sizeof(double) * 3 * S_COUNT, hipMemcpyDeviceToHost, s));
-      AN_HIP(hipStreamSynchronize(s));
-      const double* h = w.hs;
-      for (int c = 0; c < 3; ++c) {
-        if (!((mask >> c) & 1)) continue;
-        rn[c] = std::sqrt(h[3 * S_RR + c]);
-        inf[c].iterations = it;
-        if (std::sqrt(h[3 * S_SS + c]) <= tol.t[c] || rn[c] <= tol.t[c]) {
-          conv[c] = true;
-          mask &= ~(1 << c);
-        } else if (h[3 * rhoNew + c] == 0.0 || h[3 * S_R0V + c] == 0.0 ||
-                   h[3 * S_OM + c] == 0.0 || !std::isfinite(rn[c])) {
-          mask &= ~(1 << c);
-        }
-      }
-    }
+    w.stamp = base + (unsigned)maxIter + 1;  // past every stamp this solve may publish
   }
   k3_unpack<<<gN, kB, 0, s>>>(n, w.x.p, x[0], x[1], x[2]);
   AN_CHECK_LAUNCH();
+  AN_HIP(hipMemcpyAsync(w.hs, st, sizeof(Bicg3State), hipMemcpyDeviceToHost, s));
   AN_HIP(hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  an_status st = AN_OK;
+  const Bicg3State& h = *w.hs;
+  an_status status = AN_OK;
+  an_solve_info inf[3] = {};
   for (int c = 0; c < 3; ++c) {
     if (!((activeIn >> c) & 1)) continue;
-    if (!((solving >> c) & 1)) inf[c].iterations = 0;
-    inf[c].converged = conv[c] ? 1 : 0;
-    inf[c].finalResNorm = bnorm[c] == 0.0 ? 0.0 : rn[c];
-    inf[c].finalRelResNorm = bnorm[c] == 0.0 ? 0.0 : rn[c] / bnorm[c];
+    const bool zero = (h.zero >> c) & 1;
+    inf[c].iterations = h.its[c];
+    inf[c].converged = h.conv[c];
+    inf[c].finalResNorm = zero ? 0.0 : h.rn[c];
+    inf[c].finalRelResNorm = zero ? 0.0 : h.rn[c] / h.bnorm[c];
     inf[c].solveSeconds = secs;
-    if (!conv[c]) st = AN_ENOCONV;
+    if (!h.conv[c]) status = AN_ENOCONV;
   }
   if (info)
     for (int c = 0; c < 3; ++c) info[c] = inf[c];
-  return st;
+  return status;
 }
 
 }  // namespace an

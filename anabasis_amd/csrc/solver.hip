@@ -194,35 +194,41 @@ static inline bool speculate(double rn, double rnPrev, double tol) {
   return !(rnPrev > 0.0 && rn * (rn / rnPrev) <= tol);
 }
 
-PcgSolver::PubWord::PubWord() {
+PubWord::PubWord() {
   AN_HIP(hipHostMalloc((void**)&h, 64, hipHostMallocCoherent));
   __atomic_store_n(h, 0ull, __ATOMIC_RELEASE);
   AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
 }
-PcgSolver::PubWord::~PubWord() {
+PubWord::~PubWord() {
   if (h) (void)hipHostFree(h);
 }
 
-// A bounded spin on the stamp (acquire loads of the one published word);
-// past two seconds (a stalled or profiled device) a stream synchronisation,
-// after which the word must carry the stamp.
-PcgSolver::Published PcgSolver::wait_published(unsigned stamp) {
+unsigned long long PubWord::wait(unsigned stamp, hipStream_t s, const char* what) const {
   const unsigned want = stamp & 0x3fffffffu;
-  unsigned long long w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
+  unsigned long long w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
   const auto t0 = std::chrono::steady_clock::now();
   for (unsigned i = 1; (unsigned)(w >> 34) != want; ++i) {
     if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
       AN_HIP(hipStreamSynchronize(s));
-      w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
-      if ((unsigned)(w >> 34) != want) throw Error(7, "PCG iteration was not published");
+      w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
+      if ((unsigned)(w >> 34) != want) throw Error(7, what);
       break;
     }
-    w = __atomic_load_n(pub.h, __ATOMIC_ACQUIRE);
+    w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
   }
-  float rn;
+  return w;
+}
+
+double PubWord::hint(unsigned long long w) {
+  float v;
   const unsigned bits = (unsigned)(w & 0xffffffffu);
-  std::memcpy(&rn, &bits, sizeof rn);
-  return Published{(int)((w >> 32) & 3), (double)rn};
+  std::memcpy(&v, &bits, sizeof v);
+  return (double)v;
+}
+
+PcgSolver::Published PcgSolver::wait_published(unsigned stamp) {
+  const unsigned long long w = pub.wait(stamp, s, "PCG iteration was not published");
+  return Published{PubWord::code(w), PubWord::hint(w)};
 }
 
 // FP64 PCG; r must already hold b - A x and scal[S_RR] = r.r.  The residual

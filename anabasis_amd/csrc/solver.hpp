@@ -34,6 +34,26 @@ struct PinnedScalars {
   double local_[S_COUNT];
 };
 
+// One 64-bit word in coherent host memory that a kernel stores with a
+// system-scope release (pcg_word layout: stamp, decision, float hint); the
+// host reads it with acquire loads until the stamp it waits for shows up,
+// instead of a copy and a stream synchronisation per iteration.
+struct PubWord {
+  unsigned long long* h = nullptr;  // host view
+  unsigned long long* d = nullptr;  // device view of the same memory
+  PubWord();
+  ~PubWord();
+  PubWord(const PubWord&) = delete;
+  PubWord& operator=(const PubWord&) = delete;
+  // A bounded spin on the stamp; past two seconds (a stalled or profiled
+  // device) a synchronisation of s, after which the word must carry the
+  // stamp (else Error 7 naming 'what').
+  unsigned long long wait(unsigned stamp, hipStream_t s, const char* what) const;
+  // decision and float hint of a word
+  static int code(unsigned long long w) { return (int)((w >> 32) & 3); }
+  static double hint(unsigned long long w);
+};
+
 class PcgSolver {
  public:
   an_pcg_options opt;
@@ -85,20 +105,10 @@ class PcgSolver {
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
   double read_scalar(int slot);
-  // Each iteration's outcome comes back as one 64-bit word (pcg_word:
+  // Each iteration's outcome comes back as one published word (pcg_word:
   // stamp, device-side stop decision, float ||r|| hint) that the kernel
-  // finishing r.r stores with a system-scope release into coherent host
-  // memory; the host reads it with acquire loads until the stamp is the
-  // iteration's.  The FP64 residual norms stay in hist (device) and are
-  // copied once per solve.
-  struct PubWord {
-    unsigned long long* h = nullptr;
-    unsigned long long* d = nullptr;
-    PubWord();
-    ~PubWord();
-    PubWord(const PubWord&) = delete;
-    PubWord& operator=(const PubWord&) = delete;
-  };
+  // finishing r.r stores; the FP64 residual norms stay in hist (device) and
+  // are copied once per solve.
   PubWord pub;
   unsigned pubStamp = 0;
   DevBuf<double> hist;
@@ -136,10 +146,26 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
 // Three right-hand sides sharing one matrix (interleaved work vectors); per
 // column bit-identical to bicgstab_device.  activeMask selects the columns
 // to solve; evStart/evStop (optional) time the first fused SpMV.
+//
+// The stop tests run on the device: a one-thread step after each iteration
+// updates the active-column mask that every kernel of the next iteration
+// reads, and publishes whether any column is left (PubWord), so the host
+// queues iteration k+1 before it knows the outcome of iteration k and never
+// copies scalars back in the loop.
+struct Bicg3State {
+  int mask;      // columns still iterating
+  int zero;      // columns with b = 0 (x set to 0)
+  int its[3];    // iterations done per column
+  int conv[3];   // converged per column
+  double tol[3], rn[3], bnorm[3];
+};
 struct Bicg3Work {
   Reducer red;
   DevBuf<double> scal, dinv, x, b3, r, r0, p, v, ph, s_, sh, t;
-  double* hs = nullptr;  // pinned copy of scal
+  DevBuf<double> state;          // a Bicg3State
+  Bicg3State* hs = nullptr;      // pinned copy of state, read once per solve
+  PubWord pub;
+  unsigned stamp = 0;
   int n = 0;
   void ensure(int n, int nbSpmv, hipStream_t s);
   Bicg3Work() = default;
