@@ -526,6 +526,9 @@ an_status an_pcg_solve(an_solver_t s, const double* b, double* x, an_memspace me
     xd = xh.p;
   }
   an_status st = s->pcg->solve(bb.p, xd, info);
+  // the solver returns with its last kernels queued; x is complete when
+  // this call returns, as after HYPRE's solve
+  AN_HIP(hipStreamSynchronize(s->pcg->s));
   if (mem == AN_HOST) AN_HIP(hipMemcpy(x, xd, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (st == AN_ENOCONV) set_last_error("PCG did not converge");
   return st;

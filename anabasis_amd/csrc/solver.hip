@@ -180,12 +180,6 @@ void PcgSolver::precondition(bool fp32, const ApplyDots& d) {
   AN_HIP(hipGraphLaunch(it->second, s));
 }
 
-double PcgSolver::read_scalar(int slot) {
-  AN_HIP(hipMemcpyAsync(hs.h + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
-  AN_HIP(hipStreamSynchronize(s));
-  return hs.h[slot];
-}
-
 // The next preconditioner application is queued before the host knows
 // ||r|| (hiding the round trip) unless the last two residual norms predict
 // convergence in this iteration: then an application would be wasted work
@@ -195,9 +189,11 @@ static inline bool speculate(double rn, double rnPrev, double tol) {
 }
 
 PubWord::PubWord() {
-  AN_HIP(hipHostMalloc((void**)&h, 64, hipHostMallocCoherent));
+  AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * (1 + kValues), hipHostMallocCoherent));
   __atomic_store_n(h, 0ull, __ATOMIC_RELEASE);
   AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
+  vh = reinterpret_cast<double*>(h + 1);
+  vd = reinterpret_cast<double*>(d + 1);
 }
 PubWord::~PubWord() {
   if (h) (void)hipHostFree(h);
@@ -226,6 +222,42 @@ double PubWord::hint(unsigned long long w) {
   return (double)v;
 }
 
+void HostDoubles::ensure(size_t n, hipStream_t s) {
+  if (n <= cap) return;
+  if (h) {
+    AN_HIP(hipStreamSynchronize(s));
+    (void)hipHostFree(h);
+    h = d = nullptr;
+    cap = 0;
+  }
+  const size_t c = std::max(n, (size_t)256);
+  AN_HIP(hipHostMalloc((void**)&h, sizeof(double) * c, hipHostMallocCoherent));
+  AN_HIP(hipHostGetDevicePointer((void**)&d, h, 0));
+  cap = c;
+}
+HostDoubles::~HostDoubles() {
+  if (h) (void)hipHostFree(h);
+}
+
+// dst[0 .. count) = src[0 .. count), then the stamp (system-scope release:
+// the values are visible before the word)
+__global__ void k_pub_values(const double* __restrict__ src, int count, double* dst,
+                             unsigned long long* word, unsigned stamp) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < count; ++i) dst[i] = src[i];
+  __hip_atomic_store(word, pcg_word(stamp, PCG_CONTINUE, 0.0f), __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+const double* PcgSolver::fetch_scalars(int slot, int count) {
+  if (count > PubWord::kValues) throw Error(7, "too many published values");
+  const unsigned stamp = ++pubStamp;
+  k_pub_values<<<1, 64, 0, s>>>(scal.p + slot, count, pub.vd, pub.d, stamp);
+  AN_CHECK_LAUNCH();
+  (void)pub.wait(stamp, s, "solver scalars were not published");
+  return pub.vh;
+}
+
 PcgSolver::Published PcgSolver::wait_published(unsigned stamp) {
   const unsigned long long w = pub.wait(stamp, s, "PCG iteration was not published");
   return Published{PubWord::code(w), PubWord::hint(w)};
@@ -236,10 +268,9 @@ PcgSolver::Published PcgSolver::wait_published(unsigned stamp) {
 int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut) {
   (void)b;
   const int n = A.n;
-  double rn = std::sqrt(hs.h[S_RR]);
-  *rnormOut = rn;
+  double rn = *rnormOut;
   if (rn <= tol) return 0;
-  hist.ensure((size_t)maxIter + 2);
+  hist.ensure((size_t)maxIter + 2, s);
   // flexible CG (K-cycle preconditioner): alpha = r.z / p.q with r.z in
   // S_FRZ, beta = -(z.q)/(p.q) (Polak-Ribiere: z.(r_new - r_old) / r_old.z_old
   // with r_new - r_old = -alpha q); otherwise beta = rho_new / rho_old
@@ -291,15 +322,10 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     it = maxIter;
     ret = -maxIter;
   }
-  AN_HIP(hipStreamSynchronize(s));
-  // the FP64 residual norms of iterations 1..it, copied once
-  const size_t old = history.size();
-  history.resize(old + (size_t)it);
-  if (it > 0) {
-    AN_HIP(hipMemcpy(history.data() + old, hist.p + 1, sizeof(double) * (size_t)it,
-                     hipMemcpyDeviceToHost));
-    *rnormOut = history.back();
-  }
+  // the FP64 residual norms of iterations 1..it: the host has seen every
+  // iteration's word, which the kernel stored after hist[it]
+  history.insert(history.end(), hist.h + 1, hist.h + 1 + it);
+  if (it > 0) *rnormOut = history.back();
   return ret;
 }
 
@@ -308,7 +334,7 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
                      double* rnormOut) {
   const int n = A.n;
   const DevCSR<float>& A32 = amg.lv[0].A32;
-  hist.ensure((size_t)maxIter + 2);
+  hist.ensure((size_t)maxIter + 2, s);
   fill_zero(x, sizeof(float) * (size_t)n, s);
   AN_HIP(hipMemcpyAsync(r32.p, b, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
   const bool flex = amg.flexible();
@@ -347,8 +373,7 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
     if (!spec) next_direction();
   }
   if (it > maxIter) it = maxIter;
-  AN_HIP(hipStreamSynchronize(s));
-  if (it > 0) AN_HIP(hipMemcpy(rnormOut, hist.p + it, sizeof(double), hipMemcpyDeviceToHost));
+  if (it > 0) *rnormOut = hist.h[it];  // published with iteration it's word
   return it;
 }
 
@@ -476,12 +501,11 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
   dot<double>(n, b, b, scal.p, S_BB, red, s);
   residual<double, double>(A, b, x, r.p, scal.p, red, s);
   project_initial(x);
-  AN_HIP(hipMemcpyAsync(hs.h + S_RR, scal.p + S_RR, (S_BB - S_RR + 1) * sizeof(double),
-                        hipMemcpyDeviceToHost, s));
-  AN_HIP(hipStreamSynchronize(s));
-  const double bnorm = std::sqrt(hs.h[S_BB]);
+  const double* sv = fetch_scalars(S_RR, S_BB - S_RR + 1);  // S_RR .. S_BB
+  const double bnorm = std::sqrt(sv[S_BB - S_RR]);
+  const double rn0 = std::sqrt(sv[0]);
   history.clear();
-  history.push_back(std::sqrt(hs.h[S_RR]));
+  history.push_back(rn0);
   an_solve_info inf{};
   inf.levels = amg.levels();
   inf.setupSeconds = lastSetupSeconds;
@@ -492,7 +516,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
     inf.converged = 1;
   } else {
     const double tol = std::fmax(opt.relTol * bnorm, opt.absTol);
-    double rn = std::sqrt(hs.h[S_RR]);
+    double rn = rn0;
     if (opt.precision != 1) {
       int it = pcg64(b, x, tol, opt.maxIter, &rn);
       inf.iterations = it < 0 ? -it : it;
@@ -512,8 +536,7 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
         total += it;
         axpy_f2d(n, x, e32.p, s);
         residual<double, double>(A, b, x, r.p, scal.p, red, s);
-        read_scalar(S_RR);
-        double rnNew = std::sqrt(hs.h[S_RR]);
+        double rnNew = std::sqrt(*fetch_scalars(S_RR, 1));
         history.push_back(rnNew);
         ++steps;
         if (!std::isfinite(rnNew) || (steps > 3 && rnNew > 0.9 * rn)) {

@@ -41,6 +41,11 @@ struct PinnedScalars {
 struct PubWord {
   unsigned long long* h = nullptr;  // host view
   unsigned long long* d = nullptr;  // device view of the same memory
+  // kValues doubles after the word (same allocation) for values that a
+  // kernel publishes with the word
+  static constexpr int kValues = 15;
+  double* vh = nullptr;
+  double* vd = nullptr;
   PubWord();
   ~PubWord();
   PubWord(const PubWord&) = delete;
@@ -52,6 +57,21 @@ struct PubWord {
   // decision and float hint of a word
   static int code(unsigned long long w) { return (int)((w >> 32) & 3); }
   static double hint(unsigned long long w);
+};
+
+// Doubles in coherent host memory that kernels write directly (a residual
+// history read by the host after the published word that covers it).
+struct HostDoubles {
+  double* h = nullptr;
+  double* d = nullptr;
+  size_t cap = 0;
+  // grows (after synchronising s, as kernels may still hold the old
+  // address) to at least n values
+  void ensure(size_t n, hipStream_t s);
+  HostDoubles() = default;
+  ~HostDoubles();
+  HostDoubles(const HostDoubles&) = delete;
+  HostDoubles& operator=(const HostDoubles&) = delete;
 };
 
 class PcgSolver {
@@ -104,14 +124,16 @@ class PcgSolver {
   int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
   // rn0: norm of b (the outer residual) for the speculation rule
   int pcg32(const float* b, float* x, double tol, int maxIter, double rn0, double* rnormOut);
-  double read_scalar(int slot);
   // Each iteration's outcome comes back as one published word (pcg_word:
   // stamp, device-side stop decision, float ||r|| hint) that the kernel
-  // finishing r.r stores; the FP64 residual norms stay in hist (device) and
-  // are copied once per solve.
+  // finishing r.r stores after it has written the FP64 ||r|| into hist
+  // (coherent host memory): the host reads both without a copy.
   PubWord pub;
   unsigned pubStamp = 0;
-  DevBuf<double> hist;
+  HostDoubles hist;
+  // scal[slot .. slot + count) to the host through the published word (a
+  // one-thread kernel, then a spin on its stamp); returns the host copy
+  const double* fetch_scalars(int slot, int count);
   struct Published {
     int code;
     double rnHint;
@@ -122,7 +144,7 @@ class PcgSolver {
     a.word = pub.d;
     a.stamp = ++pubStamp;
     a.tol = tol;
-    a.hist = hist.p;
+    a.hist = hist.d;
     a.idx = idx;
     return a;
   }
