@@ -486,6 +486,55 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
   yc[I] = (T)((double)omdc[I] * (double)bI);
 }
 
+// The same aggregate sums over row blocks of the member lists (moff as row
+// offsets, mem as columns; one wave per block): the lanes first load all of
+// the block's members and their r values (coalesced member reads, one
+// gather round trip), stage them in the wave's LDS slice, then lane I adds
+// aggregate I's values in member order -- k_restrict's order, so the same
+// bits, without one thread walking a whole aggregate of gathers.
+template <typename T>
+__global__ __launch_bounds__(kB) void k_restrict_rb(int nrb, const int2* __restrict__ rbs,
+                                                    const int* __restrict__ moff,
+                                                    const int* __restrict__ mem,
+                                                    const T* __restrict__ r,
+                                                    const T* __restrict__ omdc,
+                                                    T* __restrict__ bc, T* __restrict__ yc) {
+  __shared__ double vals[kW][kCap];
+  const int lb = spread_block((nrb + kW - 1) / kW);
+  if (lb < 0) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
+  const int I = q.r0 + lane;
+  int kb = 0, ke = 0;
+  if (I < q.r1) {
+    kb = moff[I];
+    ke = moff[I + 1];
+  }
+  if (q.stream && q.k1 > q.k0) {
+    int m[kStreamPer];
+    double v[kStreamPer];
+#pragma unroll
+    for (int u = 0; u < kStreamPer; ++u) m[u] = mem[min(q.k0 + lane + 64 * u, q.k1 - 1)];
+#pragma unroll
+    for (int u = 0; u < kStreamPer; ++u) v[u] = (double)r[m[u]];
+#pragma unroll
+    for (int u = 0; u < kStreamPer; ++u) {
+      const int k = q.k0 + lane + 64 * u;
+      if (k < q.k1) vals[w][k - q.k0] = v[u];
+    }
+  }
+  wave_sync();
+  if (I >= q.r1) return;
+  double s = 0.0;
+  if (q.stream)
+    for (int k = kb; k < ke; ++k) s += vals[w][k - q.k0];
+  else
+    for (int k = kb; k < ke; ++k) s += (double)r[mem[k]];
+  const T bI = (T)s;
+  bc[I] = bI;
+  yc[I] = (T)((double)omdc[I] * (double)bI);
+}
+
 // e_i = y_i + xc[agg_i]  (e may be y: an in-place, per-element update).
 // KC: the coarse level took the K-cycle correction; xc holds its second
 // step c2 and the correction (T)((a1 - a2 g/rho1) c1 + a2 c2) is formed
@@ -910,6 +959,7 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
     L.nc = cur;
     L.agg = std::move(agg);
     build_members(L.n, cur, L.agg.p, L.moff, L.mem, s);
+    build_row_blocks(cur, L.moff.p, L.mrb, s);
     lv.emplace_back();
     AmgLevel& C = lv.back();
     C.n = cur;
@@ -1102,8 +1152,12 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   T* res = (xs == r) ? x : r;
   k_level_rows<T, 0><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
                                       D::omd(L), b, xs, res, qv, noRed);
-  k_restrict<T><<<grid_for(L.nc, kB) * spread_of(L.nc), kB, 0, s>>>(L.nc, L.moff.p, L.mem.p, res, D::omd(C),
-                                                  D::b(C), D::y(C));
+  if (L.mrb.count > 0)
+    k_restrict_rb<T><<<grid_for(L.mrb.count, kW) * spread_of(L.nc), kB, 0, s>>>(
+        L.mrb.count, L.mrb.start.p, L.moff.p, L.mem.p, res, D::omd(C), D::b(C), D::y(C));
+  else
+    k_restrict<T><<<grid_for(L.nc, kB) * spread_of(L.nc), kB, 0, s>>>(
+        L.nc, L.moff.p, L.mem.p, res, D::omd(C), D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
   // e = x_pre + P xc (with a K-cycle level's combination formed here),
   // then the post-smoothing sweeps ping-pong between r and x; e starts in

@@ -21,6 +21,7 @@ struct AmgLevel {
   const DevCSR<double>* A = nullptr;
   DevBuf<double> omdinv;        // omega / a_ii
   DevBuf<int> agg, moff, mem;   // aggregation to level+1
+  RowBlocks mrb;                // row blocks of (moff, mem) for the restriction
   DevBuf<double> coarseInvT;    // coarsest: dense inverse, column-major
   DevBuf<double> b, x, r, y;    // work vectors (FP64); y = omdinv o b
   // FP32 copies for the mixed-precision V-cycle
