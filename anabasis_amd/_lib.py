@@ -116,6 +116,7 @@ SIGNATURES = {
     "an_pcg_create": (_I, [_I, _P, _P, C.c_int64, C.POINTER(PcgOptions), _I, C.POINTER(_P)]),
     "an_pcg_set_values": (_I, [_P, _P, _I, _I]),
     "an_pcg_solve": (_I, [_P, _P, _P, _I, C.POINTER(SolveInfo)]),
+    "an_pcg_set_stream": (_I, [_P, _P]),
     "an_pcg_residual_history": (_I, [_P, _P, _I, C.POINTER(C.c_int)]),
     "an_spmv": (_I, [_P, _P, _P, _I]),
     "an_amg_apply": (_I, [_P, _P, _P, _I]),

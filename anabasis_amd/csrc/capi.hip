@@ -506,9 +506,10 @@ an_status an_pcg_create(int n, const int* rowOffsets, const int* cols, int64_t n
 an_status an_pcg_set_values(an_solver_t s, const double* values, int rebuild, an_memspace mem) {
   AN_TRY
   if (!s || !values) throw Error(AN_EINVAL, "NULL argument");
-  InArr<double> v(values, (size_t)s->pcg->A.nnz, mem, 0);
+  const hipStream_t st = s->pcg->s;
+  InArr<double> v(values, (size_t)s->pcg->A.nnz, mem, st);
   s->pcg->set_values_device(v.p, rebuild != 0);
-  AN_HIP(hipStreamSynchronize(0));
+  AN_HIP(hipStreamSynchronize(st));
   return AN_OK;
   AN_CATCH
 }
@@ -518,20 +519,31 @@ an_status an_pcg_solve(an_solver_t s, const double* b, double* x, an_memspace me
   AN_TRY
   if (!s || !b || !x) throw Error(AN_EINVAL, "NULL argument");
   const int n = s->pcg->A.n;
-  InArr<double> bb(b, (size_t)n, mem, 0);
+  const hipStream_t hs = s->pcg->s;
+  InArr<double> bb(b, (size_t)n, mem, hs);
   DevBuf<double> xh;
   double* xd = x;
   if (mem == AN_HOST) {
-    xh.upload(x, (size_t)n, 0);
+    xh.upload(x, (size_t)n, hs);
     xd = xh.p;
   }
   an_status st = s->pcg->solve(bb.p, xd, info);
   // the solver returns with its last kernels queued; x is complete when
   // this call returns, as after HYPRE's solve
-  AN_HIP(hipStreamSynchronize(s->pcg->s));
-  if (mem == AN_HOST) AN_HIP(hipMemcpy(x, xd, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (mem == AN_HOST)
+    AN_HIP(hipMemcpyAsync(x, xd, sizeof(double) * n, hipMemcpyDeviceToHost, hs));
+  AN_HIP(hipStreamSynchronize(hs));
   if (st == AN_ENOCONV) set_last_error("PCG did not converge");
   return st;
+  AN_CATCH
+}
+
+an_status an_pcg_set_stream(an_solver_t s, void* hipStream) {
+  AN_TRY
+  if (!s) throw Error(AN_EINVAL, "NULL argument");
+  AN_HIP(hipStreamSynchronize(s->pcg->s));
+  s->pcg->s = static_cast<hipStream_t>(hipStream);
+  return AN_OK;
   AN_CATCH
 }
 
@@ -549,11 +561,11 @@ an_status an_spmv(an_solver_t s, const double* x, double* y, an_memspace mem) {
   AN_TRY
   if (!s || !x || !y) throw Error(AN_EINVAL, "NULL argument");
   PcgSolver& P = *s->pcg;
-  InArr<double> xx(x, (size_t)P.A.n, mem, 0);
+  InArr<double> xx(x, (size_t)P.A.n, mem, P.s);
   OutArr<double> yy(y, (size_t)P.A.n, mem);
-  spmv<double, double>(P.A, xx.p, yy.p, nullptr, P.scal.p, 0, P.red, 0);
-  yy.finish(0);
-  AN_HIP(hipStreamSynchronize(0));
+  spmv<double, double>(P.A, xx.p, yy.p, nullptr, P.scal.p, 0, P.red, P.s);
+  yy.finish(P.s);
+  AN_HIP(hipStreamSynchronize(P.s));
   return AN_OK;
   AN_CATCH
 }
@@ -563,11 +575,11 @@ an_status an_amg_apply(an_solver_t s, const double* r, double* z, an_memspace me
   if (!s || !r || !z) throw Error(AN_EINVAL, "NULL argument");
   PcgSolver& P = *s->pcg;
   if (!P.amg.built) throw Error(AN_EINVAL, "AMG hierarchy not built");
-  InArr<double> rr(r, (size_t)P.A.n, mem, 0);
+  InArr<double> rr(r, (size_t)P.A.n, mem, P.s);
   OutArr<double> zz(z, (size_t)P.A.n, mem);
   P.apply_prec(rr.p, zz.p);
-  zz.finish(0);
-  AN_HIP(hipStreamSynchronize(0));
+  zz.finish(P.s);
+  AN_HIP(hipStreamSynchronize(P.s));
   return AN_OK;
   AN_CATCH
 }

@@ -694,6 +694,34 @@ __global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
   store1(b, partial);
 }
 
+// finalize of r.r (k_finalize_nc<1>'s loads and tree, so the same bits),
+// then the iteration's stop test -- converged if ||s|| or ||r|| <= tol,
+// breakdown if rho, r0.v or omega = 0 or ||r|| is not finite -- published
+// with the FP64 ||r|| in the word's first value
+__global__ __launch_bounds__(1024) void k_bicg_fin_decide(int nparts,
+                                                          const double* __restrict__ partial,
+                                                          double* __restrict__ scal, int rhoNew,
+                                                          double tol, double* val,
+                                                          unsigned long long* word,
+                                                          unsigned stamp) {
+  __shared__ double sh[16];
+  double v[1] = {0.0};
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) v[0] += partial[i];
+  block_sums<1>(v, sh);
+  if (threadIdx.x != 0) return;
+  scal[S_RR] = v[0];
+  const double rn = sqrt(v[0]);
+  const int code = (sqrt(scal[S_SS]) <= tol || rn <= tol) ? PCG_CONVERGED
+                   : (scal[rhoNew] == 0.0 || scal[S_R0V] == 0.0 || scal[S_OM] == 0.0 ||
+                      !isfinite(rn))
+                       ? PCG_BREAKDOWN
+                       : PCG_CONTINUE;
+  val[0] = rn;
+  __hip_atomic_store(word, pcg_word(stamp, code, (float)rn), __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,
                           double relTol, double absTol, BicgWork& w, hipStream_t s,
                           an_solve_info* info) {
@@ -706,10 +734,23 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
   AN_CHECK_LAUNCH();
   dot<double>(n, b, b, w.scal.p, S_BB, w.red, s);
   residual<double, double>(A, b, x, w.r.p, w.scal.p, w.red, s);
-  AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
-  AN_HIP(hipStreamSynchronize(s));
-  const double bnorm = std::sqrt(w.hs.h[S_BB]);
-  double rn = std::sqrt(w.hs.h[S_RR]);
+  PubWord* pw = w.pub.get();
+  double bb, rr;
+  if (pw) {  // S_RR .. S_BB through the word
+    const unsigned st0 = ++w.stamp;
+    k_pub_values<<<1, 64, 0, s>>>(w.scal.p + S_RR, S_BB - S_RR + 1, pw->vd, pw->d, st0);
+    AN_CHECK_LAUNCH();
+    (void)pw->wait(st0, s, "BiCGSTAB scalars were not published");
+    rr = pw->vh[0];
+    bb = pw->vh[S_BB - S_RR];
+  } else {
+    AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+    AN_HIP(hipStreamSynchronize(s));
+    rr = w.hs.h[S_RR];
+    bb = w.hs.h[S_BB];
+  }
+  const double bnorm = std::sqrt(bb);
+  double rn = std::sqrt(rr);
   an_status st = AN_OK;
   if (bnorm == 0.0) {
     fill_zero(x, sizeof(double) * (size_t)n, s);
@@ -742,6 +783,20 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
       k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, w.scal.p, tol,
                                 w.red.partial.p);
       AN_CHECK_LAUNCH();
+      if (pw) {
+        const unsigned sti = ++w.stamp;
+        k_bicg_fin_decide<<<1, 1024, 0, s>>>(G, w.red.partial.p, w.scal.p, rhoNew, tol, pw->vd,
+                                             pw->d, sti);
+        AN_CHECK_LAUNCH();
+        const unsigned long long word = pw->wait(sti, s, "BiCGSTAB iteration was not published");
+        rn = pw->vh[0];
+        if (PubWord::code(word) == PCG_CONVERGED) {
+          conv = true;
+          break;
+        }
+        if (PubWord::code(word) == PCG_BREAKDOWN) break;
+        continue;
+      }
       finalize(w.red, G, 1, w.scal.p + S_RR, s);
       AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
       AN_HIP(hipStreamSynchronize(s));

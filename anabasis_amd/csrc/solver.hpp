@@ -2,6 +2,7 @@
 #pragma once
 
 #include <map>
+#include <memory>
 
 #include "../../include/anabasis_hip.h"
 #include "amg.hpp"
@@ -153,12 +154,20 @@ class PcgSolver {
 };
 
 // One-shot Jacobi-preconditioned BiCGSTAB on device data (x holds x0).
+// With pinned memory (the default) each iteration's stop test runs in the
+// kernel that finishes r.r and comes back through a published word; without
+// (pinnedScalars = false: callers that may not allocate) the scalars are
+// copied to a plain host array and the host decides, with the same result.
 struct BicgWork {
   Reducer red;
   DevBuf<double> scal, dinv, r, r0, p, v, ph, s_, sh, t;
   PinnedScalars hs;
+  std::unique_ptr<PubWord> pub;
+  unsigned stamp = 0;
   int n = 0;
-  explicit BicgWork(bool pinnedScalars = true) : hs(pinnedScalars) {}
+  explicit BicgWork(bool pinnedScalars = true) : hs(pinnedScalars) {
+    if (pinnedScalars) pub.reset(new PubWord());
+  }
   void ensure(int n, int nbSpmv, hipStream_t s);
 };
 an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, int maxIter,

@@ -215,6 +215,15 @@ an_status an_pcg_set_values(an_solver_t s, const double *values,
 /* Solves A x = b; x holds the initial guess on entry. */
 an_status an_pcg_solve(an_solver_t s, const double *b, double *x,
                        an_memspace mem, an_solve_info *info);
+/* Runs the handle's later work (an_pcg_set_values, an_pcg_solve, an_spmv,
+ * an_amg_apply) on the caller's HIP stream (a hipStream_t; NULL = the null
+ * stream, the default).  Device-pointer inputs are then read in that
+ * stream's order, so a caller need not synchronise before a call; each call
+ * still returns with its results complete, as the reference's synchronous
+ * HYPRE calls do (LP/src/hypre_backend.cu:35-114 on the legacy default
+ * stream).  Work already queued on the previous stream is finished first.
+ * The stream must outlive its use by the handle. */
+an_status an_pcg_set_stream(an_solver_t s, void *hipStream);
 /* Residual 2-norms of the last an_pcg_solve: ||r0|| and one per PCG
  * iteration (precision 1: ||r0|| and the FP64 residual after each
  * refinement step), the values HYPRE prints at print level 2 when the

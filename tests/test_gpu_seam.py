@@ -75,6 +75,44 @@ def test_pcg_device_pointers_match_host(gpu):
     assert info.finalResNorm == ih.finalResNorm
 
 
+def test_pcg_on_caller_stream_matches_host(gpu):
+    """an_pcg_set_stream: inputs produced on the caller's stream and handed
+    over without a synchronisation are read in that stream's order."""
+    import torch
+    om, vals, b = _pressure_system(seed=4)
+    n, nnz = om.nCells, om.nnz
+    S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0)
+    S.set_values(vals)
+    xh, ih = S.solve(b)
+    ro, ci = _dev(torch, om.rowOffsets, torch.int32), _dev(torch, om.cols, torch.int32)
+    va0, b0 = _dev(torch, vals, torch.float64), _dev(torch, b, torch.float64)
+    opt = lp.pcg_options(relTol=1e-10, absTol=0.0)
+    h = C.c_void_p()
+    torch.cuda.synchronize()
+    L.check(L.lib().an_pcg_create(n, ro.data_ptr(), ci.data_ptr(), nnz, C.byref(opt),
+                                  L.AN_DEVICE, C.byref(h)))
+    st = torch.cuda.Stream()
+    try:
+        L.check(L.lib().an_pcg_set_stream(h, C.c_void_p(st.cuda_stream)))
+        with torch.cuda.stream(st):
+            # queued on st, not synchronised before the calls below
+            va = (va0 * 2.0) * 0.5  # exact: powers of two
+            bd = b0.clone()
+            xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        L.check(L.lib().an_pcg_set_values(h, va.data_ptr(), 1, L.AN_DEVICE))
+        info = L.SolveInfo()
+        L.check(L.lib().an_pcg_solve(h, bd.data_ptr(), xd.data_ptr(), L.AN_DEVICE,
+                                     C.byref(info)))
+        out = xd.cpu().numpy()  # the call returned with x complete
+        L.check(L.lib().an_pcg_set_stream(h, None))
+    finally:
+        L.lib().an_solver_destroy(h)
+    torch.cuda.synchronize()
+    assert np.array_equal(va.cpu().numpy(), vals)
+    assert info.converged and info.iterations == ih.iterations > 3
+    assert np.array_equal(out, xh)
+
+
 def test_bicgstab_device_pointers_match_host(gpu):
     import torch
     m = lp.Mesh.hexbox(18, 16, 14, jitter=0.2, seed=5)
