@@ -6,7 +6,7 @@
 //   k_mom<FULL>      momentum matrix + rhs (main.cu:1673-1896), relaxation
 //                    (:1686-1695 / :1787-1794) and rAU (:1796-1802) fused,
 //                    one thread per row walking its faces in face order
-//   k_p_asm          pressure matrix from rAU (:1898-1950)
+//   k_p_face/k_p_rows pressure matrix from rAU (:1898-1950)
 //   k_rc             Rhie-Chow predicted flux (:1474-1521), one thread/face
 //   k_nonorth_flux   non-orthogonal pressure flux (:1538-1593), per face
 //   k_div            continuity / non-orthogonal divergence (+ pressure rhs
@@ -311,62 +311,98 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
 }
 
 // ----------------------------------------------------------- pressure ----
-__global__ __launch_bounds__(kB) void k_p_asm(MeshView m, const int* __restrict__ bcP,
-                                              const double* __restrict__ rAU,
-                                              double* __restrict__ vals) {
-  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= m.nC) return;
-  double diag = 0.0;
-  // faces four at a time: incidence entries, then their face data, then the
-  // rAU gathers, then the terms in incidence order (as one face at a time)
-  constexpr int U = 4;
-  const int k1 = m.cfOff[c + 1];
-  for (int k0 = m.cfOff[c]; k0 < k1; k0 += U) {
-    int fu[U], su[U];
+// Pressure matrix in two passes.  k_p_face (one thread per internal face,
+// unit-stride face data) computes the face coefficient A rAU_f / dpn once
+// -- 0 for a face the reference skips (dpn <= 1e-14), which leaves the sums
+// below bit for bit unchanged: d + 0.0 = d for the diagonal (it starts at
+// +0.0 and never becomes -0.0) and old + (-0.0) = old for a slot -- and
+// k_correct_flux reuses it.  k_p_rows (one wave per CSR row block) stages
+// the block's incidence entries and their coefficients in LDS with
+// coalesced loads, lets lane i sum row i in incidence order (the
+// reference's face order) into an LDS image of the block's matrix values,
+// and stores that image with unit-stride writes.  A block whose nonzeros or
+// entries exceed the LDS slices walks global memory instead (same order).
+__global__ __launch_bounds__(kB) void k_p_face(MeshView m, const double* __restrict__ rAU,
+                                               double* __restrict__ coef) {
+  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+  if (f >= m.nIF) return;
+  const double dpn = m.fDpn[f];
+  double c = 0.0;
+  if (!(dpn <= 1e-14)) {
+    const int2 pn = m.fPN[f];
+    const double lam = m.fLam[f];
+    const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
+    c = m.fA[f] * rAUf / dpn;
+  }
+  coef[f] = c;
+}
+
+static constexpr int kW = kB / 64;
+static constexpr int kEnt = 512;  // staged incidence entries per wave
+__global__ __launch_bounds__(kB) void k_p_rows(MeshView m, int nrb, const int2* __restrict__ rbs,
+                                               const int* __restrict__ bcP,
+                                               const double* __restrict__ rAU,
+                                               const double* __restrict__ coef,
+                                               double* __restrict__ vals) {
+  __shared__ double sv[kW][kStreamCap];  // the block's matrix values
+  __shared__ double ec[kW][kEnt];        // coefficient of each incidence entry
+  __shared__ int es[kW][kEnt];           // its cfSlot
+  const int lb = xcd_block(gridDim.x);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
+  if (q.r1 <= q.r0) return;  // wave-uniform; no block barrier below
+  const int c = q.r0 + lane;
+  const int E0 = m.cfOff[q.r0], E1 = m.cfOff[q.r1];
+  const bool staged = q.stream && (E1 - E0) <= kEnt;
+  if (staged && E1 > E0) {
+    constexpr int P = kEnt / 64;
+    int ent[P], slt[P];
+    double cv[P];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = min(k0 + u, k1 - 1);
-      fu[u] = m.cfEnt[k] >> 1;
-      su[u] = m.cfSlot[k];
+    for (int u = 0; u < P; ++u) {
+      const int k = min(E0 + lane + 64 * u, E1 - 1);
+      ent[u] = m.cfEnt[k];
+      slt[u] = m.cfSlot[k];
     }
-    double dpn[U], lam[U], area[U];
-    int2 pn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      dpn[u] = m.fDpn[fu[u]];
-      lam[u] = m.fLam[fu[u]];
-      pn[u] = m.fPN[fu[u]];
-      area[u] = m.fA[fu[u]];
-    }
-    double rP[U], rN[U];
+    for (int u = 0; u < P; ++u) cv[u] = coef[ent[u] >> 1];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      rP[u] = rAU[pn[u].x];
-      rN[u] = rAU[pn[u].y];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (k0 + u >= k1) break;
-      const int slotInfo = su[u];
-      if (dpn[u] <= 1e-14) {
-        if (slotInfo & 1) vals[slotInfo >> 1] = 0.0;
-        continue;
+    for (int u = 0; u < P; ++u) {
+      const int k = E0 + lane + 64 * u;
+      if (k < E1) {
+        ec[w][k - E0] = cv[u];
+        es[w][k - E0] = slt[u];
       }
-      const double rAUf = (1.0 - lam[u]) * rP[u] + lam[u] * rN[u];
-      const double coeff = area[u] * rAUf / dpn[u];
-      diag += coeff;
-      const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
-      vals[slotInfo >> 1] = old + (-coeff);
     }
   }
-  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
-    const int b = m.cbFace[j];
-    if (bcP[b] != 1) continue;
-    const double dpn = m.bDpn[b];
-    if (dpn <= 1e-14) continue;
-    diag += m.bA[b] * rAU[c] / dpn;
+  wave_sync();
+  if (c < q.r1) {
+    double diag = 0.0;
+    const int k1 = m.cfOff[c + 1];
+    for (int k = m.cfOff[c]; k < k1; ++k) {
+      const double co = staged ? ec[w][k - E0] : coef[m.cfEnt[k] >> 1];
+      const int si = staged ? es[w][k - E0] : m.cfSlot[k];
+      diag += co;
+      double* t = staged ? &sv[w][(si >> 1) - q.k0] : &vals[si >> 1];
+      const double old = (si & 1) ? 0.0 : *t;
+      *t = old + (-co);
+    }
+    for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
+      const int b = m.cbFace[j];
+      if (bcP[b] != 1) continue;
+      const double dpn = m.bDpn[b];
+      if (dpn <= 1e-14) continue;
+      diag += m.bA[b] * rAU[c] / dpn;
+    }
+    const int dp = m.diagPos[c];
+    if (staged)
+      sv[w][dp - q.k0] = diag;
+    else
+      vals[dp] = diag;
   }
-  vals[m.diagPos[c]] = diag;
+  wave_sync();
+  if (staged)
+    for (int k = q.k0 + lane; k < q.k1; k += 64) vals[k] = sv[w][k - q.k0];
 }
 
 // --------------------------------------------------------- face kernels ---
@@ -502,14 +538,21 @@ __global__ __launch_bounds__(kB) void k_correct_flux(MeshView m, FaceArgs a,
                                                      const double* __restrict__ phiStar,
                                                      const double* __restrict__ pCorr,
                                                      const double* __restrict__ phiNo,
+                                                     const double* __restrict__ pcoef,
                                                      double* __restrict__ phi) {
   const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (f >= m.nF) return;
   if (f < m.nIF) {
     const int2 pn = m.fPN[f];
-    const double lam = m.fLam[f];
-    const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
-    const double coeff = m.fA[f] * rAUf / m.fDpn[f];
+    const double dpn = m.fDpn[f];
+    double coeff;
+    if (!(dpn <= 1e-14)) {
+      coeff = pcoef[f];  // k_p_face's A rAU_f / dpn, the same expression
+    } else {
+      const double lam = m.fLam[f];
+      const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
+      coeff = m.fA[f] * rAUf / dpn;
+    }
     phi[f] = phiStar[f] - coeff * (pCorr[pn.y] - pCorr[pn.x]) - phiNo[f];
   } else {
     const int b = f - m.nIF;
@@ -790,6 +833,7 @@ SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int*
                         hipMemcpyDeviceToDevice, s));
   Amom.finish_pattern(s);
   pvals.alloc((size_t)Amom.nnz);
+  pCoef.alloc((size_t)std::max(M.nIF, 1));
   an_pcg_options po = par.pressure;
   po.maxIter = par.pMaxit;
   po.absTol = par.pTol;
@@ -1015,7 +1059,9 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
       for (int c = 0; c < 3; ++c) momentum_component(c, &st);
     // pressure matrix + AMG (re)setup (main.cu:3407-3411)
     mark(5);
-    k_p_asm<<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, rAU.p, pvals.p);
+    k_p_face<<<grid_for(M.nIF, kB), kB, 0, s>>>(mv, rAU.p, pCoef.p);
+    k_p_rows<<<grid_for(Amom.rb.count, kW), kB, 0, s>>>(mv, Amom.rb.count, Amom.rb.start.p, bcP.p,
+                                                       rAU.p, pCoef.p, pvals.p);
     AN_CHECK_LAUNCH();
     const int every = std::max(par.pAmgRebuildEvery, 1);
     const bool setup = (step == 1) || (((step - 1) % every) == 0) || !psolver->amg.built;
@@ -1057,7 +1103,7 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
         k_nonorth_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p,
                                                        phiNo.p);
         k_correct_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, phiStar.p, pCorr.p, phiNo.p,
-                                                       phi.p);
+                                                       pCoef.p, phi.p);
         k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phi.p, nullptr, divCorr.p);
         AN_CHECK_LAUNCH();
       }

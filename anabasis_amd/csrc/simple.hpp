@@ -51,6 +51,7 @@ class SimpleSolver {
   DevBuf<double> phiStar, phi, phiNo, divStar, divCorr, rhs;
   DevCSR<double> Amom;
   DevBuf<double> pvals;
+  DevBuf<double> pCoef;  // per internal face: A rAU_f / dpn (0 where skipped)
   std::unique_ptr<PcgSolver> psolver;
   BicgWork bw;
   // fused u/v/w momentum (par.fuseMomentumRhs): per-component gradients,
