@@ -225,6 +225,10 @@ struct DevMesh {
   HostMesh host;
   bool onDevice = false;
   int nC = 0, nF = 0, nIF = 0, nBF = 0, nPatches = 0;
+  // internal faces with !(|n.d| > 1e-30), which the non-orthogonal flux and
+  // its divergence skip; none on a valid mesh, and the kernels then drop the
+  // per-face test
+  int nTinyDpn = 0;
   DevAddressing addr;
   DevBuf<double4> cgeo;
   DevBuf<int2> fPN;

@@ -55,6 +55,8 @@ void DevMesh::to_device(hipStream_t s) {
       ty[f] = A[f] * n[1] - (A[f] / dpn[f]) * d1;
       tz[f] = A[f] * n[2] - (A[f] / dpn[f]) * d2;
     }
+    nTinyDpn = 0;
+    for (int f = 0; f < nIF; ++f) nTinyDpn += !(std::fabs(dpn[f]) > 1.0e-30);
     fPN.upload(pn, s);
     fLam.upload(lam, s);
     fDpn.upload(dpn, s);

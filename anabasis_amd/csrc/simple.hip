@@ -455,6 +455,9 @@ __global__ __launch_bounds__(kB) void k_rc(MeshView m, FaceArgs a, const double*
   phi[f] = out;  // phi = phi* (main.cu:3439)
 }
 
+// CHECK false: no internal face has |n.d| <= 1e-30 (DevMesh::nTinyDpn), so
+// the test and its load are dropped (same results)
+template <bool CHECK>
 __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
                                                      const double* __restrict__ rAU,
                                                      const double* __restrict__ gx,
@@ -465,8 +468,7 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
   if (f >= m.nF) return;
   double flux = 0.0;
   if (f < m.nIF) {
-    const double dpn = m.fDpn[f];
-    if (fabs(dpn) > 1.0e-30) {
+    if (!CHECK || fabs(m.fDpn[f]) > 1.0e-30) {
       const int2 pn = m.fPN[f];
       const double lam = m.fLam[f];
       const double rAUf = (1.0 - lam) * rAU[pn.x] + lam * rAU[pn.y];
@@ -490,9 +492,10 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
 }
 
 // MODE 0: div = continuity of phi (all faces).
-// MODE 1: non-orthogonal divergence of phiNo (active faces only) and the
-//         pressure rhs b = -divStar + divNo.
-template <int MODE>
+// MODE 1: non-orthogonal divergence of phiNo (active faces only; with CHECK
+//         false every internal face is active) and the pressure rhs
+//         b = -divStar + divNo.
+template <int MODE, bool CHECK = true>
 __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ bcP,
                                             const double* __restrict__ flux,
                                             const double* __restrict__ divStar,
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
     for (int u = 0; u < U; ++u) {
       const int f = eu[u] >> 1;
       fl[u] = flux[f];
-      on[u] = MODE == 0 || fabs(m.fDpn[f]) > 1.0e-30;
+      on[u] = MODE == 0 || !CHECK || fabs(m.fDpn[f]) > 1.0e-30;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -1080,10 +1083,22 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
     const int nPass = std::max(par.nNonOrthCorr, 0) + 1;
     int pcgTotal = 0, pcgLast = 0;
     double tP = 0.0;
+    const bool check = M.nTinyDpn > 0;
+    auto nonorth_flux = [&]() {
+      if (check)
+        k_nonorth_flux<true><<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p,
+                                                             phiNo.p);
+      else
+        k_nonorth_flux<false><<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p,
+                                                              g[2].p, phiNo.p);
+    };
     for (int pass = 1; pass <= nPass; ++pass) {
       grad(1, pCorr.p, g);
-      k_nonorth_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p, phiNo.p);
-      k_div<1><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phiNo.p, divStar.p, rhs.p);
+      nonorth_flux();
+      if (check)
+        k_div<1, true><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phiNo.p, divStar.p, rhs.p);
+      else
+        k_div<1, false><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phiNo.p, divStar.p, rhs.p);
       AN_CHECK_LAUNCH();
       psolver->opt.relTol = (pass == nPass) ? 0.0 : par.pRelTol;
       an_solve_info info{};
@@ -1100,8 +1115,7 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
       pcgTotal += info.iterations;
       if (pass == nPass) {
         grad(1, pCorr.p, g);
-        k_nonorth_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p,
-                                                       phiNo.p);
+        nonorth_flux();
         k_correct_flux<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, phiStar.p, pCorr.p, phiNo.p,
                                                        pCoef.p, phi.p);
         k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phi.p, nullptr, divCorr.p);
