@@ -123,7 +123,7 @@ void PcgSolver::set_values_device(const double* vals, bool rebuild) {
     auto t0 = std::chrono::steady_clock::now();
     drop_graphs();  // the captured cycles point into the old hierarchy
     amg.setup(A, s);
-    if (opt.precision == 1) {
+    if (opt.precision == 1 || opt.precision == 2) {
       amg.make_fp32(s);
       r32.alloc(A.n);
       z32.alloc(A.n);
@@ -154,8 +154,8 @@ void PcgSolver::precondition(bool fp32, const ApplyDots& d) {
     launch(s);
     return;
   }
-  const long long key = ((long long)(d.ra.out - scal.p) << 3) | (d.yReady ? 4 : 0) |
-                        (d.q ? 2 : 0) | (fp32 ? 1 : 0);
+  const long long slot = d.ra.out ? (long long)(d.ra.out - scal.p) : (long long)S_COUNT;
+  const long long key = (slot << 3) | (d.yReady ? 4 : 0) | (d.q ? 2 : 0) | (fp32 ? 1 : 0);
   auto it = applyGraphs.find(key);
   if (it == applyGraphs.end()) {
     // captured on a private non-blocking stream (the solver's own stream may
@@ -180,6 +180,12 @@ void PcgSolver::precondition(bool fp32, const ApplyDots& d) {
   AN_HIP(hipGraphLaunch(it->second, s));
 }
 
+void PcgSolver::precondition_p2(bool yReady, const double* qv, double* out) {
+  const ApplyDots none{nullptr, RedArgs{nullptr, nullptr}, yReady};
+  precondition(true, none);  // z32 = M32^-1 r32
+  z_from_fp32(A.n, z32.p, r.p, qv, z.p, red.args(out), s);
+}
+
 // The next preconditioner application is queued before the host knows
 // ||r|| (hiding the round trip) unless the last two residual norms predict
 // convergence in this iteration: then an application would be wasted work
@@ -199,15 +205,24 @@ PubWord::~PubWord() {
   if (h) (void)hipHostFree(h);
 }
 
+// Stamps only grow (modulo 2^30); a word at or after the awaited stamp
+// answers the wait.  A later stamp is the outcome of an iteration queued
+// before the host looked (the three-column BiCGSTAB runs one ahead), which
+// carries the earlier one: CONTINUE only if every earlier iteration
+// continued.
+static inline bool stamp_reached(unsigned have, unsigned want) {
+  return ((have - want) & 0x3fffffffu) < 0x20000000u;
+}
+
 unsigned long long PubWord::wait(unsigned stamp, hipStream_t s, const char* what) const {
   const unsigned want = stamp & 0x3fffffffu;
   unsigned long long w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
   const auto t0 = std::chrono::steady_clock::now();
-  for (unsigned i = 1; (unsigned)(w >> 34) != want; ++i) {
+  for (unsigned i = 1; !stamp_reached((unsigned)(w >> 34), want); ++i) {
     if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
       AN_HIP(hipStreamSynchronize(s));
       w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
-      if ((unsigned)(w >> 34) != want) throw Error(7, what);
+      if (!stamp_reached((unsigned)(w >> 34), want)) throw Error(7, what);
       break;
     }
     w = __atomic_load_n(h, __ATOMIC_ACQUIRE);
@@ -274,9 +289,18 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
   // flexible CG (K-cycle preconditioner): alpha = r.z / p.q with r.z in
   // S_FRZ, beta = -(z.q)/(p.q) (Polak-Ribiere: z.(r_new - r_old) / r_old.z_old
   // with r_new - r_old = -alpha q); otherwise beta = rho_new / rho_old
-  const bool flex = amg.flexible();
-  const ApplyDots d0{nullptr, red.args(scal.p + (flex ? S_FRZ : S_RHO0))};
-  precondition(false, d0);  // r.z from the cycle's last kernel
+  // precision 2: the same FP64 iteration around the FP32 cycle, always
+  // flexible (an FP32 cycle is not exactly the symmetric operator CG's
+  // rho_new / rho_old assumes)
+  const bool p2 = opt.precision == 2;
+  const bool flex = amg.flexible() || p2;
+  if (p2) {
+    copy_d2f(n, r.p, r32.p, s);
+    precondition_p2(false, nullptr, scal.p + S_FRZ);
+  } else {
+    const ApplyDots d0{nullptr, red.args(scal.p + (flex ? S_FRZ : S_RHO0))};
+    precondition(false, d0);  // r.z from the cycle's last kernel
+  }
   pcg_direction<double>(n, p.p, z.p, scal.p, 0, 0, true, s);
   int it = 1, ret = 0;
   double rnPrev = 0.0;
@@ -289,14 +313,22 @@ int PcgSolver::pcg64(const double* b, double* x, double tol, int maxIter, double
     // the update also writes the preconditioner's y = omdinv o r and
     // publishes the iteration's outcome to the host
     const PcgPub pa = pub_args(tol, it);
-    pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv.p,
-                       amg.lv[0].y.p, &pa);
+    if (p2)
+      pcg_update_p2(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv32.p,
+                    r32.p, amg.lv[0].y32.p, &pa);
+    else
+      pcg_update<double>(n, x, r.p, p.p, q.p, scal.p, rhoOld, S_PQ, red, s, amg.lv[0].omdinv.p,
+                         amg.lv[0].y.p, &pa);
     auto next_direction = [&]() {
       EvPair* e2 = next_ev(vcEv, vcUsed);
       if (e2) AN_HIP(hipEventRecord(e2->a, s));
-      const ApplyDots dd = flex ? ApplyDots{q.p, red.args(scal.p + S_FRZ), true}
-                                : ApplyDots{nullptr, red.args(scal.p + rhoNew), true};
-      precondition(false, dd);
+      if (p2) {
+        precondition_p2(true, q.p, scal.p + S_FRZ);
+      } else {
+        const ApplyDots dd = flex ? ApplyDots{q.p, red.args(scal.p + S_FRZ), true}
+                                  : ApplyDots{nullptr, red.args(scal.p + rhoNew), true};
+        precondition(false, dd);
+      }
       if (e2) AN_HIP(hipEventRecord(e2->b, s));
       if (flex)
         pcg_direction<double>(n, p.p, z.p, scal.p, S_FZQ, S_PQ, false, s, -1.0);
