@@ -1221,14 +1221,15 @@ static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
   finalize(H.kred, nb, 3, ks + 2, s);
 }
 
+// dots with no output (ra.out null) only carry yReady
 void AmgHierarchy::apply(const double* r, double* z, hipStream_t s, const ApplyDots* dots) {
   const double* y = dots && dots->yReady ? lv[0].y.p : nullptr;
-  vcycle<double>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots);
+  vcycle<double>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots && dots->ra.out ? dots : nullptr);
 }
 
 void AmgHierarchy::apply32(const float* r, float* z, hipStream_t s, const ApplyDots* dots) {
   const float* y = dots && dots->yReady ? lv[0].y32.p : nullptr;
-  vcycle<float>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots);
+  vcycle<float>(*this, 0, r, y, z, s, std::max(opt.sweeps, 1), dots && dots->ra.out ? dots : nullptr);
 }
 
 }  // namespace an

@@ -490,7 +490,7 @@ an_status an_pcg_create(int n, const int* rowOffsets, const int* cols, int64_t n
   an_pcg_options o;
   default_pcg_options(&o);
   if (opt) o = *opt;
-  if (o.precision != 0 && o.precision != 1) throw Error(AN_EINVAL, "precision must be 0 or 1");
+  if (o.precision < 0 || o.precision > 2) throw Error(AN_EINVAL, "precision must be 0, 1 or 2");
   auto* h = new an_solver_s;
   try {
     h->pcg.reset(new PcgSolver(n, rowOffsets, cols, nnz, o, mem == AN_DEVICE, 0));
@@ -870,7 +870,7 @@ an_status an_simple_create(an_mesh_t mesh, const an_simple_params* par, const in
   if (!mesh || !par || !patchUType || !patchPType || !uFaceBC || !vFaceBC || !wFaceBC ||
       !pFaceBC || !out)
     throw Error(AN_EINVAL, "NULL argument");
-  if (par->precision != 0 && par->precision != 1) throw Error(AN_EINVAL, "precision must be 0/1");
+  if (par->precision < 0 || par->precision > 2) throw Error(AN_EINVAL, "precision must be 0, 1 or 2");
   if (!(par->uRelax > 0.0) || !(par->mu >= 0.0)) throw Error(AN_EINVAL, "invalid uRelax / mu");
   auto* h = new an_simple_s;
   try {
@@ -1037,6 +1037,8 @@ an_status an_batch_create(int nProblems, const int* nRows, const int* const* row
   an_pcg_options o;
   default_pcg_options(&o);
   if (opt) o = *opt;
+  if (o.precision != 0 && o.precision != 1)
+    throw Error(AN_EINVAL, "batched solves take precision 0 or 1");
   std::vector<int> rows(nRows, nRows + nProblems);
   std::vector<const int*> ro(rowOffsets, rowOffsets + nProblems), ci(cols, cols + nProblems);
   auto h = std::make_unique<an_batch_s>();

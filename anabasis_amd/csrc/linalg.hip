@@ -348,6 +348,78 @@ __global__ __launch_bounds__(1024) void k_finalize_pub(int nparts,
   }
 }
 
+// k_pcg_update<double> for an FP32 preconditioner: the FP64 x and r, and
+// the cycle's input r32 = (float) r with its first sweep y32 = omd32 o r32
+__global__ __launch_bounds__(kBlock) void k_pcg_update_p2(int n, double* __restrict__ x,
+                                                          double* __restrict__ r,
+                                                          const double* __restrict__ p,
+                                                          const double* __restrict__ q,
+                                                          const double* scal, int num, int den,
+                                                          RedArgs ra,
+                                                          const float* __restrict__ omd32,
+                                                          float* __restrict__ r32,
+                                                          float* __restrict__ y32) {
+  __shared__ double sh[kWaves];
+  const double d = scal[den], a = scal[num] / d;
+  const double alpha = (d > 0.0 && isfinite(a)) ? a : 0.0;
+  double v = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const float rf = (float)ri;
+    r32[i] = rf;
+    y32[i] = (float)((double)omd32[i] * (double)rf);
+    v += ri * ri;
+  }
+  const double bs = block_sum(v, sh);
+  store_partials<1>(&bs, blockIdx.x, ra);
+}
+
+void pcg_update_p2(int n, double* x, double* r, const double* p, const double* q, double* scal,
+                   int num, int den, Reducer& red, hipStream_t s, const float* omd32, float* r32,
+                   float* y32, const PcgPub* pub) {
+  const int nb = red_blocks(n);
+  k_pcg_update_p2<<<nb, kBlock, 0, s>>>(n, x, r, p, q, scal, num, den, red.args(scal + S_RR), omd32,
+                                        r32, y32);
+  AN_CHECK_LAUNCH();
+  if (pub && pub->word) {
+    k_finalize_pub<<<1, 1024, 0, s>>>(nb, red.partial.p, scal, *pub);
+    AN_CHECK_LAUNCH();
+  } else {
+    finalize(red, nb, 1, scal + S_RR, s);
+  }
+}
+
+// z = (double) z32 with the Krylov products in FP64: r.z (and z.q)
+template <int DOT>
+__global__ __launch_bounds__(kBlock) void k_z_from32(int n, const float* __restrict__ z32,
+                                                     const double* __restrict__ r,
+                                                     const double* __restrict__ q,
+                                                     double* __restrict__ z, RedArgs ra) {
+  __shared__ double sh[2 * kWaves];
+  double d[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double zi = (double)z32[i];
+    z[i] = zi;
+    d[0] += r[i] * zi;
+    if (DOT == 2) d[1] += zi * q[i];
+  }
+  block_sums<DOT>(d, sh);
+  store_partials<DOT>(d, blockIdx.x, ra);
+}
+
+void z_from_fp32(int n, const float* z32, const double* r, const double* q, double* z,
+                 const RedArgs& ra, hipStream_t s) {
+  const int nb = red_blocks(n);
+  if (q)
+    k_z_from32<2><<<nb, kBlock, 0, s>>>(n, z32, r, q, z, ra);
+  else
+    k_z_from32<1><<<nb, kBlock, 0, s>>>(n, z32, r, q, z, ra);
+  AN_CHECK_LAUNCH();
+  finalize_parts(ra, nb, q ? 2 : 1, s);
+}
+
 template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s, const T* omd, T* y, const PcgPub* pub) {

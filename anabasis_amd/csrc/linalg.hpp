@@ -199,6 +199,14 @@ template <typename T>
 void pcg_update(int n, T* x, T* r, const T* p, const T* q, double* scal, int num, int den,
                 Reducer& red, hipStream_t s, const T* omd = nullptr, T* y = nullptr,
                 const PcgPub* pub = nullptr);
+// pcg_update<double> for an FP32 preconditioner: also r32 = (float) r and
+// y32 = omd32 o r32 (the FP32 cycle's input and first sweep)
+void pcg_update_p2(int n, double* x, double* r, const double* p, const double* q, double* scal,
+                   int num, int den, Reducer& red, hipStream_t s, const float* omd32, float* r32,
+                   float* y32, const PcgPub* pub);
+// z = (double) z32; ra.out[0] = r.z and, with q, ra.out[1] = z.q (FP64)
+void z_from_fp32(int n, const float* z32, const double* r, const double* q, double* z,
+                 const RedArgs& ra, hipStream_t s);
 // p = z + beta p, beta = sign * scal[num]/scal[den]  (first: p = z)
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,

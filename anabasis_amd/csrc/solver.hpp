@@ -112,6 +112,9 @@ class PcgSolver {
   // with r.z (and z.q) into d; replayed from a graph captured on first use
   // per (precision, target, q) and dropped when the hierarchy is rebuilt
   void precondition(bool fp32, const ApplyDots& d);
+  // precision 2: z32 = M32^-1 r32 (y32 ready when yReady), then z = (double)
+  // z32 with out[0] = r.z and, with qv, out[1] = z.q in FP64
+  void precondition_p2(bool yReady, const double* qv, double* out);
   void drop_graphs();
   std::map<long long, hipGraphExec_t> applyGraphs;
   hipStream_t cs = nullptr;  // capture stream

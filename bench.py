@@ -10,11 +10,13 @@ H = 0.41 Umax = 0.45, outlet p = 0, no-slip walls, nu = 1e-3.  One step = one
 outer iteration; warmup iterations (including the first AMG setup) are not
 timed.  FP64 throughout.
 
-Also reported: the same loop with the mixed-precision pressure solve, and the
-pressure-solve metric -- 128^3 hex box with interior vertices jittered by
-+-20% h (seed 20261203), Neumann everywhere except fixed_value 0 on x = 0,
-b ~ N(0,1) mean-projected, x0 = 0, solved to ||r||/||b|| <= 1e-10 with FP64
-PCG+AMG and with FP32-inner / FP64-refined PCG+AMG.
+Also reported: the same loop with the two mixed-precision pressure solves
+(precision 1: FP32 PCG+AMG inside FP64 refinement; precision 2: FP64 PCG
+preconditioned by the FP32 AMG cycle), and the pressure-solve metric -- 128^3
+hex box with interior vertices jittered by +-20% h (seed 20261203), Neumann
+everywhere except fixed_value 0 on x = 0, b ~ N(0,1) mean-projected, x0 = 0,
+solved to ||r||/||b|| <= 1e-10 with FP64 PCG+AMG and with both mixed
+solves.
 
 Roofline: every Krylov SpMV of the timed FP64 pressure solves is bracketed by
 HIP events recorded around the kernel itself (hipExtLaunchKernel) on the
@@ -98,7 +100,7 @@ def run_cylinder_case(cells, max_iters):
     m, vel, prs = simple_case(cells)
     out = {"cells": m.nCells}
     fields = {}
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         S = sm.Simple(m, sm.default_params(precision=prec), vel, prs)
         t0 = time.time()
         st, conv = S.iterate(max_iters)
@@ -114,14 +116,15 @@ def run_cylinder_case(cells, max_iters):
         if prec == 0:
             out.update(r)
         else:
+            name = "mixed" if prec == 1 else "fp32_precond"
             F64 = np.array(out["force_total"])
-            out["mixed"] = dict(r, force_diff_rel_to_norm=float(
+            out[name] = dict(r, force_diff_rel_to_norm=float(
                 np.abs(f[6:9] - F64).max() / np.linalg.norm(F64)),
-                field_rel_diff={k: float(np.linalg.norm(fields[1][k] - fields[0][k]) /
+                field_rel_diff={k: float(np.linalg.norm(fields[prec][k] - fields[0][k]) /
                                          max(np.linalg.norm(fields[0][k]), 1e-300))
                                 for k in "uvwp"})
-            log(f"  mixed vs FP64 at convergence: force {out['mixed']['force_diff_rel_to_norm']:.2e}"
-                f" |F|, fields {out['mixed']['field_rel_diff']}")
+            log(f"  {name} vs FP64 at convergence: force {out[name]['force_diff_rel_to_norm']:.2e}"
+                f" |F|, fields {out[name]['field_rel_diff']}")
     return out
 
 
@@ -423,7 +426,7 @@ def run_psolve(n, reps):
     log(f"psolve mesh {n}^3: nC={m.nCells} nnz={pat.nnz} ({time.time() - t0:.1f}s)")
     A = sp.csr_matrix((vals, pat.cols, pat.rowOffsets))
     res = {}
-    for prec, name in ((0, "fp64"), (1, "mixed")):
+    for prec, name in ((0, "fp64"), (1, "mixed"), (2, "fp32_precond")):
         # time to solution from scratch: no projection onto earlier solves
         S = lp.PCGSolver(pat.rowOffsets, pat.cols, relTol=1e-10, absTol=0.0, precision=prec,
                          profile=1, **{**AMG_OVERRIDES, "rhsProjection": 0})
@@ -448,9 +451,10 @@ def run_psolve(n, reps):
             f"relres {relres:.2e} spmv {r['spmv_avg_s'] * 1e6:.1f}us vcycle "
             f"{r['vcycle_avg_s'] * 1e6:.1f}us")
         del S
-    res["mixed_vs_fp64_rel_diff"] = float(np.linalg.norm(res["mixed"]["x"] - res["fp64"]["x"]) /
-                                          np.linalg.norm(res["fp64"]["x"]))
-    for k in ("fp64", "mixed"):
+    for k in ("mixed", "fp32_precond"):
+        res[f"{k}_vs_fp64_rel_diff"] = float(np.linalg.norm(res[k]["x"] - res["fp64"]["x"]) /
+                                             np.linalg.norm(res["fp64"]["x"]))
+    for k in ("fp64", "mixed", "fp32_precond"):
         res[k].pop("x")
     return m, pat, res
 
@@ -521,19 +525,27 @@ def main():
 
     m, vel, prs = simple_case(args.cells)
     r64 = run_simple(m, vel, prs, 0, args.warmup, args.steps)
-    mixed = None
+    mixed = fp32p = None
+    f64s = r64.pop("fields")
     if not args.no_mixed:
-        rmx = run_simple(m, vel, prs, 1, args.warmup, args.steps)
-        f64s = r64.pop("fields")
-        fmx = rmx.pop("fields")
-        mixed = dict(iters_per_s=rmx["iters_per_s"], massRes=rmx["massRes"],
-                     pcg_total_last=rmx["pcg_total"],
-                     rel_diff_vs_fp64={k: float(np.linalg.norm(fmx[k] - f64s[k]) /
-                                                max(np.linalg.norm(f64s[k]), 1e-300))
-                                       for k in "uvwp"},
-                     force_cylinder=rmx["force_cylinder"])
-    else:
-        r64.pop("fields")
+        # precision 1: FP32 PCG + AMG inside FP64 refinement; precision 2:
+        # FP64 PCG preconditioned by the FP32 AMG cycle
+        runs = {}
+        for prec in (1, 2):
+            rmx = run_simple(m, vel, prs, prec, args.warmup, args.steps)
+            fmx = rmx.pop("fields")
+            F64 = np.array(r64["force_cylinder"])
+            runs[prec] = dict(
+                iters_per_s=rmx["iters_per_s"], massRes=rmx["massRes"],
+                pcg_total_last=rmx["pcg_total"],
+                p_pcg_iterations_per_step=rmx["phase_seconds_timed"]["p_pcg_iterations"] / args.steps,
+                rel_diff_vs_fp64={k: float(np.linalg.norm(fmx[k] - f64s[k]) /
+                                           max(np.linalg.norm(f64s[k]), 1e-300))
+                                  for k in "uvwp"},
+                force_cylinder=rmx["force_cylinder"],
+                force_diff_rel_to_norm=float(np.abs(np.array(rmx["force_cylinder"]) - F64).max() /
+                                             np.linalg.norm(F64)))
+        mixed, fp32p = runs[1], runs[2]
     ps = None
     roof = None
     if not args.no_psolve:
@@ -581,6 +593,7 @@ def main():
                    "cells": m.nCells, "faces": m.nFaces},
         "simple_fp64": r64,
         "simple_mixed": mixed,
+        "simple_fp32_precond": fp32p,
         "psolve": ps,
         "momentum3": mom3,
         "mms_tet": mms,

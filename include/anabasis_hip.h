@@ -157,7 +157,9 @@ typedef struct {
   int maxIter;         /* HypreOptions::maxIter */
   double relTol;       /* stop when ||r|| <= max(relTol*||b||, absTol) */
   double absTol;
-  int precision;       /* 0 FP64; 1 FP32 inner PCG+AMG in FP64 refinement */
+  int precision;       /* 0 FP64; 1 FP32 inner PCG+AMG in FP64 refinement;
+                        * 2 FP64 PCG (SpMV, residuals, inner products, stop
+                        * test) preconditioned by the FP32 AMG cycle */
   int coarseSize;      /* AMG: coarsest level size (dense solve) */
   int maxLevels;
   double strength;     /* AMG: pairwise matching strength threshold */
@@ -324,7 +326,7 @@ an_status an_solve_steady_scalar_transport(
  * The PCG iteration is captured once in a HIP graph and replayed until
  * every problem has converged.  No reference counterpart (the reference
  * solves one system per HYPRE call, LP/src/hypre_backend.cu:3-114).
- * opt: as an_pcg_create (precision 1 = FP32 PCG+AMG inside FP64
+ * opt: as an_pcg_create (precision 0 or 1; 1 = FP32 PCG+AMG inside FP64
  * refinement).  Patterns are copied. */
 typedef struct an_batch_s *an_batch_t;
 an_status an_batch_create(int nProblems, const int *nRows,
@@ -435,7 +437,7 @@ typedef struct {
   int pMaxit;
   double pTol, pRelTol;
   int pAmgRebuildEvery;
-  int precision;        /* 0 FP64 pressure solve, 1 mixed */
+  int precision;        /* pressure solve: as an_pcg_options.precision */
   int fuseMomentumRhs;  /* 1: u/v/w share one 3-RHS BiCGSTAB per pass */
   an_pcg_options pressure;
 } an_simple_params;

@@ -367,6 +367,18 @@ def test_pcg_warm_start_zero_rhs_and_mixed(meshes, omeshes):
     xm, im = M.solve(b)
     assert im.converged and im.finalRelResNorm <= 1e-10 and im.refinementSteps >= 2
     assert np.linalg.norm(xm - x1) / np.linalg.norm(x1) < 1e-8
+    # precision 2: the FP64 iteration around the FP32 cycle keeps the FP64
+    # stop test and about the FP64 iteration count
+    P2 = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, precision=2)
+    P2.set_values(vals)
+    x2p, i2p = P2.solve(b)
+    h2 = P2.residual_history()
+    assert i2p.converged and i2p.finalRelResNorm <= 1e-10 and i2p.refinementSteps == 0
+    assert i2p.iterations <= i1.iterations + 2
+    assert len(h2) == i2p.iterations + 1 and h2[-1] == i2p.finalResNorm
+    r2 = b - sp.csr_matrix((vals, om.cols, om.rowOffsets)) @ x2p
+    assert np.linalg.norm(r2) <= 1.01e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(x2p - x1) / np.linalg.norm(x1) < 1e-8
 
 
 def test_bicgstab_matches_oracle(meshes, omeshes):

@@ -117,15 +117,18 @@ def test_simple_fused_momentum_is_bitwise_identical(case):
     assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
 
 
-def test_mixed_precision_converged_state_matches_fp64_oracle(gpu):
-    """The mixed-precision pressure solve (FP32 PCG + AMG inside FP64
-    refinement) run to the case's own stop test on the 16,660-cell cylinder
-    channel: the same stop iteration as the FP64 oracle, fields within 1e-9
-    relative and the cylinder force within 1e-8 of |F| (measured: fields
-    4e-11, force 9.4e-12 |F|)."""
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_precision_converged_state_matches_fp64_oracle(gpu, precision):
+    """The mixed-precision pressure solves -- 1: FP32 PCG + AMG inside FP64
+    refinement; 2: FP64 PCG preconditioned by the FP32 AMG cycle -- run to
+    the case's own stop test on the 16,660-cell cylinder channel: the same
+    stop iteration as the FP64 oracle, fields within 1e-9 relative and the
+    cylinder force within 1e-8 of |F| (measured, precision 1: fields 4e-11,
+    force 9.4e-12 |F|)."""
     m = lp.Mesh.cylinder(20000)
     vel, prs = sm.cylinder_case_bcs()
-    par = sm.default_params(velTol=1e-13, pTol=1e-13, precision=1, pressure=dict(coarseSize=64))
+    par = sm.default_params(velTol=1e-13, pTol=1e-13, precision=precision,
+                            pressure=dict(coarseSize=64))
     S = sm.Simple(m, par, vel, prs)
     om = O.OMesh.from_product(m)
     u0 = [S.field(k) for k in "uvwp"]
