@@ -45,7 +45,8 @@ class PcgOptions(C.Structure):
                 ("precision", C.c_int), ("coarseSize", C.c_int), ("maxLevels", C.c_int),
                 ("strength", C.c_double), ("pairPasses", C.c_int), ("jacobiOmega", C.c_double),
                 ("innerRelTol", C.c_double), ("profile", C.c_int), ("sweeps", C.c_int),
-                ("kcycle", C.c_int), ("graphs", C.c_int), ("rhsProjection", C.c_int)]
+                ("kcycle", C.c_int), ("graphs", C.c_int), ("rhsProjection", C.c_int),
+                ("smoother", C.c_int), ("chebLevels", C.c_int), ("chebFraction", C.c_double)]
 
 
 class SolveInfo(C.Structure):

@@ -334,6 +334,71 @@ __global__ void k_omdinv(int n, const int* __restrict__ ro, const int* __restric
   out[i] = (T)((d != 0.0) ? omega * (1.0 / d) : 0.0);
 }
 
+// omdinv_i = cc[0] / a_ii written as k_omdinv writes omega / a_ii
+template <typename T>
+__global__ void k_omdinv_dev(int n, const int* __restrict__ ro, const int* __restrict__ ci,
+                             const double* __restrict__ va, const double* __restrict__ cc,
+                             T* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int k = ro[i]; k < ro[i + 1]; ++k)
+    if (ci[k] == i) d = va[k];
+  out[i] = (T)((d != 0.0) ? cc[0] * (1.0 / d) : 0.0);
+}
+
+// Block maxima of the Gershgorin bound of D^-1 A: sum_k |a_ik| / |a_ii| over
+// rows with a_ii != 0 (each row sum in row order; a maximum is the same
+// number in any order of evaluation)
+__global__ __launch_bounds__(kB) void k_gersh(int n, const int* __restrict__ ro,
+                                              const int* __restrict__ ci,
+                                              const double* __restrict__ va,
+                                              double* __restrict__ partial) {
+  __shared__ double sh[kW];
+  double g = 0.0;
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    double s = 0.0, d = 0.0;
+    for (int k = ro[i]; k < ro[i + 1]; ++k) {
+      s += fabs(va[k]);
+      if (ci[k] == i) d = va[k];
+    }
+    if (d != 0.0) g = fmax(g, s / fabs(d));
+  }
+  for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = g;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kW; ++k) g = fmax(g, sh[k]);
+    partial[blockIdx.x] = g;
+  }
+}
+
+// The Chebyshev coefficients from the bound (the oracle's cheb_coefficients,
+// expression for expression): lmin = fraction lmax, theta, delta, sigma,
+// rho1 = 1/sigma, rho2 = 1/(2 sigma - rho1); cc = {1/theta, 1 + rho2 rho1,
+// -(rho2 rho1), (2 rho2 / delta) theta}
+__global__ __launch_bounds__(kB) void k_cheb_coef(int nparts, const double* __restrict__ partial,
+                                                  double fraction, double* __restrict__ cc) {
+  __shared__ double sh[kW];
+  double g = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kB) g = fmax(g, partial[i]);
+  for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_down(g, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = g;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < kW; ++k) g = fmax(g, sh[k]);
+  const double lmax = g > 0.0 ? g : 1.0;
+  const double lmin = fraction * lmax;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  const double rho1 = 1.0 / sigma;
+  const double rho2 = 1.0 / (2.0 * sigma - rho1);
+  cc[0] = 1.0 / theta;
+  cc[1] = 1.0 + rho2 * rho1;
+  cc[2] = -(rho2 * rho1);
+  cc[3] = (2.0 * rho2 / delta) * theta;
+}
+
 template <typename T>
 __global__ void k_cast(long long n, const double* __restrict__ a, T* __restrict__ b) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -365,7 +430,11 @@ __device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
 //   prolong:  e_i = y_i + xc[agg_i]
 //   up:       x_i = e_i + omdinv_i (b_i - sum_j a_ij e_j)   (KIND 1, v = e)
 // so the two CSR-stream kernels gather a single vector, like an SpMV.
-// DOT (KIND 1 only) adds
+// The degree-2 Chebyshev smoother's second step (cc = {1/theta, cv, cw, cr},
+// omdinv = (1/theta) / a_ii):
+//   x_i = cv v_i + cr omdinv_i (b_i - sum_j a_ij v_j)          (KIND 2, from 0)
+//   x_i = (cv v_i + cw wv_i) + cr omdinv_i (b_i - sum_j a_ij v_j) (KIND 3)
+// DOT (KIND 1 and 3) adds
 // the partial sums of b_i x_i (DOT 1) and x_i qv_i (DOT 2) of the result and
 // finishes them in the kernel: the preconditioned CG's r.z and z.q.
 template <typename T, int KIND, int DOT = 0>
@@ -377,7 +446,9 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
                                                    const T* __restrict__ b,
                                                    const T* __restrict__ v,
                                                    T* __restrict__ out,
-                                                   const T* __restrict__ qv, RedArgs ra) {
+                                                   const T* __restrict__ qv, RedArgs ra,
+                                                   const T* __restrict__ wv = nullptr,
+                                                   const double* __restrict__ cc = nullptr) {
   __shared__ double prod[kW][kCap];
   __shared__ double sh[2 * kW];
   const int need = (nrb + kW - 1) / kW;
@@ -393,15 +464,16 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
   const int i = q.r0 + lane;
   auto vj = [&](int j) { return (double)v[j]; };
   int kb = 0, ke = 0;
-  double bi = 0.0, vi = 0.0, oi = 0.0, qi = 0.0;
+  double bi = 0.0, vi = 0.0, oi = 0.0, qi = 0.0, wi = 0.0;
   if (i < q.r1) {
     kb = ro[i];
     ke = ro[i + 1];
     bi = (double)b[i];
-    if (KIND == 1) {
+    if (KIND >= 1) {
       vi = (double)v[i];
       oi = (double)omdinv[i];
     }
+    if (KIND == 3) wi = (double)wv[i];
     if (DOT == 2) qi = (double)qv[i];
   }
   if (q.stream && q.k1 > q.k0) stage_products(q, lane, ci, va, vj, prod[w]);
@@ -417,7 +489,13 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
     if (KIND == 0) {
       out[i] = (T)(bi - s);
     } else {
-      const T o = (T)(vi + oi * (bi - s));
+      T o;
+      if (KIND == 1) {
+        o = (T)(vi + oi * (bi - s));
+      } else {
+        const double t = oi * (bi - s);
+        o = KIND == 2 ? (T)(cc[1] * vi + cc[3] * t) : (T)((cc[1] * vi + cc[2] * wi) + cc[3] * t);
+      }
       out[i] = o;
       if (DOT >= 1) d[0] = bi * (double)o;
       if (DOT == 2) d[1] = (double)o * qi;
@@ -969,6 +1047,12 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
   }
   // lv may have reallocated while growing: re-point owned matrices
   for (size_t l = 1; l < lv.size(); ++l) lv[l].A = &lv[l].own;
+  for (size_t l = 0; l < lv.size(); ++l) {
+    AmgLevel& L = lv[l];
+    L.cheb = opt.smoother == 1 && !L.isCoarsest &&
+             (opt.chebLevels <= 0 || (int)l < opt.chebLevels);
+    if (L.cheb) cheb_setup(L, s);
+  }
   AmgLevel& Lc = lv.back();
   if (Lc.n <= 4096) dense_inverse_device(*Lc.A, Lc.coarseInvT, s);
   int maxNb = 0;
@@ -989,6 +1073,23 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
   built = true;
 }
 
+// Chebyshev data of a level from its current values: the Gershgorin bound,
+// the coefficients and omdinv = (1/theta) / a_ii (and its FP32 copy)
+void AmgHierarchy::cheb_setup(AmgLevel& L, hipStream_t s) {
+  const DevCSR<double>& A = *L.A;
+  const int nb = std::min(grid_for(L.n, kB), 1024);
+  L.chebCoef.ensure(4);
+  DevBuf<double> part((size_t)nb);
+  k_gersh<<<nb, kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, part.p);
+  k_cheb_coef<<<1, kB, 0, s>>>(nb, part.p, opt.chebFraction, L.chebCoef.p);
+  k_omdinv_dev<double><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p,
+                                                        L.chebCoef.p, L.omdinv.p);
+  if (L.omdinv32.p)
+    k_omdinv_dev<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p,
+                                                         L.chebCoef.p, L.omdinv32.p);
+  AN_CHECK_LAUNCH();
+}
+
 bool AmgHierarchy::flexible() const {
   for (const auto& L : lv)
     if (L.kLevel) return true;
@@ -998,12 +1099,18 @@ bool AmgHierarchy::flexible() const {
 void AmgHierarchy::refresh_fine(hipStream_t s) {
   if (!built) return;
   AmgLevel& L = lv[0];
-  k_omdinv<double><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p, L.A->vals.p,
-                                                    opt.omega, L.omdinv.p);
+  if (L.cheb) {
+    PoolScope scope(&pool);
+    cheb_setup(L, s);  // the bound and omdinv (and its FP32 copy) of the new values
+  } else {
+    k_omdinv<double><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p,
+                                                      L.A->vals.p, opt.omega, L.omdinv.p);
+    if (has32)
+      k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p,
+                                                       L.A->vals.p, opt.omega, L.omdinv32.p);
+  }
   AN_CHECK_LAUNCH();
   if (has32) {
-    k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.A->rowOff.p, L.A->cols.p, L.A->vals.p,
-                                                     opt.omega, L.omdinv32.p);
     k_cast<float><<<grid_for(L.A->nnz, kB), kB, 0, s>>>(L.A->nnz, L.A->vals.p, L.A32.vals.p);
     AN_CHECK_LAUNCH();
   }
@@ -1025,8 +1132,12 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
     L.A32.vals.alloc((size_t)A.nnz);
     k_cast<float><<<grid_for(A.nnz, kB), kB, 0, s>>>(A.nnz, A.vals.p, L.A32.vals.p);
     L.omdinv32.alloc((size_t)L.n);
-    k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p, opt.omega,
-                                                     L.omdinv32.p);
+    if (L.cheb)
+      k_omdinv_dev<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p,
+                                                           L.chebCoef.p, L.omdinv32.p);
+    else
+      k_omdinv<float><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, A.rowOff.p, A.cols.p, A.vals.p,
+                                                       opt.omega, L.omdinv32.p);
     AN_CHECK_LAUNCH();
     L.b32.alloc((size_t)L.n);
     L.x32.alloc((size_t)L.n);
@@ -1142,12 +1253,24 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
       finalize_parts(dots->ra, G, qv ? 2 : 1, s);
     }
   };
-  // pre-smoothing: x0 = y; extra sweeps ping-pong between x and r
+  // Chebyshev level: the cycle's y (= omdinv o b, omdinv = (1/theta) / a_ii)
+  // is the first step from zero; the second goes to r, the residual of it
+  // to x, e = x2 + P xc in place in r, then the two post steps from e:
+  // y = e + omdinv (b - A e) and x = (cv y + cw e) + cr omdinv (b - A y)
+  const double* cc = L.cheb ? L.chebCoef.p : nullptr;
   const T* xs = y;
-  for (int k = 1; k < sweeps; ++k) {
-    T* to = (xs == x) ? r : x;
-    sweep(xs, to, false);
-    xs = to;
+  if (L.cheb) {
+    k_level_rows<T, 2><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
+                                             A.vals.p, D::omd(L), b, y, r, (const T*)nullptr,
+                                             noRed, (const T*)nullptr, cc);
+    xs = r;
+  } else {
+    // pre-smoothing: x0 = y; extra sweeps ping-pong between x and r
+    for (int k = 1; k < sweeps; ++k) {
+      T* to = (xs == x) ? r : x;
+      sweep(xs, to, false);
+      xs = to;
+    }
   }
   T* res = (xs == r) ? x : r;
   k_level_rows<T, 0><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
@@ -1163,7 +1286,7 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   // then the post-smoothing sweeps ping-pong between r and x; e starts in
   // the buffer from which an odd/even number of sweeps ends in x (in place
   // when that buffer already holds x_pre)
-  T* e = (sweeps % 2 == 1) ? r : x;
+  T* e = L.cheb ? r : (sweeps % 2 == 1) ? r : x;
   if (C.kLevel) {
     kcorrect<T>(H, l + 1, s, sweeps);
     const double* ks = H.kscal.p + 8 * (size_t)(l + 1);
@@ -1173,6 +1296,27 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
     vcycle<T>(H, l + 1, D::b(C), D::y(C), D::x(C), s, sweeps);
     k_prolong<T, false><<<grid_for(L.n, kB) * sp, kB, 0, s>>>(L.n, xs, D::x(C), nullptr, nullptr,
                                                          L.agg.p, e);
+  }
+  if (L.cheb) {
+    T* x1 = D::y(L);  // y's first-step value is used up
+    sweep(e, x1, false);
+    const int2* rbs = A.rb.start.p;
+    const int nrb = A.rb.count;
+    if (dots && qv)
+      k_level_rows<T, 3, 2><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
+                                                  D::omd(L), b, x1, x, qv, dots->ra, e, cc);
+    else if (dots)
+      k_level_rows<T, 3, 1><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
+                                                  D::omd(L), b, x1, x, qv, dots->ra, e, cc);
+    else
+      k_level_rows<T, 3><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
+                                               D::omd(L), b, x1, x, qv, noRed, e, cc);
+    if (dots) {
+      AN_CHECK_LAUNCH();
+      finalize_parts(dots->ra, G, qv ? 2 : 1, s);
+    }
+    AN_CHECK_LAUNCH();
+    return;
   }
   const T* cur = e;
   for (int k = 0; k < sweeps; ++k) {

@@ -13,6 +13,9 @@ struct AmgOptions {
   double omega = 0.75;
   int sweeps = 1;  // Jacobi sweeps per side of the coarse correction
   int kcycle = 0;  // levels 1..kcycle (not the coarsest): K-cycle correction
+  int smoother = 0;            // 1: degree-2 Chebyshev on the chebLevels finest levels
+  int chebLevels = 1;          // <= 0: every smoothed level
+  double chebFraction = 0.3;   // lmin = chebFraction * lmax
 };
 
 struct AmgLevel {
@@ -22,6 +25,10 @@ struct AmgLevel {
   DevBuf<double> omdinv;        // omega / a_ii
   DevBuf<int> agg, moff, mem;   // aggregation to level+1
   RowBlocks mrb;                // row blocks of (moff, mem) for the restriction
+  // degree-2 Chebyshev smoother: chebCoef (device) = {1/theta, cv, cw, cr};
+  // omdinv then holds (1/theta) / a_ii
+  bool cheb = false;
+  DevBuf<double> chebCoef;
   DevBuf<double> coarseInvT;    // coarsest: dense inverse, column-major
   DevBuf<double> b, x, r, y;    // work vectors (FP64); y = omdinv o b
   // FP32 copies for the mixed-precision V-cycle
@@ -58,6 +65,7 @@ class AmgHierarchy {
   // Refreshes level-0 smoother data after the fine values changed (stale
   // coarse levels, as a HYPRE hierarchy reused between setups).
   void refresh_fine(hipStream_t s);
+  void cheb_setup(AmgLevel& L, hipStream_t s);
   void make_fp32(hipStream_t s);
   // z = M^-1 r, one cycle from zero (V, or K on the kcycle levels)
   void apply(const double* r, double* z, hipStream_t s, const ApplyDots* dots = nullptr);

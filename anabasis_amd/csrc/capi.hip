@@ -491,6 +491,9 @@ an_status an_pcg_create(int n, const int* rowOffsets, const int* cols, int64_t n
   default_pcg_options(&o);
   if (opt) o = *opt;
   if (o.precision < 0 || o.precision > 2) throw Error(AN_EINVAL, "precision must be 0, 1 or 2");
+  if (o.smoother != 0 && o.smoother != 1) throw Error(AN_EINVAL, "smoother must be 0 or 1");
+  if (o.smoother == 1 && !(o.chebFraction > 0.0 && o.chebFraction < 1.0))
+    throw Error(AN_EINVAL, "chebFraction must be in (0, 1)");
   auto* h = new an_solver_s;
   try {
     h->pcg.reset(new PcgSolver(n, rowOffsets, cols, nnz, o, mem == AN_DEVICE, 0));
@@ -1039,6 +1042,7 @@ an_status an_batch_create(int nProblems, const int* nRows, const int* const* row
   if (opt) o = *opt;
   if (o.precision != 0 && o.precision != 1)
     throw Error(AN_EINVAL, "batched solves take precision 0 or 1");
+  if (o.smoother != 0) throw Error(AN_EINVAL, "batched solves take the Jacobi smoother");
   std::vector<int> rows(nRows, nRows + nProblems);
   std::vector<const int*> ro(rowOffsets, rowOffsets + nProblems), ci(cols, cols + nProblems);
   auto h = std::make_unique<an_batch_s>();

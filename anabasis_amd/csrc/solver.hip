@@ -37,6 +37,9 @@ void default_pcg_options(an_pcg_options* o) {
   o->kcycle = 2;
   o->graphs = 0;  // replay measured no faster than launching here
   o->rhsProjection = 0;
+  o->smoother = 0;
+  o->chebLevels = 1;
+  o->chebFraction = 0.3;
 }
 
 static constexpr int kB = 256;
@@ -67,6 +70,9 @@ PcgSolver::PcgSolver(int n, const int* rowOff, const int* cols, long long nnz,
   amg.opt.omega = o.jacobiOmega;
   amg.opt.sweeps = o.sweeps > 0 ? o.sweeps : 1;
   amg.opt.kcycle = o.kcycle > 0 ? o.kcycle : 0;
+  amg.opt.smoother = o.smoother;
+  amg.opt.chebLevels = o.chebLevels;
+  amg.opt.chebFraction = o.chebFraction;
   AN_HIP(hipStreamSynchronize(s));
 }
 

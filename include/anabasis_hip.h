@@ -184,6 +184,14 @@ typedef struct {
                           x0 + span X with the smallest residual before
                           iterating; the set is cleared when matrix values
                           are set.  0 (default): off */
+  int smoother;        /* AMG: 0 damped Jacobi (jacobiOmega, sweeps); 1
+                          degree-2 Chebyshev on the chebLevels finest levels
+                          (all smoothed levels when chebLevels <= 0), on the
+                          interval [chebFraction lmax, lmax] of D^-1 A with
+                          lmax its Gershgorin bound; Jacobi below.  The
+                          batched solver takes 0 only */
+  int chebLevels;
+  double chebFraction;
 } an_pcg_options;
 void an_pcg_default_options(an_pcg_options *o);
 

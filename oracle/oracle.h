@@ -226,6 +226,12 @@ typedef struct {
                           preconditioned by the next cycle (K-cycle); the
                           outer PCG then uses the flexible (Polak-Ribiere)
                           beta.  0: V-cycle */
+  int smoother;        /* 0: damped Jacobi (jacobiOmega, sweeps); 1: degree-2
+                          Chebyshev on levels 0..chebLevels-1 (all smoothed
+                          levels when chebLevels <= 0), Jacobi below */
+  int chebLevels;
+  double chebFraction; /* Chebyshev interval [chebFraction lmax, lmax] with
+                          lmax the Gershgorin bound of D^-1 A */
 } or_amg_options;
 
 int or_pcg_amg(int n, const int *rowOffsets, const int *cols,

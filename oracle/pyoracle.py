@@ -50,13 +50,15 @@ class AmgOptions(C.Structure):
     _fields_ = [("maxIter", C.c_int), ("relTol", C.c_double), ("absTol", C.c_double),
                 ("coarseSize", C.c_int), ("maxLevels", C.c_int), ("strength", C.c_double),
                 ("pairPasses", C.c_int), ("jacobiOmega", C.c_double), ("sweeps", C.c_int),
-                ("kcycle", C.c_int)]
+                ("kcycle", C.c_int), ("smoother", C.c_int), ("chebLevels", C.c_int),
+                ("chebFraction", C.c_double)]
 
 
 def amg_options(maxIter=4000, relTol=0.0, absTol=1e-12, coarseSize=256, maxLevels=25,
-                strength=0.25, pairPasses=3, jacobiOmega=0.85, sweeps=1, kcycle=2):
+                strength=0.25, pairPasses=3, jacobiOmega=0.85, sweeps=1, kcycle=2, smoother=0,
+                chebLevels=1, chebFraction=0.3):
     return AmgOptions(maxIter, relTol, absTol, coarseSize, maxLevels, strength, pairPasses,
-                      jacobiOmega, sweeps, kcycle)
+                      jacobiOmega, sweeps, kcycle, smoother, chebLevels, chebFraction)
 
 
 class OMesh:
@@ -210,6 +212,41 @@ def amg_vcycle(ro, ci, va, r, **opts):
     return z, nl
 
 
+def pcg_create(ro, ci, **opts):
+    """The reusable PCG + AMG handle (or_pcg_create): hierarchy kept between
+    solves; pcg_set_values(..., rebuild=False) refreshes only level 0."""
+    L = lib()
+    L.or_pcg_create.restype = C.c_void_p
+    o = amg_options(**opts)
+    ro, ci = _i(ro), _i(ci)
+    h = L.or_pcg_create(len(ro) - 1, _p(ro), _p(ci), C.byref(o))
+    _handles[h] = [ro, ci, None]  # the oracle keeps the arrays' addresses
+    return h
+
+
+def pcg_set_values(h, va, rebuild=True):
+    va = _d(va).copy()
+    _handles[h][2] = va
+    lib().or_pcg_set_values(C.c_void_p(h), _p(va), int(rebuild))
+
+
+def pcg_solve(h, b, x0=None, relTol=0.0, absTol=1e-12, maxIter=4000):
+    ro = _handles[h][0]
+    x = np.zeros(len(ro) - 1) if x0 is None else _d(x0).copy()
+    rr = C.c_double()
+    it = lib().or_pcg_solve(C.c_void_p(h), _p(_d(b)), _p(x), C.c_double(relTol),
+                            C.c_double(absTol), int(maxIter), C.byref(rr))
+    return x, it, rr.value
+
+
+def pcg_destroy(h):
+    lib().or_pcg_destroy(C.c_void_p(h))
+    _handles.pop(h, None)
+
+
+_handles = {}
+
+
 def bicgstab(ro, ci, va, b, x0=None, maxIter=1000, relTol=0.0, absTol=1e-10):
     n = len(ro) - 1
     x = np.zeros(n) if x0 is None else _d(x0).copy()
@@ -240,7 +277,8 @@ def simple_params_from(gp):
     amg = amg_options(maxIter=gp.pMaxit, relTol=gp.pRelTol, absTol=gp.pTol,
                       coarseSize=pr.coarseSize, maxLevels=pr.maxLevels, strength=pr.strength,
                       pairPasses=pr.pairPasses, jacobiOmega=pr.jacobiOmega, sweeps=pr.sweeps,
-                      kcycle=pr.kcycle)
+                      kcycle=pr.kcycle, smoother=pr.smoother, chebLevels=pr.chebLevels,
+                      chebFraction=pr.chebFraction)
     return SimpleParams(gp.rho, gp.mu, gp.nVelNonOrthCorr, gp.nNonOrthCorr, gp.uRelax,
                         gp.pRelax, gp.corrTol, gp.tolMass, gp.tolVel, gp.velMaxit, gp.velTol,
                         gp.velRelTol, gp.pMaxit, gp.pTol, gp.pRelTol, gp.pAmgRebuildEvery, amg)

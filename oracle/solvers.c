@@ -85,6 +85,8 @@ typedef struct {
   double *va;
   int *diag;
   double *omdinv;
+  int cheb;     /* degree-2 Chebyshev smoother on this level */
+  double cc[4]; /* 1/theta, cv, cw, cr of the Chebyshev steps */
   int nc;
   int *agg;  /* n   -> coarse row */
   int *moff; /* nc+1 */
@@ -379,6 +381,60 @@ static void level_diag(or_level *L, double omega) {
   }
 }
 
+/* Degree-2 Chebyshev smoothing for A with D = diag(A) (Adams, Brezina, Hu &
+ * Tuminaro 2003, "Parallel multigrid smoothing: polynomial versus
+ * Gauss-Seidel", and Saad, "Iterative Methods", 12.3).  lmax: the Gershgorin
+ * bound max_i sum_k |a_ik| / |a_ii| of D^-1 A (a maximum: the same number in
+ * any order of evaluation), lmin = fraction lmax; theta = (lmax + lmin)/2,
+ * delta = (lmax - lmin)/2, sigma = theta/delta, rho1 = 1/sigma,
+ * rho2 = 1/(2 sigma - rho1).  Steps from x0:
+ *   x1 = x0 + D^-1 (b - A x0) / theta
+ *   x2 = (1 + rho2 rho1) x1 - rho2 rho1 x0 + (2 rho2/delta) D^-1 (b - A x1)
+ * kept as omdinv = (1/theta) / a_ii (the Jacobi sweep with omega = 1/theta)
+ * and cc = {1/theta, cv = 1 + rho2 rho1, cw = -(rho2 rho1),
+ * cr = (2 rho2/delta) theta}, so that the second step is
+ * (cv x1 + cw x0) + cr (omdinv (b - A x1)). */
+static void cheb_coefficients(double gmax, double fraction, double cc[4]) {
+  const double lmax = gmax > 0.0 ? gmax : 1.0;
+  const double lmin = fraction * lmax;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  const double rho1 = 1.0 / sigma;
+  const double rho2 = 1.0 / (2.0 * sigma - rho1);
+  cc[0] = 1.0 / theta;
+  cc[1] = 1.0 + rho2 * rho1;
+  cc[2] = -(rho2 * rho1);
+  cc[3] = (2.0 * rho2 / delta) * theta;
+}
+
+static double gershgorin_dinv_a(const or_level *L) {
+  double g = 0.0;
+  for (int i = 0; i < L->n; ++i) {
+    double s = 0.0, d = 0.0;
+    for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) {
+      s += fabs(L->va[k]);
+      if (L->ci[k] == i) d = L->va[k];
+    }
+    if (d != 0.0) {
+      const double q = s / fabs(d);
+      if (q > g) g = q;
+    }
+  }
+  return g;
+}
+
+static void level_cheb(or_level *L, double fraction) {
+  cheb_coefficients(gershgorin_dinv_a(L), fraction, L->cc);
+  L->cheb = 1;
+  free(L->diag);
+  free(L->omdinv);
+  level_diag(L, L->cc[0]);
+}
+
+static int level_is_cheb(const or_amg_options *o, int l, const or_level *L) {
+  return o->smoother == 1 && !L->isCoarsest && (o->chebLevels <= 0 || l < o->chebLevels);
+}
+
 static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
                       const double *va, const or_amg_options *o) {
   memset(H, 0, sizeof(*H));
@@ -454,6 +510,8 @@ static void amg_setup(or_amg *H, int n, const int *ro, const int *ci,
     ++l;
   }
   H->nLevels = l + 1;
+  for (int k = 0; k < H->nLevels; ++k)
+    if (level_is_cheb(o, k, &H->lv[k])) level_cheb(&H->lv[k], o->chebFraction);
   or_level *Lc = &H->lv[l];
   if (Lc->n <= 4096) Lc->coarseInv = dense_inverse(Lc->n, Lc->ro, Lc->ci, Lc->va);
   H->xb = (double **)calloc((size_t)H->nLevels, sizeof(double *));
@@ -550,10 +608,23 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
     }
     return;
   }
-  /* pre-smoothing from zero: first sweep is x = omega D^-1 b */
+  /* pre-smoothing from zero: first sweep is x = omega D^-1 b (Chebyshev:
+     the first step, omega = 1/theta) */
   OR_PFOR
   for (int i = 0; i < L->n; ++i) x[i] = L->omdinv[i] * b[i];
-  for (int s = 1; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
+  if (L->cheb) {
+    /* second step from x0 = 0: x2 = cv x1 + cr omdinv (b - A x1) */
+    OR_PFOR
+    for (int i = 0; i < L->n; ++i) {
+      double s = 0.0;
+      for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) s += L->va[k] * x[L->ci[k]];
+      const double t = L->omdinv[i] * (b[i] - s);
+      r[i] = L->cc[1] * x[i] + L->cc[3] * t;
+    }
+    memcpy(x, r, sizeof(double) * (size_t)L->n);
+  } else {
+    for (int s = 1; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
+  }
   /* residual and restriction (member order) */
   OR_PFOR
   for (int i = 0; i < L->n; ++i) {
@@ -575,7 +646,27 @@ static void vcycle(or_amg *H, int l, const double *b, double *x) {
     vcycle(H, l + 1, bc, xc);
   OR_PFOR
   for (int i = 0; i < L->n; ++i) x[i] = x[i] + xc[L->agg[i]];
-  for (int s = 0; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
+  if (L->cheb) {
+    /* x0 = e (in r), x1 = e + omdinv (b - A e), x2 from x1 and e */
+    memcpy(r, x, sizeof(double) * (size_t)L->n);
+    double *x1 = (double *)malloc(sizeof(double) * (size_t)L->n);
+    OR_PFOR
+    for (int i = 0; i < L->n; ++i) {
+      double s = 0.0;
+      for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) s += L->va[k] * r[L->ci[k]];
+      x1[i] = r[i] + L->omdinv[i] * (b[i] - s);
+    }
+    OR_PFOR
+    for (int i = 0; i < L->n; ++i) {
+      double s = 0.0;
+      for (int k = L->ro[i]; k < L->ro[i + 1]; ++k) s += L->va[k] * x1[L->ci[k]];
+      const double t = L->omdinv[i] * (b[i] - s);
+      x[i] = (L->cc[1] * x1[i] + L->cc[2] * r[i]) + L->cc[3] * t;
+    }
+    free(x1);
+  } else {
+    for (int s = 0; s < H->sweeps; ++s) jacobi_sweep(L, b, x, r);
+  }
 }
 
 int or_amg_aggregate(int n, const int *ro, const int *ci, const double *va,
@@ -834,6 +925,7 @@ void or_pcg_set_values(void *hv, const double *va, int rebuild) {
     free(L->diag);
     free(L->omdinv);
     level_diag(L, h->opt.jacobiOmega);
+    if (L->cheb) level_cheb(L, h->opt.chebFraction);
   }
 }
 

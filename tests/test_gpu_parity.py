@@ -166,6 +166,64 @@ def test_spmv_amg_vcycle_bitexact(meshes, omeshes, kind):
 
 
 @pytest.mark.parametrize("kind", ["hex", "cyl"])
+def test_chebyshev_vcycle_bitexact(meshes, omeshes, kind):
+    """The degree-2 Chebyshev smoother (Gershgorin bound, coefficients and
+    the scaled diagonal computed on the GPU): the oracle's V-cycle bit for
+    bit on the finest level only, on every smoothed level, in FP64 and --
+    to FP32 rounding -- in the FP32 cycle of precision 2."""
+    m, om = meshes[kind], omeshes[kind]
+    dirich = ("xmin",) if kind == "hex" else ("patch_2_0",)
+    vals = laplacian(om, m, dirich)
+    x = np.random.default_rng(3).standard_normal(om.nCells)
+    for levels in (1, 0):
+        kw = dict(coarseSize=64, kcycle=0, smoother=1, chebLevels=levels)
+        S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, **kw)
+        S.set_values(vals)
+        z = S.amg_apply(x)
+        zo, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, **kw)
+        assert nl >= 3 and np.array_equal(z, zo), levels
+        zj, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, x, coarseSize=64, kcycle=0)
+        assert np.linalg.norm(zj - z) > 1e-4 * np.linalg.norm(z)
+        # new values without a rebuild: the finest level's bound and
+        # coefficients follow them (the oracle's stale-hierarchy refresh)
+        S.set_values(2.0 * vals, rebuild=False)
+        h = O.pcg_create(om.rowOffsets, om.cols, **kw)
+        try:
+            O.pcg_set_values(h, vals, rebuild=True)
+            O.pcg_set_values(h, 2.0 * vals, rebuild=False)
+            xs = S.solve(x)[0]
+            xo, ito, _ = O.pcg_solve(h, x, relTol=1e-10, absTol=0.0, maxIter=500)
+            assert ito > 0
+        finally:
+            O.pcg_destroy(h)
+        assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_chebyshev_pcg_matches_oracle(meshes, omeshes):
+    """PCG + K-cycle AMG with Chebyshev smoothing on every smoothed level:
+    the oracle's iteration count and residual history (1e-8), FP64 and
+    precision 2."""
+    m, om = meshes["cyl"], omeshes["cyl"]
+    vals = laplacian(om, m, ("patch_2_0",))
+    b = np.random.default_rng(43).standard_normal(om.nCells)
+    kw = dict(coarseSize=64, kcycle=1, smoother=1, chebLevels=0)
+    xo, it, rr, ho = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                               hist_cap=500, **kw)
+    ho = ho[: it + 1]
+    for precision in (0, 2):
+        S = lp.PCGSolver(om.rowOffsets, om.cols, relTol=1e-10, absTol=0.0, precision=precision,
+                         **kw)
+        S.set_values(vals)
+        x, info = S.solve(b)
+        assert info.converged and np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        if precision == 0:
+            h = S.residual_history()
+            assert info.iterations == it and np.all(np.abs(h - ho) <= 1e-8 * ho)
+        else:
+            assert abs(info.iterations - it) <= 2
+
+
+@pytest.mark.parametrize("kind", ["hex", "cyl"])
 def test_kcycle_matches_oracle(meshes, omeshes, kind):
     """K-cycle preconditioner: the oracle's cycle; its inner products are
     two-stage block sums (another summation order than the oracle's serial

@@ -416,6 +416,91 @@ def test_kcycle_level_one_composes_with_the_fine_level():
     assert np.array_equal(z2, 2.0 * z)
 
 
+def _cheb_textbook(A, b, x0, lmin, lmax, degree):
+    """Chebyshev acceleration of Jacobi (Saad, Iterative Methods for Sparse
+    Linear Systems, 2nd ed., Algorithm 12.1 with the preconditioner D):
+    d0 = D^-1 r0 / theta; x_{k+1} = x_k + d_k; rho_{k+1} = 1/(2 sigma - rho_k);
+    d_{k+1} = rho_{k+1} rho_k d_k + (2 rho_{k+1}/delta) D^-1 r_{k+1}."""
+    dinv = 1.0 / A.diagonal()
+    theta, delta = (lmax + lmin) / 2, (lmax - lmin) / 2
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    x = x0.copy()
+    d = dinv * (b - A @ x) / theta
+    for k in range(degree):
+        x = x + d
+        if k == degree - 1:
+            break
+        rho_new = 1.0 / (2.0 * sigma - rho)
+        d = rho_new * rho * d + (2.0 * rho_new / delta) * (dinv * (b - A @ x))
+        rho = rho_new
+    return x
+
+
+def test_chebyshev_recurrence_is_the_chebyshev_polynomial():
+    """The textbook recurrence above multiplies the error of an eigenvector
+    of D^-1 A (eigenvalue lam) by T2((theta - lam)/delta) / T2(theta/delta):
+    on the 1-D Laplacian tridiag(-1, 2, -1) the eigenpairs are
+    lam_k = 1 - cos(k pi/(n+1)), v_k(j) = sin(j k pi/(n+1))."""
+    n = 40
+    A = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+    lmax, lmin = 2.0, 0.6
+    theta, delta = (lmax + lmin) / 2, (lmax - lmin) / 2
+    T2 = lambda t: 2 * t * t - 1
+    j = np.arange(1, n + 1)
+    for k in (1, 7, 20, 33, 40):
+        v = np.sin(j * k * np.pi / (n + 1))
+        lam = 1 - np.cos(k * np.pi / (n + 1))
+        x = _cheb_textbook(A, A @ v, np.zeros(n), lmin, lmax, 2)
+        p2 = T2((theta - lam) / delta) / T2(theta / delta)
+        assert np.abs(x - (1 - p2) * v).max() <= 1e-13 * np.abs(v).max(), k
+
+
+def test_chebyshev_two_level_cycle_matches_textbook_composition():
+    """smoother=1 on level 0 of a two-level hierarchy: the oracle's cycle is
+    the textbook degree-2 Chebyshev pre-smoothing from zero, restriction by
+    the aggregates, the exact coarse solve of the Galerkin matrix and the
+    degree-2 Chebyshev post-smoothing from the corrected iterate, on the
+    interval [0.3 lmax, lmax] with lmax = max_i sum_k |a_ik| / |a_ii|."""
+    om, vals = spd_system(12, 11, 10, seed=9)
+    A = csr(om, vals)
+    n = om.nCells
+    kw = dict(coarseSize=n - 1, maxLevels=2, kcycle=0)
+    agg, nc = O.amg_aggregate(om.rowOffsets, om.cols, vals, **kw)
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, nc))
+    A1 = (P.T @ A @ P).toarray()
+    lmax = (abs(A).sum(axis=1).A1 / np.abs(A.diagonal())).max()
+    lmin = 0.3 * lmax
+    b = np.random.default_rng(10).standard_normal(n)
+    xp = _cheb_textbook(A, b, np.zeros(n), lmin, lmax, 2)
+    e = xp + P @ np.linalg.solve(A1, P.T @ (b - A @ xp))
+    x = _cheb_textbook(A, b, e, lmin, lmax, 2)
+    z, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, b, smoother=1, chebFraction=0.3, **kw)
+    assert nl == 2
+    assert np.linalg.norm(z - x) <= 1e-11 * np.linalg.norm(x)
+    zj, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, b, **kw)
+    assert np.linalg.norm(zj - z) > 1e-3 * np.linalg.norm(z)
+    # still a symmetric positive preconditioner
+    b2 = np.random.default_rng(11).standard_normal(n)
+    z2, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, b2, smoother=1, **kw)
+    assert abs(b2 @ z - b @ z2) <= 1e-12 * abs(b2 @ z) and b @ z > 0
+
+
+def test_pcg_with_chebyshev_smoother_converges():
+    om, vals = spd_system(16, 15, 14)
+    A = csr(om, vals)
+    b = np.random.default_rng(12).standard_normal(om.nCells)
+    xd = spla.spsolve(A.tocsc(), b)
+    its = {}
+    for sm in (0, 1):
+        x, it, rr, _ = O.pcg_amg(om.rowOffsets, om.cols, vals, b, relTol=1e-10, absTol=0.0,
+                                 coarseSize=40, kcycle=1, smoother=sm, chebLevels=0)
+        assert it > 0 and rr <= 1e-10
+        assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+        its[sm] = it
+    assert its[1] <= its[0]
+
+
 def test_pcg_kcycle_converges_in_fewer_iterations():
     om, vals = spd_system(16, 15, 14)
     A = csr(om, vals)
