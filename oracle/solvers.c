@@ -24,8 +24,11 @@
  *     - 'pairPasses' matchings per level (Galerkin sum in between)
  *     - coarse matrix = aggregate sum P^T A P, summed per coarse row in member
  *       order then CSR order
- *     - damped Jacobi pre/post smoothing (omega), dense Cholesky inverse on the
- *       coarsest level.
+ *     - damped Jacobi pre/post smoothing (omega), or (smoother = 1) degree-2
+ *       Chebyshev smoothing in D^-1 A on [chebFraction lmax, lmax], lmax its
+ *       Gershgorin bound, on the chebLevels finest levels; dense Cholesky
+ *       inverse on the coarsest level
+ *     - optionally (kcycle) the K-cycle coarse correction on levels 1..kcycle.
  *   BiCGSTAB with right Jacobi preconditioning, same stop rule.
  *
  * Pinned by tests/test_oracle.py and tests/test_oracle_ops.py: PCG+AMG and
@@ -34,7 +37,14 @@
  * aggregates partition the rows into connected sets with a coarsening
  * ratio above 3; the V-cycle is a linear, symmetric, positive operator, and
  * is the exact inverse when the coarsest level is the matrix itself (both
- * blocked-sum widths of the dense solve).  The aggregation itself defines
+ * blocked-sum widths of the dense solve).  The K-cycle step satisfies the
+ * Galerkin conditions of span{c1, c2} and composes with the fine level as a
+ * scipy restatement does (test_kcycle_*).  The Chebyshev smoother: the
+ * textbook recurrence (Saad, Algorithm 12.1) reproduces the closed-form
+ * error polynomial T2((theta-lam)/delta)/T2(theta/delta) on the 1-D
+ * Laplacian's eigenvectors, and this file's two-level cycle equals that
+ * recurrence composed with an exact Galerkin coarse solve to 1e-11
+ * (test_chebyshev_*).  The aggregation itself defines
  * the algorithm (it replaces HYPRE BoomerAMG), so its exact aggregates have
  * no outside reference: the GPU is checked against this file bit for bit.
  */
