@@ -1120,15 +1120,16 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
   PoolScope scope(&pool);
   for (auto& L : lv) {
     const DevCSR<double>& A = *L.A;
+    // the FP32 matrix shares the FP64 one's pattern and row blocks (A
+    // outlives A32: the fine matrix outlives the hierarchy, a coarse one
+    // is its own level's), so an FP32 cycle and the FP64 SpMV of precision
+    // 2 stream one set of indices
     L.A32.n = A.n;
     L.A32.nnz = A.nnz;
-    L.A32.rowOff.alloc((size_t)A.n + 1);
-    AN_HIP(hipMemcpyAsync(L.A32.rowOff.p, A.rowOff.p, sizeof(int) * ((size_t)A.n + 1),
-                          hipMemcpyDeviceToDevice, s));
-    L.A32.cols.alloc((size_t)A.nnz);
-    AN_HIP(hipMemcpyAsync(L.A32.cols.p, A.cols.p, sizeof(int) * (size_t)A.nnz,
-                          hipMemcpyDeviceToDevice, s));
-    L.A32.finish_pattern(s);
+    L.A32.rowOff.borrow(const_cast<int*>(A.rowOff.p), (size_t)A.n + 1);
+    L.A32.cols.borrow(const_cast<int*>(A.cols.p), (size_t)A.nnz);
+    L.A32.rb.start.borrow(const_cast<int2*>(A.rb.start.p), (size_t)A.rb.count + 1);
+    L.A32.rb.count = A.rb.count;
     L.A32.vals.alloc((size_t)A.nnz);
     k_cast<float><<<grid_for(A.nnz, kB), kB, 0, s>>>(A.nnz, A.vals.p, L.A32.vals.p);
     L.omdinv32.alloc((size_t)L.n);

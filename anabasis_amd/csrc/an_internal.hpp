@@ -143,15 +143,18 @@ struct DevBuf {
   size_t n = 0;
   DevPool* pool = nullptr;  // owner of p when it came from a pool
   size_t bytes = 0;         // pool block size
+  bool borrowed = false;    // p belongs to another buffer (borrow())
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), pool(o.pool), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept
+      : p(o.p), n(o.n), pool(o.pool), bytes(o.bytes), borrowed(o.borrowed) {
     o.p = nullptr;
     o.n = 0;
     o.pool = nullptr;
     o.bytes = 0;
+    o.borrowed = false;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
@@ -160,12 +163,22 @@ struct DevBuf {
       n = o.n;
       pool = o.pool;
       bytes = o.bytes;
+      borrowed = o.borrowed;
       o.p = nullptr;
       o.n = 0;
       o.pool = nullptr;
       o.bytes = 0;
+      o.borrowed = false;
     }
     return *this;
+  }
+  // a view of memory another buffer owns (released without a free); the
+  // owner must outlive it
+  void borrow(T* q, size_t count) {
+    release();
+    p = q;
+    n = count;
+    borrowed = true;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
@@ -182,7 +195,7 @@ struct DevBuf {
   }
   void ensure(size_t count) { if (count > n || !p) alloc(count); }
   void release() {
-    if (p) {
+    if (p && !borrowed) {
       if (pool)
         pool->put(p, bytes);
       else
@@ -192,6 +205,7 @@ struct DevBuf {
     n = 0;
     pool = nullptr;
     bytes = 0;
+    borrowed = false;
   }
   void upload(const T* h, size_t count, hipStream_t s = 0) {
     ensure(count);
