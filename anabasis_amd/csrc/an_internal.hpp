@@ -235,6 +235,17 @@ struct DevAddressing {
 void build_addressing(DevAddressing& A, int nC, int nF, int nIF, const int* d_owner,
                       const int* d_neigh, hipStream_t s);
 
+// The GSV1 least-squares gradient stored as its per-term operator (a source
+// and a coefficient vector per term), laid out for thread-per-cell reads
+// that coalesce: cells in slices of 64 (one wave), term j of the slice's
+// cell i at off[slice] + 64 j + i, a slice as wide as its longest row.
+// src >= 0 is a neighbour cell, src < 0 the boundary face -src - 1.
+struct LsqOperator {
+  DevBuf<int> off, len, src;
+  DevBuf<double> cx, cy, cz;
+  long long slots = 0;
+};
+
 struct DevMesh {
   HostMesh host;
   bool onDevice = false;
@@ -252,6 +263,7 @@ struct DevMesh {
   // non-orthogonal face vector T = A n - (A / n.d) d, internal and boundary
   DevBuf<double> fTx, fTy, fTz, bTx, bTy, bTz;
   DevBuf<double> gsvInv;  // GSV1 LSQ normal-matrix inverse per cell (9 x nC)
+  LsqOperator lsqOp;      // the GSV1 gradient's per-term operator, sliced
   void to_device(hipStream_t s);
 };
 

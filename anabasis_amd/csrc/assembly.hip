@@ -57,6 +57,12 @@ MeshView view_of(const DevMesh& d) {
   m.bTy = d.bTy.p;
   m.bTz = d.bTz.p;
   m.gsvInv = d.gsvInv.p;
+  m.lsqOff = d.lsqOp.off.p;
+  m.lsqLen = d.lsqOp.len.p;
+  m.lsqSrc = d.lsqOp.src.p;
+  m.lsqCx = d.lsqOp.cx.p;
+  m.lsqCy = d.lsqOp.cy.p;
+  m.lsqCz = d.lsqOp.cz.p;
   return m;
 }
 

@@ -17,10 +17,16 @@ struct MeshView {
   const double *bDpn, *bA, *bNx, *bNy, *bNz, *bDx, *bDy, *bDz;
   const double *fTx, *fTy, *fTz, *bTx, *bTy, *bTz;  // T = A n - (A / n.d) d
   const double* gsvInv;  // GSV1 LSQ normal-matrix inverse, 9 x nC SoA (may be null)
+  // GSV1 gradient as a sliced per-term operator (may be null): cells in
+  // slices of 64, term j of cell c at lsqOff[c >> 6] + 64 j + (c & 63)
+  const int *lsqOff, *lsqLen, *lsqSrc;
+  const double *lsqCx, *lsqCy, *lsqCz;
 };
 MeshView view_of(const DevMesh& m);
 // the GSV1 gradient's per-cell normal-matrix inverse (geometry only)
 void build_gsv1_inverse(const MeshView& m, DevBuf<double>& inv9, hipStream_t s);
+// the sliced GSV1 operator from the cached inverse (needs m.gsvInv)
+void build_lsq_operator(const MeshView& m, LsqOperator& op, hipStream_t s);
 
 // ------------------------------------------------ shared device helpers
 // LSQ normal matrix and its inverse (gradient.hip, adjoint.hip)

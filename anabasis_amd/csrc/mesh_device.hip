@@ -109,6 +109,7 @@ void DevMesh::to_device(hipStream_t s) {
     bTy.upload(bty, s);
     bTz.upload(btz, s);
     build_gsv1_inverse(view_of(*this), gsvInv, s);
+    build_lsq_operator(view_of(*this), lsqOp, s);
   }
   AN_HIP(hipStreamSynchronize(s));
   onDevice = true;

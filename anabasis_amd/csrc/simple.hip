@@ -419,8 +419,7 @@ __global__ __launch_bounds__(kB) void k_rc(MeshView m, FaceArgs a, const double*
                                            const double* __restrict__ gy,
                                            const double* __restrict__ gz,
                                            const double* __restrict__ rAU, double rho,
-                                           double* __restrict__ phiStar,
-                                           double* __restrict__ phi) {
+                                           double* __restrict__ phiStar) {
   const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (f >= m.nF) return;
   double out;
@@ -451,8 +450,9 @@ __global__ __launch_bounds__(kB) void k_rc(MeshView m, FaceArgs a, const double*
     const double wf = dir ? a.wB[b] : w[P];
     out = rho * m.bA[b] * (uf * m.bNx[b] + vf * m.bNy[b] + wf * m.bNz[b]);
   }
+  // phi = phi* (main.cu:3439) is not stored: until k_correct_flux rewrites
+  // every face of phi, the loop reads phi* only (its continuity included)
   phiStar[f] = out;
-  phi[f] = out;  // phi = phi* (main.cu:3439)
 }
 
 // CHECK false: no internal face has |n.d| <= 1e-30 (DevMesh::nTinyDpn), so
@@ -502,14 +502,29 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
                                             double* __restrict__ out) {
   const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= m.nC) return;
+  // Latency, not bandwidth, bounds this walk (it runs at full occupancy), so
+  // every load that does not depend on another is issued together: the
+  // cell's bounds, divStar and -- speculatively, clamped in range -- its
+  // first boundary entry; then up to eight incidence entries; then all
+  // their fluxes.  A cell with at most eight internal faces and one
+  // boundary face waits on three memory round trips.  Sums stay in face
+  // order.
+  const int k0 = m.cfOff[c], k1 = m.cfOff[c + 1];
+  const int j0 = m.cbOff[c], j1 = m.cbOff[c + 1];
+  const double ds = (MODE == 1) ? divStar[c] : 0.0;
+  const int bFirst = m.nBF > 0 ? m.cbFace[min(j0, m.nBF - 1)] : 0;
   double d = 0.0;
-  // faces four at a time (entries, then their values), summed in order
-  constexpr int U = 4;
-  const int k1 = m.cfOff[c + 1];
-  for (int k0 = m.cfOff[c]; k0 < k1; k0 += U) {
+  constexpr int U = 8;
+  double bfl = 0.0;
+  bool bOn = false;
+  for (int kb = k0; kb < k1; kb += U) {
     int eu[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) eu[u] = m.cfEnt[min(k0 + u, k1 - 1)];
+    for (int u = 0; u < U; ++u) eu[u] = m.cfEnt[min(kb + u, k1 - 1)];
+    if (kb == k0 && j1 > j0) {  // the first boundary face's flux rides along
+      bfl = flux[m.nIF + bFirst];
+      bOn = MODE == 0 || (bcP[bFirst] == 1 && fabs(m.bDpn[bFirst]) > 1.0e-30);
+    }
     double fl[U];
     bool on[U];
 #pragma unroll
@@ -520,7 +535,7 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (k0 + u >= k1) break;
+      if (kb + u >= k1) break;
       if (!on[u]) continue;
       if (eu[u] & 1)
         d += -fl[u];
@@ -528,12 +543,21 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
         d += fl[u];
     }
   }
-  for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
-    const int b = m.cbFace[j];
-    if (MODE == 1 && !(bcP[b] == 1 && fabs(m.bDpn[b]) > 1.0e-30)) continue;
-    d += flux[m.nIF + b];
+  for (int j = j0; j < j1; ++j) {
+    const int b = (j == j0) ? bFirst : m.cbFace[j];
+    double fv;
+    bool onb;
+    if (j == j0 && k1 > k0) {
+      fv = bfl;
+      onb = bOn;
+    } else {
+      fv = flux[m.nIF + b];
+      onb = MODE == 0 || (bcP[b] == 1 && fabs(m.bDpn[b]) > 1.0e-30);
+    }
+    if (!onb) continue;
+    d += fv;
   }
-  out[c] = (MODE == 0) ? d : (-divStar[c] + d);
+  out[c] = (MODE == 0) ? d : (-ds + d);
 }
 
 __global__ __launch_bounds__(kB) void k_correct_flux(MeshView m, FaceArgs a,
@@ -1076,8 +1100,8 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
     mark(0);
     FaceArgs fa{bcU.p, bcP.p, bcUv[0].p, bcUv[1].p, bcUv[2].p, bcPv.p};
     k_rc<<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, q[0].p, q[1].p, q[2].p, p.p, gp[0].p, gp[1].p,
-                                         gp[2].p, rAU.p, par.rho, phiStar.p, phi.p);
-    k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phi.p, nullptr, divStar.p);
+                                         gp[2].p, rAU.p, par.rho, phiStar.p);
+    k_div<0><<<grid_for(nC, kB), kB, 0, s>>>(mv, bcP.p, phiStar.p, nullptr, divStar.p);
     AN_CHECK_LAUNCH();
     fill_zero(pCorr.p, sizeof(double) * nC, s);
     const int nPass = std::max(par.nNonOrthCorr, 0) + 1;
