@@ -1,9 +1,10 @@
 """Discrete adjoint of the steady scalar transport solve on the GPU.
 
-Checked against (1) an FP64 reference gradient: full central finite
+Checked against (1) an FP64 reference gradient: central finite
 differences of the CPU oracle's forward model (oracle LSQ gradient + oracle
 transport assembly + a sparse direct solve, iterated to the converged
-non-orthogonal fixed point), (2) central finite differences of the GPU
+non-orthogonal fixed point), in full on 36 cells and along random
+directions on 960 cells, (2) central finite differences of the GPU
 forward model, and (3) the transpose identity of the least-squares gradient;
 and wrapped in a torch.autograd.Function."""
 import numpy as np
@@ -112,6 +113,32 @@ def test_adjoint_matches_fp64_reference_gradient(gpu):
     assert abs(J0) > 0
     assert np.linalg.norm(dk - rdk) <= 1e-7 * np.linalg.norm(rdk), (dk, rdk)
     assert np.linalg.norm(db - rdb) <= 1e-7 * np.linalg.norm(rdb)
+
+
+def test_adjoint_matches_fp64_reference_directions(gpu):
+    """The gradient on a 960-cell mesh (all k and every boundary value) along
+    random directions against central differences, eps = 1e-6, of the CPU
+    oracle's forward model."""
+    m, flux, k, types, vals = make_case((12, 10, 8), 11)
+    om = O.OMesh.from_product(m)
+    Jf, dJ = ad.outlet_functional(m, "xmax", weight=lambda x: 1.0 + x[:, 1])
+    P = ad.TransportProblem(m, flux, types, nNonOrthCorr=60, absTol=1e-15)
+    phi = P.solve(k, vals)
+    assert np.linalg.norm(phi - oracle_forward(om, flux, k, types, vals)) <= \
+        1e-11 * np.linalg.norm(phi)
+    dk, db, _ = P.adjoint(k, vals, phi, dJ(phi))
+    rng = np.random.default_rng(3)
+    eps = 1e-6
+    for _ in range(3):
+        xi = rng.uniform(-1.0, 1.0, m.nCells) * k
+        fd = (Jf(oracle_forward(om, flux, k + eps * xi, types, vals)) -
+              Jf(oracle_forward(om, flux, k - eps * xi, types, vals))) / (2 * eps)
+        assert abs(fd - dk @ xi) <= 1e-6 * (np.abs(dk) @ np.abs(xi)), (fd, dk @ xi)
+    for _ in range(2):
+        eta = rng.uniform(-1.0, 1.0, m.nBoundaryFaces)
+        fd = (Jf(oracle_forward(om, flux, k, types, vals + eps * eta)) -
+              Jf(oracle_forward(om, flux, k, types, vals - eps * eta))) / (2 * eps)
+        assert abs(fd - db @ eta) <= 1e-6 * (np.abs(db) @ np.abs(eta)), (fd, db @ eta)
 
 
 def test_adjoint_matches_gpu_finite_differences(gpu):
