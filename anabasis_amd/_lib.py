@@ -79,6 +79,15 @@ class SimpleParams(C.Structure):
                 ("fuseMomentumRhs", C.c_int), ("pressure", PcgOptions)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("rowsPerWave", C.c_int), ("rowSeconds", C.c_double * 3), ("launch", C.c_int),
+                ("persistentBlocksPerCU", C.c_int), ("launchSeconds", C.c_double * 4),
+                ("launchesLastSolve", C.c_int)]
+
+
+AN_BATCH_AUTO, AN_BATCH_EAGER, AN_BATCH_GRAPH, AN_BATCH_PERSISTENT = 0, 1, 2, 3
+
+
 class SimpleStats(C.Structure):
     _fields_ = [("step", C.c_int), ("massRes", C.c_double), ("duRel", C.c_double),
                 ("dvRel", C.c_double), ("dwRel", C.c_double), ("dpRel", C.c_double),
@@ -141,6 +150,8 @@ SIGNATURES = {
     "an_batch_solve": (_I, [_P, _P, _P, _I, C.POINTER(SolveInfo)]),
     "an_batch_tuning": (_I, [_P, _pI, _P]),
     "an_batch_use_graph": (_I, [_P, _I]),
+    "an_batch_set_launch": (_I, [_P, _I]),
+    "an_batch_get_tuning": (_I, [_P, C.c_void_p]),
     "an_batch_destroy": (_I, [_P]),
     "an_transport_adjoint": (_I, [_P, _P, _P, _P, _P, _P, _P, C.POINTER(TransportOptions), _P,
                                   _P, _P, _P, _P, _I, C.POINTER(SolveInfo)]),

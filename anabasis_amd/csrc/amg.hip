@@ -1152,32 +1152,6 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
   has32 = true;
 }
 
-// Per-precision views of a level's matrix and work vectors.
-template <typename T>
-struct LevelData;
-template <>
-struct LevelData<double> {
-  static const DevCSR<double>& A(const AmgLevel& L) { return *L.A; }
-  static const double* omd(const AmgLevel& L) { return L.omdinv.p; }
-  static double* b(AmgLevel& L) { return L.b.p; }
-  static double* x(AmgLevel& L) { return L.x.p; }
-  static double* r(AmgLevel& L) { return L.r.p; }
-  static double* y(AmgLevel& L) { return L.y.p; }
-  static double* kc(AmgLevel& L) { return L.kc.p; }
-  static double* kv(AmgLevel& L) { return L.kv.p; }
-};
-template <>
-struct LevelData<float> {
-  static const DevCSR<float>& A(const AmgLevel& L) { return L.A32; }
-  static const float* omd(const AmgLevel& L) { return L.omdinv32.p; }
-  static float* b(AmgLevel& L) { return L.b32.p; }
-  static float* x(AmgLevel& L) { return L.x32.p; }
-  static float* r(AmgLevel& L) { return L.r32.p; }
-  static float* y(AmgLevel& L) { return L.y32.p; }
-  static float* kc(AmgLevel& L) { return L.kc32.p; }
-  static float* kv(AmgLevel& L) { return L.kv32.p; }
-};
-
 // x = cycle(b) on level l; y = omdinv o b (computed here when null).
 // sweeps > 1: x starts at y (the first sweep from zero) and takes sweeps-1
 // further Jacobi sweeps before the residual; after the coarse correction

@@ -41,6 +41,32 @@ struct AmgLevel {
   bool isCoarsest = false;
 };
 
+// Per-precision views of a level's matrix and work vectors.
+template <typename T>
+struct LevelData;
+template <>
+struct LevelData<double> {
+  static const DevCSR<double>& A(const AmgLevel& L) { return *L.A; }
+  static const double* omd(const AmgLevel& L) { return L.omdinv.p; }
+  static double* b(AmgLevel& L) { return L.b.p; }
+  static double* x(AmgLevel& L) { return L.x.p; }
+  static double* r(AmgLevel& L) { return L.r.p; }
+  static double* y(AmgLevel& L) { return L.y.p; }
+  static double* kc(AmgLevel& L) { return L.kc.p; }
+  static double* kv(AmgLevel& L) { return L.kv.p; }
+};
+template <>
+struct LevelData<float> {
+  static const DevCSR<float>& A(const AmgLevel& L) { return L.A32; }
+  static const float* omd(const AmgLevel& L) { return L.omdinv32.p; }
+  static float* b(AmgLevel& L) { return L.b32.p; }
+  static float* x(AmgLevel& L) { return L.x32.p; }
+  static float* r(AmgLevel& L) { return L.r32.p; }
+  static float* y(AmgLevel& L) { return L.y32.p; }
+  static float* kc(AmgLevel& L) { return L.kc32.p; }
+  static float* kv(AmgLevel& L) { return L.kv32.p; }
+};
+
 // Inner products finished by a preconditioner application z = M^-1 r:
 // out[0] = r.z and, when q is given (same precision as r), out[1] = z.q.
 struct ApplyDots {

@@ -1,5 +1,6 @@
 // batch.hpp -- many independent PCG+AMG solves in one block-diagonal system,
-// iterated by a captured HIP graph.
+// iterated by one persistent kernel launch (captured in a HIP graph) or by
+// a captured graph of a few iterations replayed until every problem is done.
 #pragma once
 
 #include <vector>
@@ -41,8 +42,17 @@ class BatchSolver {
   // autotuner: rows per block tried, their SpMV+dot time (s), choice
   std::vector<int> tuneRows;
   std::vector<double> tuneSeconds;
+  // launch: AN_BATCH_EAGER, AN_BATCH_GRAPH or AN_BATCH_PERSISTENT, or
+  // AN_BATCH_AUTO (the tuner's choice, `chosen`); the tuner times one PCG
+  // iteration of graph replay and of the persistent kernel at
+  // tuneBlocksPerCU workgroups per CU (launchSeconds, same order)
+  int launch = 0;
+  int chosen = 2;
+  int persistBlocksPerCU = 0;     // chosen grid of the persistent kernel
+  std::vector<int> tuneBlocksPerCU;
+  std::vector<double> launchSeconds;
   int graphIters = 0;             // iterations per captured graph launch
-  bool useGraph = true;           // false: the same launches issued eagerly
+  int lastLaunches = 0;           // kernel-sequence launches of the last solve
   double lastSolveSeconds = 0.0;
 
  private:
@@ -53,7 +63,32 @@ class BatchSolver {
   int* hAct = nullptr;  // pinned
   double* hScal = nullptr;  // pinned, nP * S_COUNT
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  // the persistent kernel: barrier words, its arguments and level views in
+  // device memory (per precision), the coarsest dense solve's chunk sums,
+  // and the captured {barrier reset, kernel} graph per precision
+  struct PersistGraph {
+    hipGraphExec_t exec = nullptr;
+    int grid = 0, maxIter = 0;
+    DevBuf<char> args, levels;
+    void drop();
+  };
+  DevBuf<unsigned> bar;
+  DevBuf<double> dpart;
+  PersistGraph gp64, gp32;
+  int cus = 0, maxPerCU64 = 0, maxPerCU32 = 0;
+  long long waitTicks = 0;
   bool valuesSet = false;
+
+  int mode() const;  // the launch a solve uses
+  template <typename T>
+  void build_persist(const DevCSR<T>& M, T* xv, T* rv, T* zv, T* pv, T* qv, int tolOff,
+                     int maxIter, PersistGraph& g);
+  // (re)captures g for this grid and iteration cap when needed, launches it
+  template <typename T>
+  void launch_persist(const DevCSR<T>& M, T* xv, T* rv, T* zv, T* pv, T* qv, int tolOff,
+                      PersistGraph& g, int blocksPerCU, int maxIter);
+  void check_persist();  // synchronises; throws when a barrier wait gave up
+  void tune_launch();
 
   void build_layout(int rows, BatchLayout& L);
   double time_layout(const BatchLayout& L);
@@ -65,7 +100,7 @@ class BatchSolver {
   hipGraphExec_t capture(const DevCSR<T>& M, T* xv, T* rv, T* zv, T* pv, T* qv);
   template <typename T>
   int run_pcg(const DevCSR<T>& M, T* xv, T* rv, T* zv, T* pv, T* qv, hipGraphExec_t& g,
-              int maxIter);
+              PersistGraph& gp, int maxIter);
   void residual64();  // r = b - A x, S_RR per problem
   void finalize(int slot);
   void read_scalars();

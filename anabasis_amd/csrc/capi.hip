@@ -1090,7 +1090,33 @@ an_status an_batch_tuning(an_batch_t b, int* rowsPerWave, double* seconds) {
 an_status an_batch_use_graph(an_batch_t b, int enable) {
   AN_TRY
   if (!b) throw Error(AN_EINVAL, "NULL argument");
-  b->b->useGraph = enable != 0;
+  b->b->launch = enable ? AN_BATCH_AUTO : AN_BATCH_EAGER;
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_set_launch(an_batch_t b, int launch) {
+  AN_TRY
+  if (!b) throw Error(AN_EINVAL, "NULL argument");
+  if (launch < AN_BATCH_AUTO || launch > AN_BATCH_PERSISTENT)
+    throw Error(AN_EINVAL, "unknown batch launch mode");
+  b->b->launch = launch;
+  return AN_OK;
+  AN_CATCH
+}
+
+an_status an_batch_get_tuning(an_batch_t b, an_batch_tuning_info* out) {
+  AN_TRY
+  if (!b || !out) throw Error(AN_EINVAL, "NULL argument");
+  const BatchSolver& s = *b->b;
+  *out = an_batch_tuning_info{};
+  out->rowsPerWave = s.lay.rows;
+  for (size_t i = 0; i < 3 && i < s.tuneSeconds.size(); ++i) out->rowSeconds[i] = s.tuneSeconds[i];
+  out->launch = s.chosen;
+  out->persistentBlocksPerCU = s.persistBlocksPerCU;
+  for (size_t i = 0; i < 4 && i < s.launchSeconds.size(); ++i)
+    out->launchSeconds[i] = s.launchSeconds[i];
+  out->launchesLastSolve = s.lastLaunches;
   return AN_OK;
   AN_CATCH
 }

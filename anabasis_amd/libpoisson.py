@@ -374,8 +374,9 @@ def bicgstab3(rowOffsets, cols, values, bs, x0s=None, maxIter=1000, relTol=0.0, 
 
 class BatchPCG:
     """Many independent SPD systems solved together (an_batch_*): one
-    block-diagonal CSR, one AMG hierarchy, per-problem PCG scalars, the
-    iteration replayed from a captured HIP graph."""
+    block-diagonal CSR, one AMG hierarchy, per-problem PCG scalars; the
+    iterations run as one persistent launch or a replayed HIP graph, as the
+    autotuner chose (set_launch overrides)."""
 
     def __init__(self, patterns, **opts):
         self.ro = [i32(r) for r, _ in patterns]
@@ -408,6 +409,17 @@ class BatchPCG:
 
     def use_graph(self, enable=True):
         check(lib().an_batch_use_graph(self._h, int(enable)))
+
+    def set_launch(self, mode):
+        """L.AN_BATCH_AUTO / _EAGER / _GRAPH / _PERSISTENT."""
+        check(lib().an_batch_set_launch(self._h, int(mode)))
+
+    def tuning_info(self):
+        t = L.BatchTuning()
+        check(lib().an_batch_get_tuning(self._h, C.byref(t)))
+        return dict(rowsPerWave=t.rowsPerWave, rowSeconds=list(t.rowSeconds), launch=t.launch,
+                    persistentBlocksPerCU=t.persistentBlocksPerCU,
+                    launchSeconds=list(t.launchSeconds), launchesLastSolve=t.launchesLastSolve)
 
     def tuning(self):
         rows = C.c_int()

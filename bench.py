@@ -338,7 +338,9 @@ def run_batch_mms(nprob, precision):
     blocks x 6 tets, n log-uniform in [8, 64] (seeded), each with a random
     smooth manufactured solution u = sum_k a_k sin(pi w_k.x + c_k); solved
     together by the batched PCG+AMG (graph-replayed iteration) to 1e-10
-    relative, and one after another by the single-system solver."""
+    relative -- with the launch the autotuner chose (one persistent kernel
+    launch per solve, or a replayed graph of four iterations), and with each
+    launch forced -- and one after another by the single-system solver."""
     from anabasis_amd import libpoisson as lp
     rng = np.random.default_rng(SEED + 3)
     sizes = np.round(np.exp(rng.uniform(np.log(8), np.log(64), nprob))).astype(int)
@@ -372,6 +374,18 @@ def run_batch_mms(nprob, precision):
     xs, infos = B.solve(rhs)
     t_batch = time.time() - t0
     rows, secs = B.tuning()
+    tune = B.tuning_info()
+    from anabasis_amd import _lib as L
+    by_launch, same = {}, True
+    for name, mode in (("persistent", L.AN_BATCH_PERSISTENT), ("graph", L.AN_BATCH_GRAPH)):
+        B.set_launch(mode)
+        B.solve(rhs)  # warm: captures this launch's graph
+        t0 = time.time()
+        xm, _ = B.solve(rhs)
+        by_launch[name] = {"solve_s": time.time() - t0,
+                           "launches": B.tuning_info()["launchesLastSolve"]}
+        same = same and all(np.array_equal(a, b) for a, b in zip(xm, xs))
+    B.set_launch(L.AN_BATCH_AUTO)
     # the same problems one at a time
     t_seq = 0.0
     worst = 0.0
@@ -386,16 +400,31 @@ def run_batch_mms(nprob, precision):
         del S
     its = [i.iterations for i in infos]
     conv = all(i.converged for i in infos)
-    log(f"batch MMS prec={precision}: setup {setup:.2f}s batched solve {t_batch:.3f}s vs "
-        f"one-by-one {t_seq:.3f}s; its {min(its)}..{max(its)} converged={conv} "
-        f"rows/wave {rows} tuning {[round(t * 1e6, 1) for t in secs]}us")
+    lname = {L.AN_BATCH_GRAPH: "graph", L.AN_BATCH_PERSISTENT: "persistent"}
+    log(f"batch MMS prec={precision}: setup {setup:.2f}s batched solve {t_batch:.3f}s "
+        f"({lname.get(tune['launch'])} launch chosen; persistent "
+        f"{by_launch['persistent']['solve_s']:.3f}s in {by_launch['persistent']['launches']} "
+        f"launch(es), graph {by_launch['graph']['solve_s']:.3f}s in "
+        f"{by_launch['graph']['launches']}; same bits {same}) vs one-by-one {t_seq:.3f}s; its "
+        f"{min(its)}..{max(its)} converged={conv} rows/wave {rows} tuning "
+        f"{[round(t * 1e6, 1) for t in secs]}us, per-iteration graph/persistent x1,2,4 per CU "
+        f"{[round(t * 1e6, 1) for t in tune['launchSeconds']]}us")
     return dict(problems=nprob, cells=cells, precision=precision, setup_s=setup,
                 batched_solve_s=t_batch, one_by_one_solve_s=t_seq,
                 iterations_min=min(its), iterations_max=max(its), all_converged=conv,
                 max_rel_diff_vs_single=worst, autotuner_rows_per_wave=rows,
                 autotuner_spmv_s={"64": secs[0], "32": secs[1], "16": secs[2]},
+                autotuner_launch=lname.get(tune["launch"]),
+                autotuner_persistent_blocks_per_cu=tune["persistentBlocksPerCU"],
+                autotuner_iteration_s={"graph": tune["launchSeconds"][0],
+                                       "persistent_1_per_cu": tune["launchSeconds"][1],
+                                       "persistent_2_per_cu": tune["launchSeconds"][2],
+                                       "persistent_4_per_cu": tune["launchSeconds"][3]},
+                by_launch=by_launch, launches_bitwise_equal=same,
                 note="solve times exclude host->device copies of the values; "
-                     "batched = one graph-replayed iteration sequence for all problems")
+                     "persistent = one kernel launch (in a captured HIP graph) runs every "
+                     "iteration of every problem; graph = a captured graph of four iterations "
+                     "replayed until every problem is done")
 
 
 # ----------------------------------------------------------- adjoint ----

@@ -331,8 +331,8 @@ an_status an_solve_steady_scalar_transport(
  * apps/poisson_mms_lib run on a set of meshes) solved together: one
  * block-diagonal CSR, one aggregation-AMG hierarchy that never couples
  * problems, and per-problem PCG scalars, stop tests and iteration counts.
- * The PCG iteration is captured once in a HIP graph and replayed until
- * every problem has converged.  No reference counterpart (the reference
+ * The PCG iterations run as one persistent launch or as a replayed HIP
+ * graph (an_batch_set_launch).  No reference counterpart (the reference
  * solves one system per HYPRE call, LP/src/hypre_backend.cu:3-114).
  * opt: as an_pcg_create (precision 0 or 1; 1 = FP32 PCG+AMG inside FP64
  * refinement).  Patterns are copied. */
@@ -353,9 +353,38 @@ an_status an_batch_solve(an_batch_t b, const double *const *rhs,
 /* Autotuner record: chosen rows per wave, and the timed SpMV seconds of the
  * three candidates (64, 32, 16 rows) in seconds[3]. */
 an_status an_batch_tuning(an_batch_t b, int *rowsPerWave, double *seconds);
-/* enable != 0 (default): replay the captured graph; 0: issue the same
- * launches one by one (a check that the graph reproduces eager results). */
+/* enable != 0 (default): the tuner's launch (AN_BATCH_AUTO); 0: issue the
+ * launches one by one (AN_BATCH_EAGER). */
 an_status an_batch_use_graph(an_batch_t b, int enable);
+/* How a batched solve runs its PCG iterations; every mode gives the same
+ * bits.  AN_BATCH_PERSISTENT: ONE kernel launch (captured once in a HIP
+ * graph) whose co-resident workgroups run every phase of every iteration
+ * -- SpMV, per-problem sums and stop tests, each V-cycle level, direction
+ * -- separated by grid barriers, until no problem is active.  Its barrier
+ * waits are bounded (2 s each): if the workgroups cannot all be resident
+ * (another kernel holding the GPU that long), the solve returns AN_EHIP
+ * instead of waiting.  AN_BATCH_GRAPH: a captured graph of four iterations
+ * replayed until no problem is active (one host read per replay).
+ * AN_BATCH_AUTO (default): the faster of the two on this matrix, timed by
+ * the tuner in an_batch_set_values. */
+#define AN_BATCH_AUTO 0
+#define AN_BATCH_EAGER 1
+#define AN_BATCH_GRAPH 2
+#define AN_BATCH_PERSISTENT 3
+an_status an_batch_set_launch(an_batch_t b, int launch);
+/* The autotuner's record (an_batch_set_values): block height chosen and
+ * the SpMV time of each candidate; launch chosen and the time of one PCG
+ * iteration by graph replay and by the persistent kernel at 1, 2 and 4
+ * workgroups per CU (0 where that grid cannot be resident). */
+typedef struct {
+  int rowsPerWave;
+  double rowSeconds[3];      /* 64, 32, 16 rows per wave */
+  int launch;                /* AN_BATCH_GRAPH or AN_BATCH_PERSISTENT */
+  int persistentBlocksPerCU; /* grid of the persistent kernel */
+  double launchSeconds[4];   /* graph, persistent at 1, 2, 4 per CU */
+  int launchesLastSolve;     /* kernel-sequence launches of the last solve */
+} an_batch_tuning_info;
+an_status an_batch_get_tuning(an_batch_t b, an_batch_tuning_info *out);
 an_status an_batch_destroy(an_batch_t b);
 /* --------------------------------------------------------------- adjoint */
 /* Discrete adjoint of libscalar::solve_steady_scalar_transport

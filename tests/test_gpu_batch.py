@@ -1,10 +1,12 @@
 """Batched PCG + AMG (an_batch_*): many uneven Poisson MMS systems in one
-block-diagonal solve iterated by a captured HIP graph."""
+block-diagonal solve, iterated by one persistent launch or a replayed HIP
+graph (the same bits either way)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from anabasis_amd import _lib as L
 from anabasis_amd import libpoisson as lp
 from oracle import pyoracle as O
 
@@ -86,3 +88,40 @@ def test_batch_mixed_precision(problems):
             A = sp.csr_matrix((vals, om.cols, om.rowOffsets))
             assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
             assert inf.refinementSteps >= 1
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_persistent_launch_equals_graph_and_eager(problems, precision):
+    """One persistent launch per (inner) solve gives the bits of the replayed
+    graph and of the eager launches, twice in a row; the tuner timed every
+    candidate launch."""
+    pats = [(om.rowOffsets, om.cols) for om, _, _ in problems]
+    rhs = [b for _, _, b in problems]
+    B = lp.BatchPCG(pats, relTol=1e-10, absTol=0.0, precision=precision)
+    B.set_values([v for _, v, _ in problems])
+    t = B.tuning_info()
+    assert t["launch"] in (L.AN_BATCH_GRAPH, L.AN_BATCH_PERSISTENT)
+    assert t["launchSeconds"][0] > 0 and t["launchSeconds"][1] > 0
+    assert t["persistentBlocksPerCU"] >= 1
+    out = {}
+    for mode in (L.AN_BATCH_PERSISTENT, L.AN_BATCH_PERSISTENT, L.AN_BATCH_GRAPH,
+                 L.AN_BATCH_EAGER):
+        B.set_launch(mode)
+        xs, infos = B.solve(rhs)
+        if mode == L.AN_BATCH_PERSISTENT:
+            steps = max(i.refinementSteps for i in infos) if precision == 1 else 1
+            assert B.tuning_info()["launchesLastSolve"] == steps
+        its = [i.iterations for i in infos]
+        if mode in out:
+            assert its == out[mode][1]
+            assert all(np.array_equal(a, b) for a, b in zip(xs, out[mode][0]))
+        out[mode] = (xs, its)
+    ref_x, ref_its = out[L.AN_BATCH_EAGER]
+    for mode in (L.AN_BATCH_PERSISTENT, L.AN_BATCH_GRAPH):
+        assert out[mode][1] == ref_its
+        for a, b in zip(out[mode][0], ref_x):
+            assert np.array_equal(a, b)
+    for (om, vals, b), x in zip(problems, ref_x):
+        if np.any(b):
+            A = sp.csr_matrix((vals, om.cols, om.rowOffsets))
+            assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
