@@ -151,7 +151,7 @@ void Reducer::init(int maxBlocks, hipStream_t s) {
 
 // ------------------------------------------------------------------ SpMV
 // MODE 0: y = A x ; MODE 1: y = b - A x.  DOT 0: none; 1: dot(dw, y);
-// 2: dot(y, y).
+// 2: dot(y, y); 3 (MODE 1): dot(y, y) and dot(b, b), two partials per block.
 template <typename VT, typename XT, typename YT, int MODE, int DOT>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict__ rbs,
                                                  const int* __restrict__ rowOff,
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
                                                  const double* __restrict__ b, YT* __restrict__ y,
                                                  const YT* __restrict__ dw, RedArgs ra) {
   __shared__ double prod[kWaves][kCap];
-  __shared__ double sh[kWaves];
+  __shared__ double sh[2 * kWaves];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kWaves + w);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
   if (q.stream && q.k1 > q.k0)
     stage_products(q, lane, cols, vals, [&](int j) { return (double)x[j]; }, prod[w]);
   wave_sync();
-  double dv = 0.0;
+  double dv = 0.0, db = 0.0;
   if (row < q.r1) {
     double s = 0.0;
     if (q.stream)
@@ -193,9 +193,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
       v = (YT)s;
     y[row] = v;
     if (DOT == 1) dv = wi * (double)v;
-    if (DOT == 2) dv = (double)v * (double)v;
+    if (DOT >= 2) dv = (double)v * (double)v;
+    if (DOT == 3) db = bi * bi;
   }
-  if (DOT) {
+  if (DOT == 3) {
+    double d2[2] = {dv, db};
+    block_sums<2>(d2, sh);
+    store_partials<2>(d2, lb, ra);
+  } else if (DOT) {
     const double bs = block_sum(dv, sh);
     store_partials<1>(&bs, lb, ra);
   }
@@ -250,6 +255,29 @@ void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, doub
                                                      red.args(scal + S_RR));
   AN_CHECK_LAUNCH();
   finalize(red, nb, 1, scal + S_RR, s);
+}
+
+template <typename VT, typename RT>
+int residual_bb(const DevCSR<VT>& A, const double* b, const double* x, RT* r, Reducer& red,
+                hipStream_t s) {
+  const int nb = spmv_grid(A);
+  if (2 * nb > 4 * red.cap) throw Error(7, "reducer too small");
+  k_spmv<VT, double, RT, 1, 3><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
+                                                     A.cols.p, A.vals.p, x, b, r, nullptr,
+                                                     red.args(nullptr));
+  AN_CHECK_LAUNCH();
+  return nb;
+}
+
+template <typename VT, typename XT>
+int spmv_self_dot(const DevCSR<VT>& A, const XT* x, XT* y, Reducer& red, hipStream_t s) {
+  const int nb = spmv_grid(A);
+  if (nb > red.cap) throw Error(7, "reducer too small");
+  k_spmv<VT, XT, XT, 0, 2><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
+                                                 A.vals.p, x, nullptr, y, nullptr,
+                                                 red.args(nullptr));
+  AN_CHECK_LAUNCH();
+  return nb;
 }
 
 // ---------------------------------------------------------- vector ops
@@ -485,6 +513,10 @@ template void spmv<double, double>(const DevCSR<double>&, const double*, double*
                                    double*, int, Reducer&, hipStream_t, hipEvent_t, hipEvent_t);
 template void spmv<float, float>(const DevCSR<float>&, const float*, float*, const float*, double*,
                                  int, Reducer&, hipStream_t, hipEvent_t, hipEvent_t);
+template int residual_bb<double, double>(const DevCSR<double>&, const double*, const double*,
+                                         double*, Reducer&, hipStream_t);
+template int spmv_self_dot<double, double>(const DevCSR<double>&, const double*, double*,
+                                           Reducer&, hipStream_t);
 template void residual<double, double>(const DevCSR<double>&, const double*, const double*,
                                        double*, double*, Reducer&, hipStream_t);
 template void residual<double, float>(const DevCSR<double>&, const double*, const double*, float*,

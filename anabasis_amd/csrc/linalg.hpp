@@ -116,6 +116,7 @@ enum ScalarSlot { S_PQ = 0, S_RR = 1, S_RHO0 = 2, S_RHO1 = 3, S_BB = 4, S_TMP = 
                   S_FRZ = 13, S_FZQ = 14,  // flexible CG: r.z, z.q
                   S_PROJ = 16,             // 16 projection coefficients
                   S_NU = 32,               // ||A w||^2 of a new direction, then ||A x||^2
+                  S_RB = 34,               // a solve's first r.r and b.b
                   S_COUNT = 40 };
 
 // Deterministic reduction workspace: per-block partials, summed in index
@@ -212,6 +213,15 @@ template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,
                    hipStream_t s, double sign = 1.0);
 // r = b - A x ; scal[S_RR] = r.r (FP64 residual, output converted to RT)
+// r = b - A x with the block partials of r.r and b.b (two per block) left
+// in red.partial; returns the number of blocks (the caller finalizes)
+template <typename VT, typename RT>
+int residual_bb(const DevCSR<VT>& A, const double* b, const double* x, RT* r, Reducer& red,
+                hipStream_t s);
+// y = A x with the block partials of y.y left in red.partial; returns the
+// number of blocks
+template <typename VT, typename XT>
+int spmv_self_dot(const DevCSR<VT>& A, const XT* x, XT* y, Reducer& red, hipStream_t s);
 template <typename VT, typename RT>
 void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, double* scal,
               Reducer& red, hipStream_t s);

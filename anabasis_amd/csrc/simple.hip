@@ -189,14 +189,35 @@ __global__ __launch_bounds__(kB) void k_mom_faces(MeshView m, Mom3Args a,
   }
 }
 
+// FULL: one wave per CSR row block, lane i on row i; the block's matrix
+// values are built in an LDS image and stored with unit stride (the rows'
+// slot writes scattered partial cache lines).  A block whose nonzeros
+// exceed the slice writes global memory directly (same order, same bits).
+// rhs-only: one thread per cell.
+static constexpr int kMomW = kB / 64;
 template <bool FULL>
 __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
                                                   const double* __restrict__ corr,
                                                   const double* __restrict__ coef,
                                                   double* __restrict__ vals,
-                                                  double* __restrict__ rAU) {
-  const int c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-  if (c >= m.nC) return;
+                                                  double* __restrict__ rAU, int nrb,
+                                                  const int2* __restrict__ rbs) {
+  __shared__ double sv[FULL ? kMomW : 1][FULL ? kStreamCap : 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  WaveRows wr{0, 0, 0, 0, false};
+  int c;
+  if (FULL) {
+    wr = wave_rows(nrb, rbs, xcd_block(gridDim.x) * kMomW + w);
+    if (wr.r1 <= wr.r0) return;  // wave-uniform; only wave-level sync below
+    c = wr.r0 + lane;
+  } else {
+    c = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= m.nC) return;
+  }
+  const bool stage = FULL && wr.stream;
+  // the slot of CSR position k: the LDS image or global memory
+  auto slotp = [&](int k) -> double* { return stage ? &sv[w][k - wr.k0] : &vals[k]; };
+  if (!FULL || c < wr.r1) {
   const double V = m.cgeo[c].w;
   const size_t n = (size_t)m.nIF;
   double diag = 0.0;
@@ -233,13 +254,14 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
       if (k0 + u >= k1) break;
       const int slotInfo = su[u];
       if (nd[u] <= 1e-14) {  // main.cu:1826 -- the face contributes nothing
-        if (FULL && (slotInfo & 1)) vals[slotInfo >> 1] = 0.0;
+        if (FULL && (slotInfo & 1)) *slotp(slotInfo >> 1) = 0.0;
         continue;
       }
       if (FULL) {
-        const double old = (slotInfo & 1) ? 0.0 : vals[slotInfo >> 1];
+        double* t = slotp(slotInfo >> 1);
+        const double old = (slotInfo & 1) ? 0.0 : *t;
         diag += cd[u];
-        vals[slotInfo >> 1] = old + co[u];
+        *t = old + co[u];
       }
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -296,7 +318,7 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
       aR = diag * invRelax;
       if (a.active & 1) r[0] += (invRelax - 1.0) * diag * a.qIt[0][c];
     }
-    vals[dp] = aR;
+    *slotp(dp) = aR;
     rAU[c] = (fabs(aR) > 1e-30) ? V / aR : 0.0;
     aPold = aR * a.uRelax;
   } else {
@@ -307,6 +329,11 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
     if (!((a.active >> q) & 1)) continue;
     if (relax && !(FULL && q == 0)) r[q] += (invRelax - 1.0) * aPold * a.qIt[q][c];
     a.rhs[q][c] = r[q];
+  }
+  }
+  if (FULL && stage) {  // every slot of the block was written by its row
+    wave_sync();
+    for (int k = wr.k0 + lane; k < wr.k1; k += 64) vals[k] = sv[w][k - wr.k0];
   }
 }
 
@@ -1029,10 +1056,12 @@ void SimpleSolver::momentum_fused(an_simple_stats* st) {
     const int Gf = grid_for(M.nIF, kB), Gc = grid_for(nC, kB);
     if (it == 1) {  // active == 7 here: the x component builds the matrix
       k_mom_faces<true><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
-      k_mom_cells<true><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p);
+      k_mom_cells<true><<<grid_for(Amom.rb.count, kMomW), kB, 0, s>>>(
+          mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p, Amom.rb.count, Amom.rb.start.p);
     } else {
       k_mom_faces<false><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
-      k_mom_cells<false><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p);
+      k_mom_cells<false><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p, 0,
+                                           nullptr);
     }
     AN_CHECK_LAUNCH();
     for (int c = 0; c < 3; ++c)

@@ -463,17 +463,75 @@ __global__ __launch_bounds__(kB) void k_mcomb2(int n, int m, const double* __res
   y[j] = y[j] + sy * ty;
 }
 
-// X_m = w / ||Aw||, B_m = Aw / ||Aw||; nu2 = {||Aw||^2 after the
-// orthogonalisation, ||A x||^2 before}.  A remainder below 1e-8 of ||A x||
-// (x already in the span up to rounding) is stored as zero: normalising
-// rounding noise would break B = A X.
+// xout = xin + sx * X c and y += sy * B c (k_mcomb2's sums), with the block
+// partials of y.y of the result (grid-stride, at most 1024 blocks)
+__global__ __launch_bounds__(kB) void k_mcomb2_dot(int n, int m, const double* __restrict__ X,
+                                                   const double* __restrict__ B,
+                                                   const double* __restrict__ c, double sx,
+                                                   const double* xin, double* xout, double sy,
+                                                   double* __restrict__ y,
+                                                   double* __restrict__ partial) {
+  __shared__ double sh[kB / 64];
+  double v = 0.0;
+  for (int j = blockIdx.x * kB + threadIdx.x; j < n; j += gridDim.x * kB) {
+    double tx = 0.0, ty = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double ci = c[i];
+      tx += ci * X[(size_t)i * n + j];
+      ty += ci * B[(size_t)i * n + j];
+    }
+    xout[j] = xin[j] + sx * tx;
+    const double yj = y[j] + sy * ty;
+    y[j] = yj;
+    v += yj * yj;
+  }
+  double d[1] = {v};
+  block_sums<1>(d, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = d[0];
+}
+
+// A solve's first scalars, finished and published in one kernel: NC 2: the
+// residual SpMV's partials {r.r, b.b}; NC 1: r.r after the projection, with
+// b.b from scal[bbSlot].  scal[S_RR] = r.r, scal[S_BB] = b.b, and both go
+// to the host through the published word's values.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_finalize_start(int nparts,
+                                                         const double* __restrict__ partial,
+                                                         double* __restrict__ scal, int bbSlot,
+                                                         double* vd, unsigned long long* word,
+                                                         unsigned stamp) {
+  __shared__ double sh[NC * 16];
+  double v[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[c] = 0.0;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] += partial[(size_t)i * NC + c];
+  block_sums<NC>(v, sh);
+  if (threadIdx.x == 0) {
+    const double rr = v[0], bb = NC == 2 ? v[NC - 1] : scal[bbSlot];
+    scal[S_RR] = rr;
+    scal[S_BB] = bb;
+    vd[0] = rr;
+    vd[1] = bb;
+    __hip_atomic_store(word, pcg_word(stamp, PCG_CONTINUE, 0.0f), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// X_m = w / ||Aw||, B_m = Aw / ||Aw|| with nuA = ||Aw||^2 after the
+// orthogonalisation and nuB = ||A x||^2 before.  A remainder below 1e-8 of
+// ||A x|| (x already in the span up to rounding) is stored as zero:
+// normalising rounding noise would break B = A X.
 __global__ __launch_bounds__(kB) void k_pscale(int n, const double* __restrict__ w,
                                                const double* __restrict__ Aw,
-                                               const double* __restrict__ nu2,
+                                               const double* __restrict__ nuA,
+                                               const double* __restrict__ nuB,
                                                double* __restrict__ X, double* __restrict__ B) {
   const int j = blockIdx.x * kB + threadIdx.x;
   if (j >= n) return;
-  const double s = nu2[0] > 1e-16 * nu2[1] && nu2[0] > 0.0 ? 1.0 / sqrt(nu2[0]) : 0.0;
+  const double s = nuA[0] > 1e-16 * nuB[0] && nuA[0] > 0.0 ? 1.0 / sqrt(nuA[0]) : 0.0;
   X[j] = w[j] * s;
   B[j] = Aw[j] * s;
 }
@@ -491,20 +549,35 @@ void PcgSolver::proj_coef(const double* rr) {
   finalize(red, nb, projM, scal.p + S_PROJ, s);
 }
 
-// after r = b - A x: x += X c, r -= B c, scal[S_RR] = r.r
-void PcgSolver::project_initial(double* x) {
-  if (opt.rhsProjection <= 0 || projM == 0) return;
+// The solve's start: r = b - A x with r.r and b.b in the same pass; with
+// stored directions x += X c, r -= B c (c = B^T r) with r.r of the result
+// in that pass; one kernel finishes scal[S_RR], scal[S_BB] and publishes
+// them.  Returns {r.r, b.b} as the host reads them.
+std::pair<double, double> PcgSolver::start_residual(const double* b, double* x) {
   const int n = A.n;
-  const int G = grid_for(n, kB);
-  proj_coef(r.p);
-  k_mcomb2<<<G, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, 1.0, x, -1.0, r.p);
+  const int nbR = residual_bb<double, double>(A, b, x, r.p, red, s);
+  const unsigned stamp = ++pubStamp;
+  if (opt.rhsProjection > 0 && projM > 0) {
+    finalize(red, nbR, 2, scal.p + S_RB, s);  // {r.r, b.b} of the unprojected start
+    proj_coef(r.p);
+    const int nb = std::min(grid_for(n, kB), 1024);
+    k_mcomb2_dot<<<nb, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, 1.0, x, x, -1.0,
+                                   r.p, red.partial.p);
+    AN_CHECK_LAUNCH();
+    k_finalize_start<1><<<1, 1024, 0, s>>>(nb, red.partial.p, scal.p, S_RB + 1, pub.vd, pub.d,
+                                           stamp);
+  } else {
+    k_finalize_start<2><<<1, 1024, 0, s>>>(nbR, red.partial.p, scal.p, 0, pub.vd, pub.d, stamp);
+  }
   AN_CHECK_LAUNCH();
-  dot<double>(n, r.p, r.p, scal.p, S_RR, red, s);
+  (void)pub.wait(stamp, s, "solver start scalars were not published");
+  return {pub.vh[0], pub.vh[1]};
 }
 
-// adds the solution x: w = x, Aw = A x, one classical Gram-Schmidt pass of
-// Aw against B (w follows with X, so B = A X holds whatever the remaining
-// loss of orthogonality), normalised by ||Aw||
+// adds the solution x: Aw = A x (with ||A x||^2), w = x - X c and
+// Aw -= B c with c = B^T A x (one classical Gram-Schmidt pass; w follows
+// with X, so B = A X holds whatever the remaining loss of orthogonality,
+// with ||Aw||^2 in the same pass), normalised by ||Aw||
 void PcgSolver::project_add(const double* x) {
   const int maxV = std::min(opt.rhsProjection, kProjMax);
   if (maxV <= 0 || projM >= maxV) return;
@@ -514,16 +587,21 @@ void PcgSolver::project_add(const double* x) {
     projX.alloc((size_t)maxV * n);
     projB.alloc((size_t)maxV * n);
   }
-  spmv<double, double>(A, x, q.p, nullptr, scal.p, 0, red, s);  // q = A x
-  dot<double>(n, q.p, q.p, scal.p, S_NU + 1, red, s);
-  AN_HIP(hipMemcpyAsync(p.p, x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+  const int nbQ = spmv_self_dot<double, double>(A, x, q.p, red, s);  // q = A x
+  finalize(red, nbQ, 1, scal.p + S_NU + 1, s);
+  const double* w = x;
+  const double* nuA = scal.p + S_NU + 1;
   if (projM > 0) {
     proj_coef(q.p);
-    k_mcomb2<<<G, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, -1.0, p.p, -1.0, q.p);
+    const int nb = std::min(G, 1024);
+    k_mcomb2_dot<<<nb, kB, 0, s>>>(n, projM, projX.p, projB.p, scal.p + S_PROJ, -1.0, x, p.p,
+                                   -1.0, q.p, red.partial.p);
     AN_CHECK_LAUNCH();
+    finalize(red, nb, 1, scal.p + S_NU, s);
+    w = p.p;
+    nuA = scal.p + S_NU;
   }
-  dot<double>(n, q.p, q.p, scal.p, S_NU, red, s);
-  k_pscale<<<G, kB, 0, s>>>(n, p.p, q.p, scal.p + S_NU, projX.p + (size_t)projM * n,
+  k_pscale<<<G, kB, 0, s>>>(n, w, q.p, nuA, scal.p + S_NU + 1, projX.p + (size_t)projM * n,
                             projB.p + (size_t)projM * n);
   AN_CHECK_LAUNCH();
   ++projM;
@@ -536,12 +614,9 @@ an_status PcgSolver::solve(const double* b, double* x, an_solve_info* info) {
   auto t0 = std::chrono::steady_clock::now();
   // ||b|| and the initial residual in one round trip (the residual is
   // unused when b == 0)
-  dot<double>(n, b, b, scal.p, S_BB, red, s);
-  residual<double, double>(A, b, x, r.p, scal.p, red, s);
-  project_initial(x);
-  const double* sv = fetch_scalars(S_RR, S_BB - S_RR + 1);  // S_RR .. S_BB
-  const double bnorm = std::sqrt(sv[S_BB - S_RR]);
-  const double rn0 = std::sqrt(sv[0]);
+  const std::pair<double, double> rb = start_residual(b, x);
+  const double bnorm = std::sqrt(rb.second);
+  const double rn0 = std::sqrt(rb.first);
   history.clear();
   history.push_back(rn0);
   an_solve_info inf{};

@@ -3,6 +3,7 @@
 
 #include <map>
 #include <memory>
+#include <utility>
 
 #include "../../include/anabasis_hip.h"
 #include "amg.hpp"
@@ -123,7 +124,7 @@ class PcgSolver {
   DevBuf<double> projX, projB;
   int projM = 0;
   void proj_coef(const double* rr);
-  void project_initial(double* x);
+  std::pair<double, double> start_residual(const double* b, double* x);
   void project_add(const double* x);
   int pcg64(const double* b, double* x, double tol, int maxIter, double* rnormOut);
   // rn0: norm of b (the outer residual) for the speculation rule
