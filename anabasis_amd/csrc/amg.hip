@@ -770,48 +770,52 @@ __global__ void k_dense_scatter(int n, const int* __restrict__ ro, const int* __
   dabs[i] = fabs(row[i]);
 }
 
-// Left-looking Cholesky in one workgroup, L in the lower triangle of A:
-// column k needs row k's finished prefix (staged in LDS) and every row
-// below it; a pivot d <= 1e-13 max|diag| zeroes the column instead.  Each
-// sum runs over j in increasing order, as the oracle's loops do.
-static constexpr int kCholThreads = 1024;
-static constexpr int kDenseChunk = 16;  // loads in flight per thread
-__global__ __launch_bounds__(kCholThreads) void k_dense_cholesky(int n, double* __restrict__ A,
-                                                                 const double* __restrict__ dabs) {
-  __shared__ double rowk[4096];
-  __shared__ double red[kCholThreads / 64];
-  __shared__ double lk;
-  __shared__ int skip;
-  const int t = threadIdx.x, nt = blockDim.x;
-  // max |diag|: order-free
+// Dense inverse of the coarsest level in three stages, every entry formed
+// by the oracle's sequence of operations (dense_inverse, oracle/solvers.c):
+//   1. Cholesky A = L L^T, L in the lower triangle of A;
+//   2. M = L^-1 by forward substitution (a zero pivot gives a zero row);
+//   3. inverse = M^T M, entry (i, j) summed over k >= max(i, j) in
+//      increasing k (symmetric bit for bit, so row- and column-major agree).
+// Stages 1 and 2 run right-looking in panels of kPanel pivots: a one-
+// workgroup kernel walks the panel's pivots, a tiled kernel applies the
+// panel to the trailing part.  Right-looking applies to each entry the
+// updates j = 0, 1, ... in increasing order, exactly the left-looking sums
+// of the oracle, so L and M carry its bits.  A pivot d <= 1e-13 max|diag|
+// zeroes its column of L.
+static constexpr int kPanel = 32;
+static constexpr int kPanelThreads = 1024;
+static constexpr int kTile = 16;
+
+// thr[0] = 1e-13 max_i |a_ii| (a maximum: order-free)
+__global__ __launch_bounds__(kPanelThreads) void k_dense_thr(int n,
+                                                             const double* __restrict__ dabs,
+                                                             double* __restrict__ thr) {
+  __shared__ double red[kPanelThreads / 64];
+  const int t = threadIdx.x;
   double m = 0.0;
-  for (int i = t; i < n; i += nt) m = fmax(m, dabs[i]);
+  for (int i = t; i < n; i += blockDim.x) m = fmax(m, dabs[i]);
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
   if ((t & 63) == 0) red[t >> 6] = m;
   __syncthreads();
   if (t == 0) {
     double dm = 0.0;
-    for (int w = 0; w < nt / 64; ++w) dm = fmax(dm, red[w]);
-    red[0] = dm;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) dm = fmax(dm, red[w]);
+    thr[0] = 1e-13 * dm;
   }
-  __syncthreads();
-  const double thr = 1e-13 * red[0];
-  for (int k = 0; k < n; ++k) {
-    const double* Ak = A + (size_t)k * n;
-    for (int j = t; j < k; j += nt) rowk[j] = Ak[j];
-    __syncthreads();
+}
+
+// pivots k = p0 .. p1-1: the pivot, column k of L (rows below), then the
+// update of the panel's later columns j (rows i >= j) by column k
+__global__ __launch_bounds__(kPanelThreads) void k_chol_panel(int n, int p0, int p1,
+                                                              double* __restrict__ A,
+                                                              const double* __restrict__ thr) {
+  __shared__ double lk;
+  __shared__ int skip;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int k = p0; k < p1; ++k) {
     if (t == 0) {
-      double d = Ak[k];
-      int j = 0;
-      for (; j + kDenseChunk <= k; j += kDenseChunk) {
-        double a[kDenseChunk];
-#pragma unroll
-        for (int u = 0; u < kDenseChunk; ++u) a[u] = rowk[j + u];
-#pragma unroll
-        for (int u = 0; u < kDenseChunk; ++u) d -= a[u] * a[u];
-      }
-      for (; j < k; ++j) d -= rowk[j] * rowk[j];
-      skip = d <= thr;
+      const double d = A[(size_t)k * n + k];
+      skip = d <= thr[0];
       if (!skip) {
         const double l = sqrt(d);
         A[(size_t)k * n + k] = l;
@@ -823,92 +827,161 @@ __global__ __launch_bounds__(kCholThreads) void k_dense_cholesky(int n, double* 
       for (int i = k + t; i < n; i += nt) A[(size_t)i * n + k] = 0.0;
     } else {
       const double l = lk;
-      for (int i = k + 1 + t; i < n; i += nt) {
-        double* Ai = A + (size_t)i * n;
-        double s = Ai[k];
-        // s -= A[i][j] * L[k][j] for j = 0 .. k-1 in order; each chunk's
-        // loads are issued before its (ordered) updates
-        int j = 0;
-        for (; j + kDenseChunk <= k; j += kDenseChunk) {
-          double a[kDenseChunk];
-#pragma unroll
-          for (int u = 0; u < kDenseChunk; ++u) a[u] = Ai[j + u];
-#pragma unroll
-          for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * rowk[j + u];
-        }
-        for (; j < k; ++j) s -= Ai[j] * rowk[j];
-        Ai[k] = s / l;
+      for (int i = k + 1 + t; i < n; i += nt) A[(size_t)i * n + k] = A[(size_t)i * n + k] / l;
+    }
+    __syncthreads();
+    const int nj = p1 - k - 1;
+    if (nj > 0) {
+      const long long ne = (long long)(n - k - 1) * nj;
+      for (long long e = t; e < ne; e += nt) {
+        const int i = k + 1 + (int)(e / nj), j = k + 1 + (int)(e % nj);
+        if (j <= i) A[(size_t)i * n + j] -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
       }
     }
     __syncthreads();
   }
 }
 
-// Column col of the inverse: forward substitution L y = e_col, then
-// backward L^T x = y, both in the oracle's order; one thread per column,
-// W[i*n + col] holds y and is overwritten by x (coalesced across a wave).
-__global__ void k_dense_columns(int n, const double* __restrict__ L, double* __restrict__ W) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= n) return;
-  double* Wc = W + col;  // Wc[j * n] = entry j of this column
-  for (int i = 0; i < n; ++i) {
-    const double* Li = L + (size_t)i * n;
-    double s = (i == col) ? 1.0 : 0.0;
-    int j = 0;
-    for (; j + kDenseChunk <= i; j += kDenseChunk) {
-      double a[kDenseChunk], w[kDenseChunk];
-#pragma unroll
-      for (int u = 0; u < kDenseChunk; ++u) {
-        a[u] = Li[j + u];
-        w[u] = Wc[(size_t)(j + u) * n];
-      }
-#pragma unroll
-      for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * w[u];
-    }
-    for (; j < i; ++j) s -= Li[j] * Wc[(size_t)j * n];
-    const double l = Li[i];
-    Wc[(size_t)i * n] = (l != 0.0) ? s / l : 0.0;
+// trailing part after the panel [p0, p1): a_ij -= l_ik l_jk, k = p0 .. p1-1
+// in order, for j >= p1 and i >= j; one 16 x 16 tile of (i, j) per block
+__global__ __launch_bounds__(kTile* kTile) void k_chol_trail(int n, int p0, int p1,
+                                                             double* __restrict__ A) {
+  __shared__ double Li[kTile][kPanel + 1], Lj[kTile][kPanel + 1];
+  const int T = (n - p1 + kTile - 1) / kTile;
+  const int ti = blockIdx.x / T, tj = blockIdx.x % T;
+  if (ti < tj) return;  // whole block: the lower triangle of tiles only
+  const int i0 = p1 + ti * kTile, j0 = p1 + tj * kTile, w = p1 - p0;
+  for (int e = threadIdx.x; e < kTile * w; e += blockDim.x) {
+    const int r = e / w, k = e % w;
+    Li[r][k] = i0 + r < n ? A[(size_t)(i0 + r) * n + p0 + k] : 0.0;
+    Lj[r][k] = j0 + r < n ? A[(size_t)(j0 + r) * n + p0 + k] : 0.0;
   }
-  for (int i = n - 1; i >= 0; --i) {
-    double s = Wc[(size_t)i * n];
-    int j = i + 1;
-    for (; j + kDenseChunk <= n; j += kDenseChunk) {
-      double a[kDenseChunk], w[kDenseChunk];
-#pragma unroll
-      for (int u = 0; u < kDenseChunk; ++u) {
-        a[u] = L[(size_t)(j + u) * n + i];
-        w[u] = Wc[(size_t)(j + u) * n];
-      }
-#pragma unroll
-      for (int u = 0; u < kDenseChunk; ++u) s -= a[u] * w[u];
+  __syncthreads();
+  const int ri = threadIdx.x / kTile, rj = threadIdx.x % kTile;
+  const int i = i0 + ri, j = j0 + rj;
+  if (i >= n || j >= n || j > i) return;
+  double a = A[(size_t)i * n + j];
+  for (int k = 0; k < w; ++k) a -= Li[ri][k] * Lj[rj][k];
+  A[(size_t)i * n + j] = a;
+}
+
+// forward substitution, pivots k = p0 .. p1-1: row k of M (columns c <= k),
+// then the panel's later rows i updated by it
+__global__ __launch_bounds__(kPanelThreads) void k_fwd_panel(int n, int p0, int p1,
+                                                             const double* __restrict__ L,
+                                                             double* __restrict__ S) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int k = p0; k < p1; ++k) {
+    const double l = L[(size_t)k * n + k];
+    for (int c = t; c <= k; c += nt) {
+      const double v = S[(size_t)k * n + c];
+      S[(size_t)k * n + c] = (l != 0.0) ? v / l : 0.0;
     }
-    for (; j < n; ++j) s -= L[(size_t)j * n + i] * Wc[(size_t)j * n];
-    const double l = L[(size_t)i * n + i];
-    Wc[(size_t)i * n] = (l != 0.0) ? s / l : 0.0;
+    __syncthreads();
+    const int ni = p1 - k - 1;
+    if (ni > 0) {
+      const long long ne = (long long)ni * (k + 1);
+      for (long long e = t; e < ne; e += nt) {
+        const int i = k + 1 + (int)(e / (k + 1)), c = (int)(e % (k + 1));
+        S[(size_t)i * n + c] -= L[(size_t)i * n + k] * S[(size_t)k * n + c];
+      }
+    }
+    __syncthreads();
   }
 }
 
-// invT[j*n + i] = W[i*n + j]
-__global__ void k_dense_transpose(int n, const double* __restrict__ W, double* __restrict__ invT) {
+// trailing rows after the panel: s_ic -= l_ik m_kc, k = max(p0, c) .. p1-1 in
+// order (the terms k < c are zero products, which the oracle also adds to
+// a zero), for i >= p1 and c < p1; one 16 x 16 tile of (i, c) per block
+__global__ __launch_bounds__(kTile* kTile) void k_fwd_trail(int n, int p0, int p1,
+                                                            const double* __restrict__ L,
+                                                            double* __restrict__ S) {
+  __shared__ double Lt[kTile][kPanel + 1], Mt[kPanel][kTile + 1];
+  const int TC = (p1 + kTile - 1) / kTile;
+  const int ti = blockIdx.x / TC, tc = blockIdx.x % TC;
+  const int i0 = p1 + ti * kTile, c0 = tc * kTile, w = p1 - p0;
+  for (int e = threadIdx.x; e < kTile * w; e += blockDim.x) {
+    const int r = e / w, k = e % w;
+    Lt[r][k] = i0 + r < n ? L[(size_t)(i0 + r) * n + p0 + k] : 0.0;
+    const int kk = e / kTile, cc = e % kTile;
+    Mt[kk][cc] = c0 + cc < p1 ? S[(size_t)(p0 + kk) * n + c0 + cc] : 0.0;
+  }
+  __syncthreads();
+  const int ri = threadIdx.x / kTile, rc = threadIdx.x % kTile;
+  const int i = i0 + ri, c = c0 + rc;
+  if (i >= n || c >= p1) return;
+  double v = S[(size_t)i * n + c];
+  for (int k = max(c - p0, 0); k < w; ++k) v -= Lt[ri][k] * Mt[k][rc];
+  S[(size_t)i * n + c] = v;
+}
+
+// inverse = M^T M: entries (i, j) and (j, i) of a 16 x 16 tile pair, the sum
+// over k in increasing order from max(i0, j0) (terms below max(i, j) are
+// zero products added to zero), M staged 32 rows at a time
+__global__ __launch_bounds__(kTile* kTile) void k_mtm(int n, const double* __restrict__ M,
+                                                      double* __restrict__ inv) {
+  __shared__ double Mi[kPanel][kTile + 1], Mj[kPanel][kTile + 1];
+  const int T = (n + kTile - 1) / kTile;
+  const int ti = blockIdx.x / T, tj = blockIdx.x % T;
+  if (ti < tj) return;  // whole block: tile pairs below the diagonal
+  const int i0 = ti * kTile, j0 = tj * kTile;
+  const int ri = threadIdx.x / kTile, rj = threadIdx.x % kTile;
+  const int i = i0 + ri, j = j0 + rj;
+  double s = 0.0;
+  for (int kb = i0; kb < n; kb += kPanel) {  // i0 = max(i0, j0)
+    for (int e = threadIdx.x; e < kPanel * kTile; e += blockDim.x) {
+      const int kk = e / kTile, cc = e % kTile;
+      const bool in = kb + kk < n;
+      Mi[kk][cc] = in && i0 + cc < n ? M[(size_t)(kb + kk) * n + i0 + cc] : 0.0;
+      Mj[kk][cc] = in && j0 + cc < n ? M[(size_t)(kb + kk) * n + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+    const int kn = min(kPanel, n - kb);
+    for (int k = 0; k < kn; ++k) s += Mi[k][ri] * Mj[k][rj];
+    __syncthreads();
+  }
+  if (i < n && j < n) {
+    inv[(size_t)i * n + j] = s;
+    inv[(size_t)j * n + i] = s;
+  }
+}
+
+__global__ void k_identity(int n, double* __restrict__ S) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)n * n) return;
-  const size_t j = e / (size_t)n, i = e % (size_t)n;
-  invT[e] = W[i * (size_t)n + j];
+  S[e] = (e / (size_t)n == e % (size_t)n) ? 1.0 : 0.0;
 }
 
 void dense_inverse_device(const DevCSR<double>& A, DevBuf<double>& invT, hipStream_t s) {
   const int n = A.n;
   if (n <= 0 || n > 4096) throw Error(7, "dense coarsest level must have 1..4096 rows");
   const size_t nn = (size_t)n * n;
-  DevBuf<double> M(nn), W(nn), dabs((size_t)n);
-  k_dense_scatter<<<grid_for(n, 64), 64, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, M.p, dabs.p);
+  DevBuf<double> L(nn), S(nn), dabs((size_t)n), thr(1);
+  k_dense_scatter<<<grid_for(n, 64), 64, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, L.p, dabs.p);
+  k_dense_thr<<<1, kPanelThreads, 0, s>>>(n, dabs.p, thr.p);
   AN_CHECK_LAUNCH();
-  k_dense_cholesky<<<1, kCholThreads, 0, s>>>(n, M.p, dabs.p);
-  AN_CHECK_LAUNCH();
-  k_dense_columns<<<grid_for(n, 64), 64, 0, s>>>(n, M.p, W.p);
-  AN_CHECK_LAUNCH();
+  for (int p0 = 0; p0 < n; p0 += kPanel) {
+    const int p1 = std::min(p0 + kPanel, n);
+    k_chol_panel<<<1, kPanelThreads, 0, s>>>(n, p0, p1, L.p, thr.p);
+    if (p1 < n) {
+      const int T = (n - p1 + kTile - 1) / kTile;
+      k_chol_trail<<<T * T, kTile * kTile, 0, s>>>(n, p0, p1, L.p);
+    }
+    AN_CHECK_LAUNCH();
+  }
+  k_identity<<<grid_for((long long)nn, 256), 256, 0, s>>>(n, S.p);
+  for (int p0 = 0; p0 < n; p0 += kPanel) {
+    const int p1 = std::min(p0 + kPanel, n);
+    k_fwd_panel<<<1, kPanelThreads, 0, s>>>(n, p0, p1, L.p, S.p);
+    if (p1 < n) {
+      const int TI = (n - p1 + kTile - 1) / kTile, TC = (p1 + kTile - 1) / kTile;
+      k_fwd_trail<<<TI * TC, kTile * kTile, 0, s>>>(n, p0, p1, L.p, S.p);
+    }
+    AN_CHECK_LAUNCH();
+  }
   invT.alloc(nn);
-  k_dense_transpose<<<grid_for((long long)nn, 256), 256, 0, s>>>(n, W.p, invT.p);
+  const int T = (n + kTile - 1) / kTile;
+  k_mtm<<<T * T, kTile * kTile, 0, s>>>(n, S.p, invT.p);
   AN_CHECK_LAUNCH();
 }
 

@@ -323,7 +323,11 @@ static double *dense_inverse(int n, const int *ro, const int *ci,
       A[(size_t)i * n + k] = s / l;
     }
   }
-  double *inv = (double *)calloc((size_t)n * n, sizeof(double));
+  /* M = L^-1 by forward substitution, column by column (a zero pivot gives
+     a zero row of M), then inverse = M^T M: entry (i, j) is the sum over
+     k >= max(i, j) of M[k][i] M[k][j] in increasing k (symmetric bit for
+     bit) */
+  double *M = (double *)calloc((size_t)n * n, sizeof(double));
   double *y = (double *)malloc(sizeof(double) * (size_t)n);
   for (int col = 0; col < n; ++col) {
     for (int i = 0; i < n; ++i) {
@@ -332,15 +336,20 @@ static double *dense_inverse(int n, const int *ro, const int *ci,
       double l = A[(size_t)i * n + i];
       y[i] = (l != 0.0) ? s / l : 0.0;
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = y[i];
-      for (int j = i + 1; j < n; ++j) s -= A[(size_t)j * n + i] * inv[(size_t)j * n + col];
-      double l = A[(size_t)i * n + i];
-      inv[(size_t)i * n + col] = (l != 0.0) ? s / l : 0.0;
-    }
+    for (int i = 0; i < n; ++i) M[(size_t)i * n + col] = y[i];
   }
   free(y);
   free(A);
+  double *inv = (double *)calloc((size_t)n * n, sizeof(double));
+  OR_PFOR
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+      for (int k = i; k < n; ++k) s += M[(size_t)k * n + i] * M[(size_t)k * n + j];
+      inv[(size_t)i * n + j] = s;
+      inv[(size_t)j * n + i] = s;
+    }
+  free(M);
   return inv;
 }
 

@@ -348,6 +348,29 @@ def test_vcycle_is_exact_when_coarsest_is_the_matrix(dims):
     assert np.linalg.norm(z - xd) / np.linalg.norm(xd) < 1e-11
 
 
+def test_dense_coarsest_solves_a_singular_neumann_system():
+    """A pure-Neumann Laplacian as the (only, dense) level: the zero pivot is
+    skipped and M^T M (M = L^-1 with that row zero) still solves every
+    consistent right-hand side, and is symmetric."""
+    om, _ = spd_system(7, 6, 5)
+    nb = om.nFaces - om.nInternalFaces
+    t = np.zeros(nb, np.int32)
+    vals, _ = O.elliptic_assemble(om, np.ones(om.nFaces), np.zeros(om.nCells), t, np.zeros(nb),
+                                  np.zeros((om.nCells, 3)), nonorth=False)
+    A = csr(om, vals)
+    assert abs(A @ np.ones(om.nCells)).max() < 1e-12  # singular: constants in the kernel
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal(om.nCells)
+    r -= r.mean()
+    z, nl = O.amg_vcycle(om.rowOffsets, om.cols, vals, r, coarseSize=4096)
+    assert nl == 1
+    assert np.linalg.norm(A @ z - r) <= 1e-11 * np.linalg.norm(r)
+    s = rng.standard_normal(om.nCells)
+    s -= s.mean()
+    zs, _ = O.amg_vcycle(om.rowOffsets, om.cols, vals, s, coarseSize=4096)
+    assert abs(s @ z - r @ zs) <= 1e-12 * abs(s @ z)
+
+
 def test_vcycle_is_linear_and_symmetric():
     om, vals = spd_system(12, 11, 10)
     rng = np.random.default_rng(1)
