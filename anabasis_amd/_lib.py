@@ -80,7 +80,7 @@ class SimpleParams(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("rowsPerWave", C.c_int), ("rowSeconds", C.c_double * 3), ("launch", C.c_int),
+    _fields_ = [("rowsPerWave", C.c_int), ("rowSeconds", C.c_double * 5), ("launch", C.c_int),
                 ("persistentBlocksPerCU", C.c_int), ("launchSeconds", C.c_double * 4),
                 ("launchesLastSolve", C.c_int)]
 

@@ -15,8 +15,10 @@
 //   V-cycle) is captured once in a HIP graph of several iterations and
 //   replayed until no problem is active; the host reads one counter per
 //   replay.
-// * Autotuner: the block height (64, 32 or 16 rows per wave) is chosen by
-//   timing the batched SpMV on the actual matrix when values are set.
+// * Autotuner: the block height (64, 32, 16 or 4 rows per wave, or one wave
+//   per row) is chosen by timing the batched SpMV on the actual matrix when
+//   values are set, and the launch (one persistent kernel, or a replayed
+//   graph of four iterations) by timing one PCG iteration of each.
 // * Precision: FP64 PCG, or FP64 iterative refinement around an FP32 PCG
 //   (FP32 matrix, vectors and V-cycle; FP64 residuals and reductions).
 #include "batch.hpp"
@@ -765,7 +767,9 @@ void BatchSolver::set_values(const std::vector<const double*>& values, bool dev)
   }
   // autotuner: block height for the batched kernels
   fill_zero(p.p, sizeof(double) * (size_t)n, s);
-  tuneRows = {64, 32, 16};
+  // 64, 32, 16 or 4 rows per wave, or one (a wave per row: its lanes load
+  // the row's nonzeros together, one lane adds them in order)
+  tuneRows = {64, 32, 16, 4, 1};
   tuneSeconds.assign(tuneRows.size(), 0.0);
   double best = 1e300;
   int bestRows = 64;

@@ -1111,7 +1111,7 @@ an_status an_batch_get_tuning(an_batch_t b, an_batch_tuning_info* out) {
   const BatchSolver& s = *b->b;
   *out = an_batch_tuning_info{};
   out->rowsPerWave = s.lay.rows;
-  for (size_t i = 0; i < 3 && i < s.tuneSeconds.size(); ++i) out->rowSeconds[i] = s.tuneSeconds[i];
+  for (size_t i = 0; i < 5 && i < s.tuneSeconds.size(); ++i) out->rowSeconds[i] = s.tuneSeconds[i];
   out->launch = s.chosen;
   out->persistentBlocksPerCU = s.persistBlocksPerCU;
   for (size_t i = 0; i < 4 && i < s.launchSeconds.size(); ++i)

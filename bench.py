@@ -406,14 +406,16 @@ def run_batch_mms(nprob, precision):
         f"{by_launch['persistent']['solve_s']:.3f}s in {by_launch['persistent']['launches']} "
         f"launch(es), graph {by_launch['graph']['solve_s']:.3f}s in "
         f"{by_launch['graph']['launches']}; same bits {same}) vs one-by-one {t_seq:.3f}s; its "
-        f"{min(its)}..{max(its)} converged={conv} rows/wave {rows} tuning "
-        f"{[round(t * 1e6, 1) for t in secs]}us, per-iteration graph/persistent x1,2,4 per CU "
+        f"{min(its)}..{max(its)} converged={conv} rows/wave {rows} tuning 64/32/16/4/1 "
+        f"{[round(t * 1e6, 1) for t in tune['rowSeconds']]}us, per-iteration graph/persistent "
+        f"x1,2,4 per CU "
         f"{[round(t * 1e6, 1) for t in tune['launchSeconds']]}us")
     return dict(problems=nprob, cells=cells, precision=precision, setup_s=setup,
                 batched_solve_s=t_batch, one_by_one_solve_s=t_seq,
                 iterations_min=min(its), iterations_max=max(its), all_converged=conv,
                 max_rel_diff_vs_single=worst, autotuner_rows_per_wave=rows,
-                autotuner_spmv_s={"64": secs[0], "32": secs[1], "16": secs[2]},
+                autotuner_spmv_s=dict(zip(("64", "32", "16", "4", "1 (wave per row)"),
+                                          tune["rowSeconds"])),
                 autotuner_launch=lname.get(tune["launch"]),
                 autotuner_persistent_blocks_per_cu=tune["persistentBlocksPerCU"],
                 autotuner_iteration_s={"graph": tune["launchSeconds"][0],

@@ -342,8 +342,9 @@ an_status an_batch_create(int nProblems, const int *nRows,
                           const an_pcg_options *opt, an_memspace mem,
                           an_batch_t *out);
 /* Values of every problem; (re)builds the hierarchy and runs the
- * autotuner (the block height of the batched kernels: 64, 32 or 16 rows per
- * wave, timed on this matrix). */
+ * autotuner: the block height of the batched kernels (64, 32, 16 or 4 rows
+ * per wave, or a wave per row), timed on this matrix, and the launch
+ * (an_batch_set_launch). */
 an_status an_batch_set_values(an_batch_t b, const double *const *values,
                               an_memspace mem);
 /* Solves every problem; x[p] holds x0 on entry; info[nProblems]. */
@@ -378,7 +379,7 @@ an_status an_batch_set_launch(an_batch_t b, int launch);
  * workgroups per CU (0 where that grid cannot be resident). */
 typedef struct {
   int rowsPerWave;
-  double rowSeconds[3];      /* 64, 32, 16 rows per wave */
+  double rowSeconds[5];      /* 64, 32, 16, 4 rows per wave, and one wave per row */
   int launch;                /* AN_BATCH_GRAPH or AN_BATCH_PERSISTENT */
   int persistentBlocksPerCU; /* grid of the persistent kernel */
   double launchSeconds[4];   /* graph, persistent at 1, 2, 4 per CU */

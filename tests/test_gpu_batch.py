@@ -47,7 +47,9 @@ def test_batch_matches_direct_solves(problems):
     B.set_values([v for _, v, _ in problems])
     xs, infos = B.solve([b for _, _, b in problems])
     rows, secs = B.tuning()
-    assert rows in (64, 32, 16) and np.all(secs > 0)
+    assert rows in (64, 32, 16, 4, 1) and np.all(secs > 0)
+    t = B.tuning_info()
+    assert t["rowsPerWave"] == rows and all(v > 0 for v in t["rowSeconds"])
     for (om, vals, b), x, inf in zip(problems, xs, infos):
         if not np.any(b):
             assert np.all(x == 0) and inf.iterations == 0 and inf.converged
