@@ -108,9 +108,10 @@ __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A
   store3(v, blockIdx.x, partial);
 }
 
-// MODE 0: Y = A X; MODE 1: Y = B - A X.  DOT 0: none; 1: dot(DW, Y); 2: dot(Y, Y)
-// (per column, k_spmv's row blocks and trees).  Columns outside the live
-// mask are neither written nor reduced.
+// MODE 0: Y = A X; MODE 1: Y = B - A X.  DOT 0: none; 1: dot(DW, Y);
+// 2: dot(Y, Y); 3: dot(DW, Y) and dot(Y, Y) into partial[6 blk + c] and
+// partial[6 blk + 3 + c] (per column, k_spmv's row blocks and trees).
+// Columns outside the live mask are neither written nor reduced.
 template <int MODE, int DOT>
 __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ rbs,
                                               const int* __restrict__ rowOff,
@@ -130,12 +131,19 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
   const int row = q.r0 + lane;
+  // the row's own operands are loaded with the row offsets, so their latency
+  // hides behind the nonzero stream
   int kb = 0, ke = 0;
+  double bw[3] = {0.0, 0.0, 0.0};
   if (row < q.r1) {
     kb = rowOff[row];
     ke = rowOff[row + 1];
+    if (MODE == 1 || DOT == 1 || DOT == 3)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bw[c] = (MODE == 1) ? B[3 * row + c] : DW[3 * row + c];
   }
-  double dv[3] = {0.0, 0.0, 0.0};
+  static_assert(!(MODE == 1 && (DOT == 1 || DOT == 3)), "one row operand per launch");
+  double dv[3] = {0.0, 0.0, 0.0}, dt[3] = {0.0, 0.0, 0.0};
   const bool stream = q.stream && q.k1 > q.k0;
   int cidx[kStreamPer];
   double v[kStreamPer], x3[kStreamPer][3];
@@ -167,14 +175,24 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
         for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
       else
         for (int k = kb; k < ke; ++k) s += vals[k] * X[3 * cols[k] + c];
-      const double yv = (MODE == 1) ? B[3 * row + c] - s : s;
+      const double yv = (MODE == 1) ? bw[c] - s : s;
       Y[3 * row + c] = yv;
-      if (DOT == 1) dv[c] = DW[3 * row + c] * yv;
+      if (DOT == 1 || DOT == 3) dv[c] = bw[c] * yv;
       if (DOT == 2) dv[c] = yv * yv;
+      if (DOT == 3) dt[c] = yv * yv;
     }
     wave_sync();  // the slice is reused by the next column
   }
-  if (DOT) {
+  if (DOT == 3) {
+    block_sum3(dv, sh);
+    block_sum3(dt, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        partial[6 * lb + c] = dv[c];
+        partial[6 * lb + 3 + c] = dt[c];
+      }
+  } else if (DOT) {
     block_sum3(dv, sh);
     store3(dv, lb, partial);
   }
@@ -261,59 +279,26 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
   store3(acc, blockIdx.x, partial);
 }
 
-// partial[6 blk + c] = t.s, partial[6 blk + 3 + c] = t.t   (k_dot2 per column)
-__global__ __launch_bounds__(kB) void k3_dot2(int n, const double* __restrict__ T,
-                                              const double* __restrict__ S,
-                                              const Bicg3State* __restrict__ st,
-                                              double* __restrict__ partial) {
-  __shared__ double sh[3][kW];
-  if (!st->mask) return;
-  double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
-  const int St = gridDim.x * kB;
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
-    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
-    double t[2][3], sv[2][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      t[0][c] = T[3 * i + c];
-      sv[0][c] = S[3 * i + c];
-      t[1][c] = T[3 * j + c];
-      sv[1][c] = S[3 * j + c];
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (u == 1 && j == i) break;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        a[c] += t[u][c] * sv[u][c];
-        b[c] += t[u][c] * t[u][c];
-      }
-    }
-  }
-  block_sum3(a, sh);
-  block_sum3(b, sh);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      partial[6 * blockIdx.x + c] = a[c];
-      partial[6 * blockIdx.x + 3 + c] = b[c];
-    }
-}
-
-// omega = ts/tt; x += alpha ph (+ omega sh); r = s (- omega t); rr = r.r
+// omega = ts/tt; x += alpha ph (+ omega sh); r = s (- omega t), with
+// ph = dinv o p and sh = dinv o s formed here from p and s (the products
+// k3_bicg_p and k3_bicg_s stored for the SpMV) (k_bicg_x per column).
+// Partials partial[6 blk + c] = r.r and partial[6 blk + 3 + c] = the next
+// rho = r0.r, accumulated in k3_dot's element order and tree over the same
+// grid (the bits of a separate k3_dot); frozen columns add nothing.
 __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
                                                 double* __restrict__ R,
-                                                const double* __restrict__ PH,
-                                                const double* __restrict__ SH,
+                                                const double* __restrict__ P,
                                                 const double* __restrict__ S,
                                                 const double* __restrict__ T,
+                                                const double* __restrict__ dinv,
+                                                const double* __restrict__ R0,
                                                 double* __restrict__ sc,
                                                 const Bicg3State* __restrict__ st,
                                                 double* __restrict__ partial) {
   __shared__ double sh[3][kW];
   const int mask = st->mask;
   if (!mask) return;
-  double alpha[3], omega[3], acc[3] = {0.0, 0.0, 0.0};
+  double alpha[3], omega[3], acc[3] = {0.0, 0.0, 0.0}, rho[3] = {0.0, 0.0, 0.0};
   bool convS[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -326,18 +311,20 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
     const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
     const int el[2] = {i, j};
-    double xv[2][3], ph[2][3], shv[2][3], sv[2][3], tv[2][3];
+    double di[2], xv[2][3], pv[2][3], sv[2][3], tv[2][3], r0[2][3];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < 2; ++u) {
+      di[u] = dinv[el[u]];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int e = 3 * el[u] + c;
         xv[u][c] = X[e];
-        ph[u][c] = PH[e];
-        shv[u][c] = SH[e];
+        pv[u][c] = P[e];
         sv[u][c] = S[e];
         tv[u][c] = T[e];
+        r0[u][c] = R0[e];
       }
+    }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       if (u == 1 && j == i) break;
@@ -347,14 +334,15 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
         const int e = 3 * el[u] + c;
         double ri;
         if (convS[c]) {
-          X[e] = xv[u][c] + alpha[c] * ph[u][c];
+          X[e] = xv[u][c] + alpha[c] * (di[u] * pv[u][c]);
           ri = sv[u][c];
         } else {
-          X[e] = xv[u][c] + (alpha[c] * ph[u][c] + omega[c] * shv[u][c]);
+          X[e] = xv[u][c] + (alpha[c] * (di[u] * pv[u][c]) + omega[c] * (di[u] * sv[u][c]));
           ri = sv[u][c] - omega[c] * tv[u][c];
         }
         R[e] = ri;
         acc[c] += ri * ri;
+        rho[c] += r0[u][c] * ri;
       }
     }
   }
@@ -363,7 +351,13 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
     for (int c = 0; c < 3; ++c)
       if ((mask >> c) & 1) sc[3 * S_OM + c] = omega[c];
   block_sum3(acc, sh);
-  store3(acc, blockIdx.x, partial);
+  block_sum3(rho, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      partial[6 * blockIdx.x + c] = acc[c];
+      partial[6 * blockIdx.x + 3 + c] = rho[c];
+    }
 }
 
 __global__ void k3_dinv(int n, const int* __restrict__ ro, const int* __restrict__ ci,
@@ -433,7 +427,8 @@ __global__ void k3_zero_cols(int n, double* __restrict__ X, const Bicg3State* __
     if ((z >> c) & 1) X[3 * i + c] = 0.0;
 }
 
-// k3_fin<3> for r.r, then iteration 'it''s stop test per live column
+// k3_fin<6> for r.r and the next rho (into S_RR and rhoNext), then
+// iteration 'it''s stop test per live column
 // (converged: ||s|| or ||r|| <= tol; dropped: rho, r0.v or omega = 0 or ||r||
 // not finite) and the publication of the outcome: pcg_word(stamp,
 // CONTINUE while a column is live else CONVERGED, max ||r||/tol over the
@@ -442,9 +437,10 @@ __global__ __launch_bounds__(1024) void k3_fin_decide(int nparts,
                                                       const double* __restrict__ partial,
                                                       double* __restrict__ sc,
                                                       Bicg3State* __restrict__ st, int it,
-                                                      int rhoNew, unsigned long long* word,
+                                                      int rhoNew, int rhoNext,
+                                                      unsigned long long* word,
                                                       unsigned stamp) {
-  __shared__ double sh[3 * 16];
+  __shared__ double sh[6 * 16];
   const int mask = st->mask;
   if (!mask) {  // nothing was live: announce the stop again
     if (threadIdx.x == 0)
@@ -452,17 +448,18 @@ __global__ __launch_bounds__(1024) void k3_fin_decide(int nparts,
                          __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int i = threadIdx.x; i < nparts; i += blockDim.x)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] += partial[(size_t)i * 3 + c];
-  block_sums<3>(v, sh);
+    for (int c = 0; c < 6; ++c) v[c] += partial[(size_t)i * 6 + c];
+  block_sums<6>(v, sh);
   if (threadIdx.x != 0) return;
   int m = mask;
   float worst = 0.0f;
   for (int c = 0; c < 3; ++c) {
     sc[3 * S_RR + c] = v[c];
+    sc[3 * rhoNext + c] = v[3 + c];
     if (!((mask >> c) & 1)) continue;
     const double rn = sqrt(v[c]), tol = st->tol[c];
     st->rn[c] = rn;
@@ -539,13 +536,16 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   k3_zero_cols<<<gN, kB, 0, s>>>(n, w.x.p, st);
   AN_CHECK_LAUNCH();
   AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, s));
+  // rho of the first iteration; each later one comes out of k3_bicg_x
+  k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, st, part);
+  k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * S_RHO1, st);
+  AN_CHECK_LAUNCH();
   const unsigned base = w.stamp;
+  const double* dinv = w.dinv.p;
   auto queue_iteration = [&](int it) {
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
-    k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, st, part);
-    k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * rhoNew, st);
-    k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, sc, rhoNew, rhoOld,
+    k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, dinv, w.ph.p, sc, rhoNew, rhoOld,
                                 it == 1 ? 1 : 0, st);
     if (evStart && it == 1)
       hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
@@ -557,14 +557,15 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
       k3_spmv<0, 1><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.ph.p,
                                        nullptr, w.v.p, w.r0.p, st, part);
     k3_fin<3><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_R0V, st);
-    k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, w.dinv.p, w.s_.p, w.sh.p, sc, rhoNew, st, part);
+    k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, dinv, w.s_.p, w.sh.p, sc, rhoNew, st, part);
     k3_fin<3><<<1, 1024, 0, s>>>(G, part, sc + 3 * S_SS, st);
-    k3_spmv<0, 0><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
-                                     nullptr, w.t.p, nullptr, st, nullptr);
-    k3_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, st, part);
-    k3_fin<6><<<1, 1024, 0, s>>>(G, part, sc + 3 * S_TS, st);  // S_TS[0..2], S_TT[0..2]
-    k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, sc, st, part);
-    k3_fin_decide<<<1, 1024, 0, s>>>(G, part, sc, st, it, rhoNew, w.pub.d, base + (unsigned)it);
+    k3_spmv<0, 3><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
+                                     nullptr, w.t.p, w.s_.p, st, part);
+    k3_fin<6><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_TS, st);  // S_TS[0..2], S_TT[0..2]
+    k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.p.p, w.s_.p, w.t.p, dinv, w.r0.p, sc, st,
+                               part);
+    k3_fin_decide<<<1, 1024, 0, s>>>(G, part, sc, st, it, rhoNew, rhoOld, w.pub.d,
+                                     base + (unsigned)it);
     AN_CHECK_LAUNCH();
   };
   if (maxIter >= 1) {

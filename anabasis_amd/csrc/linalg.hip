@@ -151,7 +151,8 @@ void Reducer::init(int maxBlocks, hipStream_t s) {
 
 // ------------------------------------------------------------------ SpMV
 // MODE 0: y = A x ; MODE 1: y = b - A x.  DOT 0: none; 1: dot(dw, y);
-// 2: dot(y, y); 3 (MODE 1): dot(y, y) and dot(b, b), two partials per block.
+// 2: dot(y, y); 3 (MODE 1): dot(y, y) and dot(b, b); 4: dot(dw, y) and
+// dot(y, y) (two partials per block for 3 and 4).
 template <typename VT, typename XT, typename YT, int MODE, int DOT>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict__ rbs,
                                                  const int* __restrict__ rowOff,
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
     kb = rowOff[row];
     ke = rowOff[row + 1];
     if (MODE == 1) bi = b[row];
-    if (DOT == 1) wi = (double)dw[row];
+    if (DOT == 1 || DOT == 4) wi = (double)dw[row];
   }
   if (q.stream && q.k1 > q.k0)
     stage_products(q, lane, cols, vals, [&](int j) { return (double)x[j]; }, prod[w]);
@@ -192,11 +193,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
     else
       v = (YT)s;
     y[row] = v;
-    if (DOT == 1) dv = wi * (double)v;
-    if (DOT >= 2) dv = (double)v * (double)v;
+    if (DOT == 1 || DOT == 4) dv = wi * (double)v;
+    if (DOT == 2 || DOT == 3) dv = (double)v * (double)v;
     if (DOT == 3) db = bi * bi;
+    if (DOT == 4) db = (double)v * (double)v;
   }
-  if (DOT == 3) {
+  if (DOT >= 3) {
     double d2[2] = {dv, db};
     block_sums<2>(d2, sh);
     store_partials<2>(d2, lb, ra);
@@ -265,6 +267,17 @@ int residual_bb(const DevCSR<VT>& A, const double* b, const double* x, RT* r, Re
   k_spmv<VT, double, RT, 1, 3><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
                                                      A.cols.p, A.vals.p, x, b, r, nullptr,
                                                      red.args(nullptr));
+  AN_CHECK_LAUNCH();
+  return nb;
+}
+
+int spmv_dot_self(const DevCSR<double>& A, const double* x, double* y, const double* dw,
+                  Reducer& red, hipStream_t s) {
+  const int nb = spmv_grid(A);
+  if (2 * nb > 4 * red.cap) throw Error(7, "reducer too small");
+  k_spmv<double, double, double, 0, 4><<<nb, kBlock, 0, s>>>(
+      A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, x, nullptr, y, dw,
+      red.args(nullptr));
   AN_CHECK_LAUNCH();
   return nb;
 }

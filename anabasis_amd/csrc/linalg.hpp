@@ -212,7 +212,6 @@ void z_from_fp32(int n, const float* z32, const double* r, const double* q, doub
 template <typename T>
 void pcg_direction(int n, T* p, const T* z, double* scal, int num, int den, bool first,
                    hipStream_t s, double sign = 1.0);
-// r = b - A x ; scal[S_RR] = r.r (FP64 residual, output converted to RT)
 // r = b - A x with the block partials of r.r and b.b (two per block) left
 // in red.partial; returns the number of blocks (the caller finalizes)
 template <typename VT, typename RT>
@@ -222,6 +221,11 @@ int residual_bb(const DevCSR<VT>& A, const double* b, const double* x, RT* r, Re
 // number of blocks
 template <typename VT, typename XT>
 int spmv_self_dot(const DevCSR<VT>& A, const XT* x, XT* y, Reducer& red, hipStream_t s);
+// y = A x with the block partials of dw.y and y.y (two per block) left in
+// red.partial; returns the number of blocks
+int spmv_dot_self(const DevCSR<double>& A, const double* x, double* y, const double* dw,
+                  Reducer& red, hipStream_t s);
+// r = b - A x ; scal[S_RR] = r.r (FP64 residual, output converted to RT)
 template <typename VT, typename RT>
 void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, double* scal,
               Reducer& red, hipStream_t s);

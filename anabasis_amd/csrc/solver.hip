@@ -760,28 +760,20 @@ __global__ __launch_bounds__(kB) void k_bicg_s(int n, const double* __restrict__
   store1(b, partial);
 }
 
-__global__ __launch_bounds__(kB) void k_dot2(int n, const double* __restrict__ t,
-                                             const double* __restrict__ s_,
-                                             double* __restrict__ partial) {
-  __shared__ double shm[kB / 64];
-  double a = 0.0, b = 0.0;
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
-    a += t[i] * s_[i];
-    b += t[i] * t[i];
-  }
-  double ba = bsum(a, shm);
-  double bb = bsum(b, shm);
-  store2(ba, bb, partial);
-}
-
 // omega = ts/tt; if ||s|| <= tol: x += alpha ph, r = s; else x += alpha ph +
-// omega sh, r = s - omega t.  rr = r.r
+// omega sh, r = s - omega t.  ph = dinv o p and sh = dinv o s are formed
+// here from p and s (the products k_bicg_p and k_bicg_s stored for the
+// SpMV, so the same bits) rather than read back.  Block partials
+// of r.r and of the next iteration's rho = r0.r, two per block; rho is
+// accumulated in the element order and tree of dot() over the same grid,
+// so it has the bits of a separate dot(r0, r).
 __global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
                                                double* __restrict__ r,
-                                               const double* __restrict__ ph,
-                                               const double* __restrict__ sh,
+                                               const double* __restrict__ p,
                                                const double* __restrict__ s_,
                                                const double* __restrict__ t,
+                                               const double* __restrict__ dinv,
+                                               const double* __restrict__ r0,
                                                double* __restrict__ sc, double tol,
                                                double* __restrict__ partial) {
   __shared__ double shm[kB / 64];
@@ -789,41 +781,50 @@ __global__ __launch_bounds__(kB) void k_bicg_x(int n, double* __restrict__ x,
   const double tt = sc[S_TT];
   const double omega = (tt > 0.0) ? sc[S_TS] / tt : 0.0;
   const bool convS = sqrt(sc[S_SS]) <= tol;
-  double acc = 0.0;
+  double acc = 0.0, rho = 0.0;
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+    const double di = dinv[i];
     double ri;
     if (convS) {
-      x[i] += alpha * ph[i];
+      x[i] += alpha * (di * p[i]);
       ri = s_[i];
     } else {
-      x[i] += alpha * ph[i] + omega * sh[i];
+      x[i] += alpha * (di * p[i]) + omega * (di * s_[i]);
       ri = s_[i] - omega * t[i];
     }
     r[i] = ri;
     acc += ri * ri;
+    rho += r0[i] * ri;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_OM] = omega;
-  double b = bsum(acc, shm);
-  store1(b, partial);
+  const double b = bsum(acc, shm);
+  const double c = bsum(rho, shm);
+  store2(b, c, partial);
 }
 
-// finalize of r.r (k_finalize_nc<1>'s loads and tree, so the same bits),
-// then the iteration's stop test -- converged if ||s|| or ||r|| <= tol,
-// breakdown if rho, r0.v or omega = 0 or ||r|| is not finite -- published
-// with the FP64 ||r|| in the word's first value
+// finalize of r.r and of the next rho (k_finalize_nc<2>'s loads and tree,
+// so the bits of two separate finalizes) into scal[S_RR] and scal[rhoNext],
+// then, with a word, the iteration's stop test -- converged if ||s|| or
+// ||r|| <= tol, breakdown if rho, r0.v or omega = 0 or ||r|| is not
+// finite -- published with the FP64 ||r|| in the word's first value
 __global__ __launch_bounds__(1024) void k_bicg_fin_decide(int nparts,
                                                           const double* __restrict__ partial,
                                                           double* __restrict__ scal, int rhoNew,
-                                                          double tol, double* val,
+                                                          int rhoNext, double tol, double* val,
                                                           unsigned long long* word,
                                                           unsigned stamp) {
-  __shared__ double sh[16];
-  double v[1] = {0.0};
+  __shared__ double sh[2 * 16];
+  double v[2] = {0.0, 0.0};
 #pragma unroll 4
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) v[0] += partial[i];
-  block_sums<1>(v, sh);
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    v[0] += partial[2 * (size_t)i];
+    v[1] += partial[2 * (size_t)i + 1];
+  }
+  block_sums<2>(v, sh);
   if (threadIdx.x != 0) return;
   scal[S_RR] = v[0];
+  scal[rhoNext] = v[1];
+  if (!word) return;
   const double rn = sqrt(v[0]);
   const int code = (sqrt(scal[S_SS]) <= tol || rn <= tol) ? PCG_CONVERGED
                    : (scal[rhoNew] == 0.0 || scal[S_R0V] == 0.0 || scal[S_OM] == 0.0 ||
@@ -877,10 +878,11 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
   bool conv = rn <= tol;
   if (!conv) {
     AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    // rho of the first iteration; each later one comes out of k_bicg_x
+    dot<double>(n, w.r0.p, w.r.p, w.scal.p, S_RHO1, w.red, s);
     for (it = 1; it <= maxIter; ++it) {
       const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
       const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
-      dot<double>(n, w.r0.p, w.r.p, w.scal.p, rhoNew, w.red, s);
       k_bicg_p<<<G, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, w.dinv.p, w.ph.p, w.scal.p, rhoNew, rhoOld,
                                 it == 1 ? 1 : 0);
       AN_CHECK_LAUNCH();
@@ -889,17 +891,15 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
                                 w.red.partial.p);
       AN_CHECK_LAUNCH();
       finalize(w.red, G, 1, w.scal.p + S_SS, s);
-      spmv<double, double>(A, w.sh.p, w.t.p, nullptr, w.scal.p, 0, w.red, s);
-      k_dot2<<<G, kB, 0, s>>>(n, w.t.p, w.s_.p, w.red.partial.p);
-      AN_CHECK_LAUNCH();
-      finalize(w.red, G, 2, w.scal.p + S_TS, s);  // S_TS, S_TT
-      k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.ph.p, w.sh.p, w.s_.p, w.t.p, w.scal.p, tol,
-                                w.red.partial.p);
+      const int nb = spmv_dot_self(A, w.sh.p, w.t.p, w.s_.p, w.red, s);
+      finalize(w.red, nb, 2, w.scal.p + S_TS, s);  // S_TS, S_TT
+      k_bicg_x<<<G, kB, 0, s>>>(n, x, w.r.p, w.p.p, w.s_.p, w.t.p, w.dinv.p, w.r0.p, w.scal.p,
+                                tol, w.red.partial.p);
       AN_CHECK_LAUNCH();
       if (pw) {
         const unsigned sti = ++w.stamp;
-        k_bicg_fin_decide<<<1, 1024, 0, s>>>(G, w.red.partial.p, w.scal.p, rhoNew, tol, pw->vd,
-                                             pw->d, sti);
+        k_bicg_fin_decide<<<1, 1024, 0, s>>>(G, w.red.partial.p, w.scal.p, rhoNew, rhoOld, tol,
+                                             pw->vd, pw->d, sti);
         AN_CHECK_LAUNCH();
         const unsigned long long word = pw->wait(sti, s, "BiCGSTAB iteration was not published");
         rn = pw->vh[0];
@@ -910,7 +910,9 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
         if (PubWord::code(word) == PCG_BREAKDOWN) break;
         continue;
       }
-      finalize(w.red, G, 1, w.scal.p + S_RR, s);
+      k_bicg_fin_decide<<<1, 1024, 0, s>>>(G, w.red.partial.p, w.scal.p, rhoNew, rhoOld, tol,
+                                           nullptr, nullptr, 0);
+      AN_CHECK_LAUNCH();
       AN_HIP(hipMemcpyAsync(w.hs.h, w.scal.p, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
       AN_HIP(hipStreamSynchronize(s));
       const double* h = w.hs.h;
