@@ -371,28 +371,33 @@ void build_lsq_operator(const MeshView& m, LsqOperator& op, hipStream_t s) {
 
 // Apply: per cell, terms a chunk at a time -- sources and coefficients
 // (unit stride across the wave), then the phi gathers (a boundary term takes
-// its face value or phi_P by the face's BC type), then the products added in
-// term order.
+// its face value or phi_P by the face's BC type; the value is loaded with
+// the type), then the products added in term order.
 template <int NF>
 __global__ __launch_bounds__(256) void k_lsq_op(MeshView m, const int* __restrict__ bcType,
                                                 GradFields<NF> a) {
   const int P = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
   if (P >= m.nC) return;
   const int L = m.lsqLen[P];
-  const long long base = (long long)m.lsqOff[P >> 6] + (P & 63);
+  // the slice (P >> 6) is the wave's own (64-thread blocks of consecutive
+  // cells), so its offset is one wave-uniform load; 32-bit slot indices
+  // (build_lsq_operator caps the operator at 2^31 slots)
+  const int base = m.lsqOff[__builtin_amdgcn_readfirstlane(P >> 6)] + (P & 63);
   double phiP[NF], sx[NF], sy[NF], sz[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     phiP[f] = a.phi[f][P];
     sx[f] = sy[f] = sz[f] = 0.0;
   }
-  constexpr int U = NF == 1 ? 8 : 4;
+  // six terms per chunk for one field (a hex row in one chunk; 65 VGPRs,
+  // 7 waves per SIMD -- eight terms need 81 VGPRs and run 6% slower)
+  constexpr int U = NF == 1 ? 6 : 4;
   for (int j0 = 0; j0 < L; j0 += U) {
     int sv[U];
     double c0[U], c1[U], c2[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const long long t = base + 64LL * min(j0 + u, L - 1);
+      const int t = base + 64 * min(j0 + u, L - 1);
       sv[u] = m.lsqSrc[t];
       c0[u] = m.lsqCx[t];
       c1[u] = m.lsqCy[t];
@@ -405,10 +410,14 @@ __global__ __launch_bounds__(256) void k_lsq_op(MeshView m, const int* __restric
 #pragma unroll
         for (int f = 0; f < NF; ++f) ph[u][f] = a.phi[f][sv[u]];
       } else {
+        // the face value is loaded with the BC type, not after it
         const int b = -sv[u] - 1;
         const bool fixed = bcType[b] == 1;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) ph[u][f] = fixed ? a.bval[f][b] : phiP[f];
+        for (int f = 0; f < NF; ++f) {
+          const double bv = a.bval[f][b];
+          ph[u][f] = fixed ? bv : phiP[f];
+        }
       }
     }
 #pragma unroll

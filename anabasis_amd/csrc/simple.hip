@@ -346,9 +346,11 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
 // k_correct_flux reuses it.  k_p_rows (one wave per CSR row block) stages
 // the block's incidence entries and their coefficients in LDS with
 // coalesced loads, lets lane i sum row i in incidence order (the
-// reference's face order) into an LDS image of the block's matrix values,
-// and stores that image with unit-stride writes.  A block whose nonzeros or
-// entries exceed the LDS slices walks global memory instead (same order).
+// reference's face order) into registers, then writes the rows into an
+// LDS image of the block's matrix values and stores that image with
+// unit-stride writes.  A block whose entries exceed the LDS slice or that
+// has a row longer than the registers hold walks global memory instead
+// (same order).
 __global__ __launch_bounds__(kB) void k_p_face(MeshView m, const double* __restrict__ rAU,
                                                double* __restrict__ coef) {
   const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
@@ -365,22 +367,35 @@ __global__ __launch_bounds__(kB) void k_p_face(MeshView m, const double* __restr
 }
 
 static constexpr int kW = kB / 64;
-static constexpr int kEnt = 512;  // staged incidence entries per wave
+static constexpr int kEnt = 512;   // staged incidence entries per wave
+static constexpr int kRowReg = 8;  // row values a lane holds in registers
+static_assert(kStreamCap <= kEnt && kStreamCap <= 512,
+              "a stream block's image fits the slice and its offsets fit 9 bits");
 __global__ __launch_bounds__(kB) void k_p_rows(MeshView m, int nrb, const int2* __restrict__ rbs,
                                                const int* __restrict__ bcP,
                                                const double* __restrict__ rAU,
                                                const double* __restrict__ coef,
                                                double* __restrict__ vals) {
-  __shared__ double sv[kW][kStreamCap];  // the block's matrix values
-  __shared__ double ec[kW][kEnt];        // coefficient of each incidence entry
-  __shared__ int es[kW][kEnt];           // its cfSlot
+  // Per wave 5 KB of LDS (8 blocks, 32 waves per CU): the coefficient of
+  // each incidence entry, then -- once every lane has read its entries --
+  // the block's matrix image in the same slice; the entry's slot as an
+  // offset from the block's first nonzero with the first-write flag.
+  __shared__ double ec[kW][kEnt];
+  __shared__ unsigned short es[kW][kEnt];
   const int lb = xcd_block(gridDim.x);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
   if (q.r1 <= q.r0) return;  // wave-uniform; no block barrier below
   const int c = q.r0 + lane;
   const int E0 = m.cfOff[q.r0], E1 = m.cfOff[q.r1];
-  const bool staged = q.stream && (E1 - E0) <= kEnt;
+  int kb = 0, len = 0;
+  if (c < q.r1) {
+    kb = m.rowOff[c];
+    len = m.rowOff[c + 1] - kb;
+  }
+  // the image takes the block's nonzeros (a stream block has at most
+  // kStreamCap = kEnt) and every row fits the registers
+  const bool staged = q.stream && (E1 - E0) <= kEnt && __all(len <= kRowReg);
   if (staged && E1 > E0) {
     constexpr int P = kEnt / 64;
     int ent[P], slt[P];
@@ -398,21 +413,34 @@ __global__ __launch_bounds__(kB) void k_p_rows(MeshView m, int nrb, const int2* 
       const int k = E0 + lane + 64 * u;
       if (k < E1) {
         ec[w][k - E0] = cv[u];
-        es[w][k - E0] = slt[u];
+        es[w][k - E0] = (unsigned short)((((slt[u] >> 1) - q.k0) << 1) | (slt[u] & 1));
       }
     }
   }
   wave_sync();
+  double rv[kRowReg];  // staged: the row's values, position = CSR slot - kb
+#pragma unroll
+  for (int r = 0; r < kRowReg; ++r) rv[r] = 0.0;  // each is written before it is read
   if (c < q.r1) {
     double diag = 0.0;
     const int k1 = m.cfOff[c + 1];
     for (int k = m.cfOff[c]; k < k1; ++k) {
-      const double co = staged ? ec[w][k - E0] : coef[m.cfEnt[k] >> 1];
-      const int si = staged ? es[w][k - E0] : m.cfSlot[k];
-      diag += co;
-      double* t = staged ? &sv[w][(si >> 1) - q.k0] : &vals[si >> 1];
-      const double old = (si & 1) ? 0.0 : *t;
-      *t = old + (-co);
+      if (staged) {
+        const double co = ec[w][k - E0];
+        const int si = es[w][k - E0];
+        const int pos = (si >> 1) - (kb - q.k0);
+        diag += co;
+#pragma unroll
+        for (int r = 0; r < kRowReg; ++r)
+          if (pos == r) rv[r] = ((si & 1) ? 0.0 : rv[r]) + (-co);
+      } else {
+        const double co = coef[m.cfEnt[k] >> 1];
+        const int si = m.cfSlot[k];
+        diag += co;
+        double* t = &vals[si >> 1];
+        const double old = (si & 1) ? 0.0 : *t;
+        *t = old + (-co);
+      }
     }
     for (int j = m.cbOff[c]; j < m.cbOff[c + 1]; ++j) {
       const int b = m.cbFace[j];
@@ -422,14 +450,23 @@ __global__ __launch_bounds__(kB) void k_p_rows(MeshView m, int nrb, const int2* 
       diag += m.bA[b] * rAU[c] / dpn;
     }
     const int dp = m.diagPos[c];
-    if (staged)
-      sv[w][dp - q.k0] = diag;
-    else
+    if (staged) {
+#pragma unroll
+      for (int r = 0; r < kRowReg; ++r)
+        if (dp - kb == r) rv[r] = diag;
+    } else {
       vals[dp] = diag;
+    }
   }
+  if (!staged) return;
+  wave_sync();  // every lane has read its entries: the slice takes the image
+  double* img = ec[w];
+  if (c < q.r1)
+#pragma unroll
+    for (int r = 0; r < kRowReg; ++r)
+      if (r < len) img[kb - q.k0 + r] = rv[r];
   wave_sync();
-  if (staged)
-    for (int k = q.k0 + lane; k < q.k1; k += 64) vals[k] = sv[w][k - q.k0];
+  for (int k = q.k0 + lane; k < q.k1; k += 64) vals[k] = img[k - q.k0];
 }
 
 // --------------------------------------------------------- face kernels ---
@@ -485,14 +522,11 @@ __global__ __launch_bounds__(kB) void k_rc(MeshView m, FaceArgs a, const double*
 // CHECK false: no internal face has |n.d| <= 1e-30 (DevMesh::nTinyDpn), so
 // the test and its load are dropped (same results)
 template <bool CHECK>
-__global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
-                                                     const double* __restrict__ rAU,
-                                                     const double* __restrict__ gx,
-                                                     const double* __restrict__ gy,
-                                                     const double* __restrict__ gz,
-                                                     double* __restrict__ phiNo) {
-  const int f = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-  if (f >= m.nF) return;
+__device__ __forceinline__ double nonorth_face(const MeshView& m, const FaceArgs& a,
+                                               const double* __restrict__ rAU,
+                                               const double* __restrict__ gx,
+                                               const double* __restrict__ gy,
+                                               const double* __restrict__ gz, int f) {
   double flux = 0.0;
   if (f < m.nIF) {
     if (!CHECK || fabs(m.fDpn[f]) > 1.0e-30) {
@@ -505,7 +539,7 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
       // T = A n - (A / n.d) d, precomputed with the same expressions
       flux = rAUf * (gfx * m.fTx[f] + gfy * m.fTy[f] + gfz * m.fTz[f]);
     }
-  } else {
+  } else if (f < m.nF) {
     const int b = f - m.nIF;
     if (a.bcP[b] == 1) {
       const double dpn = m.bDpn[b];
@@ -515,7 +549,21 @@ __global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
       }
     }
   }
-  phiNo[f] = flux;
+  return flux;
+}
+
+// One face per thread: the kernel is bound by the gathers' cache-line
+// traffic, not by loads in flight (two or four faces per thread measured
+// slower).
+template <bool CHECK>
+__global__ __launch_bounds__(kB) void k_nonorth_flux(MeshView m, FaceArgs a,
+                                                     const double* __restrict__ rAU,
+                                                     const double* __restrict__ gx,
+                                                     const double* __restrict__ gy,
+                                                     const double* __restrict__ gz,
+                                                     double* __restrict__ phiNo) {
+  const int f = xcd_block(gridDim.x) * kB + threadIdx.x;
+  if (f < m.nF) phiNo[f] = nonorth_face<CHECK>(m, a, rAU, gx, gy, gz, f);
 }
 
 // MODE 0: div = continuity of phi (all faces).
@@ -532,8 +580,8 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
   // Latency, not bandwidth, bounds this walk (it runs at full occupancy), so
   // every load that does not depend on another is issued together: the
   // cell's bounds, divStar and -- speculatively, clamped in range -- its
-  // first boundary entry; then up to eight incidence entries; then all
-  // their fluxes.  A cell with at most eight internal faces and one
+  // first boundary entry; then up to six incidence entries; then all
+  // their fluxes.  A cell with at most six internal faces and one
   // boundary face waits on three memory round trips.  Sums stay in face
   // order.
   const int k0 = m.cfOff[c], k1 = m.cfOff[c + 1];
@@ -541,7 +589,7 @@ __global__ __launch_bounds__(kB) void k_div(MeshView m, const int* __restrict__ 
   const double ds = (MODE == 1) ? divStar[c] : 0.0;
   const int bFirst = m.nBF > 0 ? m.cbFace[min(j0, m.nBF - 1)] : 0;
   double d = 0.0;
-  constexpr int U = 8;
+  constexpr int U = 6;  // a hex cell's faces in one chunk (8: 5% slower)
   double bfl = 0.0;
   bool bOn = false;
   for (int kb = k0; kb < k1; kb += U) {
@@ -1138,12 +1186,11 @@ an_status SimpleSolver::iterate(int n, an_simple_stats* last, int* convergedOut)
     double tP = 0.0;
     const bool check = M.nTinyDpn > 0;
     auto nonorth_flux = [&]() {
+      const int G = grid_for(nF, kB);
       if (check)
-        k_nonorth_flux<true><<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p,
-                                                             phiNo.p);
+        k_nonorth_flux<true><<<G, kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p, phiNo.p);
       else
-        k_nonorth_flux<false><<<grid_for(nF, kB), kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p,
-                                                              g[2].p, phiNo.p);
+        k_nonorth_flux<false><<<G, kB, 0, s>>>(mv, fa, rAU.p, g[0].p, g[1].p, g[2].p, phiNo.p);
     };
     for (int pass = 1; pass <= nPass; ++pass) {
       grad(1, pCorr.p, g);
