@@ -154,6 +154,12 @@ __global__ void k_fill_members(int n, const int* __restrict__ agg, int* __restri
 // members of each aggregate into ascending order: up to 16 sorted in
 // registers (odd-even transposition network, padding INT_MAX), larger
 // aggregates by insertion in place
+// mpos[mem[k]] = k: each row's place in the member list
+__global__ void k_member_pos(int n, const int* __restrict__ mem, int* __restrict__ mpos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) mpos[mem[k]] = k;
+}
+
 __global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __restrict__ mem) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
@@ -437,6 +443,8 @@ __device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
 // DOT (KIND 1 and 3) adds
 // the partial sums of b_i x_i (DOT 1) and x_i qv_i (DOT 2) of the result and
 // finishes them in the kernel: the preconditioned CG's r.z and z.q.
+// KIND 0 with opos stores r_i at opos[i], its place in the aggregation's
+// member list, so the restriction sums contiguous runs.
 template <typename T, int KIND, int DOT = 0>
 __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restrict__ rbs,
                                                    const int* __restrict__ ro,
@@ -448,7 +456,8 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
                                                    T* __restrict__ out,
                                                    const T* __restrict__ qv, RedArgs ra,
                                                    const T* __restrict__ wv = nullptr,
-                                                   const double* __restrict__ cc = nullptr) {
+                                                   const double* __restrict__ cc = nullptr,
+                                                   const int* __restrict__ opos = nullptr) {
   __shared__ double prod[kW][kCap];
   __shared__ double sh[2 * kW];
   const int need = (nrb + kW - 1) / kW;
@@ -463,12 +472,13 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
   const WaveRows q = wave_rows(nrb, rbs, lb * kW + w);
   const int i = q.r0 + lane;
   auto vj = [&](int j) { return (double)v[j]; };
-  int kb = 0, ke = 0;
+  int kb = 0, ke = 0, oi_pos = i;
   double bi = 0.0, vi = 0.0, oi = 0.0, qi = 0.0, wi = 0.0;
   if (i < q.r1) {
     kb = ro[i];
     ke = ro[i + 1];
     bi = (double)b[i];
+    if (KIND == 0 && opos) oi_pos = opos[i];
     if (KIND >= 1) {
       vi = (double)v[i];
       oi = (double)omdinv[i];
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
       for (int k = kb; k < ke; ++k) s += (double)va[k] * vj(ci[k]);
     }
     if (KIND == 0) {
-      out[i] = (T)(bi - s);
+      out[oi_pos] = (T)(bi - s);
     } else {
       T o;
       if (KIND == 1) {
@@ -532,11 +542,13 @@ __global__ void k_scale(int n, const T* __restrict__ omdinv, const T* __restrict
   if (i < n) y[i] = (T)((double)omdinv[i] * (double)b[i]);
 }
 
-// bc_I = sum of the members' residuals (ascending member order);
-// yc_I = omdinv_c,I * bc_I for the coarse level's own down kernel.
-// Members are read in batches of 8 (all index loads, then all gathers;
-// slots past the end re-read the last member and are not added), so a
-// thread waits on two memory round trips per batch, not two per member.
+// bc_I = sum of the members' residuals in ascending member order;
+// yc_I = omdinv_c,I * bc_I for the coarse level's own down kernel.  The
+// residual is read as r[mem[k]] or, with mem null, from r stored in
+// member-list order (r[k] is member k's residual, as the down sweep writes
+// it on coarse levels).  One thread per aggregate, loads in batches of 8
+// (all index loads, then all values; slots past the end re-read the last
+// member and are not added).
 template <typename T>
 __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __restrict__ mem,
                            const T* __restrict__ r, const T* __restrict__ omdc,
@@ -552,7 +564,7 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
     int idx[U];
     double v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) idx[u] = mem[min(b + u, m1 - 1)];
+    for (int u = 0; u < U; ++u) idx[u] = mem ? mem[min(b + u, m1 - 1)] : min(b + u, m1 - 1);
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = (double)r[idx[u]];
 #pragma unroll
@@ -566,10 +578,11 @@ __global__ void k_restrict(int nc, const int* __restrict__ moff, const int* __re
 
 // The same aggregate sums over row blocks of the member lists (moff as row
 // offsets, mem as columns; one wave per block): the lanes first load all of
-// the block's members and their r values (coalesced member reads, one
-// gather round trip), stage them in the wave's LDS slice, then lane I adds
-// aggregate I's values in member order -- k_restrict's order, so the same
-// bits, without one thread walking a whole aggregate of gathers.
+// the block's members' r values (with mem: coalesced member reads, one
+// gather round trip; without: unit-stride loads), stage them in the wave's
+// LDS slice, then lane I adds aggregate I's values in member order --
+// k_restrict's order, so the same bits, without one thread walking a whole
+// aggregate.
 template <typename T>
 __global__ __launch_bounds__(kB) void k_restrict_rb(int nrb, const int2* __restrict__ rbs,
                                                     const int* __restrict__ moff,
@@ -592,7 +605,10 @@ __global__ __launch_bounds__(kB) void k_restrict_rb(int nrb, const int2* __restr
     int m[kStreamPer];
     double v[kStreamPer];
 #pragma unroll
-    for (int u = 0; u < kStreamPer; ++u) m[u] = mem[min(q.k0 + lane + 64 * u, q.k1 - 1)];
+    for (int u = 0; u < kStreamPer; ++u) {
+      const int k = min(q.k0 + lane + 64 * u, q.k1 - 1);
+      m[u] = mem ? mem[k] : k;
+    }
 #pragma unroll
     for (int u = 0; u < kStreamPer; ++u) v[u] = (double)r[m[u]];
 #pragma unroll
@@ -607,7 +623,7 @@ __global__ __launch_bounds__(kB) void k_restrict_rb(int nrb, const int2* __restr
   if (q.stream)
     for (int k = kb; k < ke; ++k) s += vals[w][k - q.k0];
   else
-    for (int k = kb; k < ke; ++k) s += (double)r[mem[k]];
+    for (int k = kb; k < ke; ++k) s += (double)r[mem ? mem[k] : k];
   const T bI = (T)s;
   bc[I] = bI;
   yc[I] = (T)((double)omdc[I] * (double)bI);
@@ -616,7 +632,8 @@ __global__ __launch_bounds__(kB) void k_restrict_rb(int nrb, const int2* __restr
 // e_i = y_i + xc[agg_i]  (e may be y: an in-place, per-element update).
 // KC: the coarse level took the K-cycle correction; xc holds its second
 // step c2 and the correction (T)((a1 - a2 g/rho1) c1 + a2 c2) is formed
-// here from the step's scalars ks.
+// here from the step's scalars ks.  (Two or four elements per thread
+// measured slower at 2.08 M rows.)
 template <typename T, bool KC>
 __global__ void k_prolong(int n, const T* y, const T* __restrict__ xc, const T* __restrict__ c1,
                           const double* __restrict__ ks, const int* __restrict__ agg, T* e) {
@@ -1110,6 +1127,9 @@ void AmgHierarchy::setup(const DevCSR<double>& fine, hipStream_t s) {
     L.nc = cur;
     L.agg = std::move(agg);
     build_members(L.n, cur, L.agg.p, L.moff, L.mem, s);
+    L.mpos.alloc((size_t)L.n);
+    k_member_pos<<<grid_for(L.n, kB), kB, 0, s>>>(L.n, L.mem.p, L.mpos.p);
+    AN_CHECK_LAUNCH();
     build_row_blocks(cur, L.moff.p, L.mrb, s);
     lv.emplace_back();
     AmgLevel& C = lv.back();
@@ -1321,14 +1341,22 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
     }
   }
   T* res = (xs == r) ? x : r;
+  // Below the fine level the residual goes to res in member-list order
+  // (res is read only by the restriction), which then sums contiguous runs.
+  // On the fine level the scattered stores cost the down sweep more than
+  // the restriction's gathers (38.9 -> 53.2 us against 12.1 -> 7.9 us at
+  // 2.08 M rows), so r stays in row order there.
+  const int* mpos = l > 0 ? L.mpos.p : nullptr;
+  const int* mem = l > 0 ? nullptr : L.mem.p;
   k_level_rows<T, 0><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      D::omd(L), b, xs, res, qv, noRed);
+                                      D::omd(L), b, xs, res, qv, noRed, (const T*)nullptr,
+                                      (const double*)nullptr, mpos);
   if (L.mrb.count > 0)
     k_restrict_rb<T><<<grid_for(L.mrb.count, kW) * spread_of(L.nc), kB, 0, s>>>(
-        L.mrb.count, L.mrb.start.p, L.moff.p, L.mem.p, res, D::omd(C), D::b(C), D::y(C));
+        L.mrb.count, L.mrb.start.p, L.moff.p, mem, res, D::omd(C), D::b(C), D::y(C));
   else
     k_restrict<T><<<grid_for(L.nc, kB) * spread_of(L.nc), kB, 0, s>>>(
-        L.nc, L.moff.p, L.mem.p, res, D::omd(C), D::b(C), D::y(C));
+        L.nc, L.moff.p, mem, res, D::omd(C), D::b(C), D::y(C));
   AN_CHECK_LAUNCH();
   // e = x_pre + P xc (with a K-cycle level's combination formed here),
   // then the post-smoothing sweeps ping-pong between r and x; e starts in

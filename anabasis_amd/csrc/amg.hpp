@@ -24,6 +24,7 @@ struct AmgLevel {
   const DevCSR<double>* A = nullptr;
   DevBuf<double> omdinv;        // omega / a_ii
   DevBuf<int> agg, moff, mem;   // aggregation to level+1
+  DevBuf<int> mpos;             // mpos[mem[k]] = k (the down sweep's store order)
   RowBlocks mrb;                // row blocks of (moff, mem) for the restriction
   // degree-2 Chebyshev smoother: chebCoef (device) = {1/theta, cv, cw, cr};
   // omdinv then holds (1/theta) / a_ii
