@@ -760,13 +760,62 @@ __global__ __launch_bounds__(kB) void k_kprod(int nrb, const int2* __restrict__ 
   store_partials<NC>(d, lb, ra);
 }
 
-// b -= a1 v1 (the second step's right-hand side, in place); y = omdinv o b
+// b -= a1 v1 (the second step's right-hand side, in place); y = omdinv o b.
+// Every block first finishes ks[0..1] = {c1.Ac1, c1.b} from k_kprod<T, 2>'s
+// nparts block partials itself, in the order of the 1024-thread
+// k_finalize_nc<2> (its 1024 threads' strided sums, each wave's shuffle
+// tree, then the 16 wave sums in order), so the scalars have the bits a
+// separate finalize would give; block 0 stores them for the prolongation.
+// A coarse level has few blocks, so the repeated sums cost less than the
+// separate launch.
+static constexpr int kFinThreads = 1024;  // the finalize order reproduced
 template <typename T>
-__global__ void k_kres(int n, const double* __restrict__ ks, const T* __restrict__ v1,
-                       const T* __restrict__ omd, T* __restrict__ b, T* __restrict__ y) {
+__global__ __launch_bounds__(kB) void k_kres(int n, int nparts, const double* __restrict__ partial,
+                                             double* __restrict__ ks, const T* __restrict__ v1,
+                                             const T* __restrict__ omd, T* __restrict__ b,
+                                             T* __restrict__ y) {
+  constexpr int J = kFinThreads / kB;  // finalize threads per thread here
+  constexpr int NW = kFinThreads / 64;
+  __shared__ double sh[2 * NW];
+  __shared__ double a1s;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc[J][2];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    acc[j][0] = acc[j][1] = 0.0;
+    for (int i = threadIdx.x + kB * j; i < nparts; i += kFinThreads) {
+      acc[j][0] += partial[2 * (size_t)i];
+      acc[j][1] += partial[2 * (size_t)i + 1];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      for (int off = 32; off > 0; off >>= 1) acc[j][c] += __shfl_down(acc[j][c], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) sh[c * NW + w + (kB / 64) * j] = acc[j][c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double k2[2];
+    for (int c = 0; c < 2; ++c) {
+      double t = 0.0;
+      for (int i = 0; i < NW; ++i) t += sh[c * NW + i];
+      k2[c] = t;
+    }
+    if (blockIdx.x == 0) {
+      ks[0] = k2[0];
+      ks[1] = k2[1];
+    }
+    a1s = kcyc_a1(k2);
+  }
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double a1 = kcyc_a1(ks);
+  const double a1 = a1s;
   const T bi = (T)((double)b[i] - a1 * (double)v1[i]);
   b[i] = bi;
   y[i] = (T)((double)omd[i] * (double)bi);
@@ -1431,8 +1480,7 @@ static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps) {
   k_kprod<T, 2><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, c1,
                                   b, none, v1, H.kred.args(ks));
   AN_CHECK_LAUNCH();
-  finalize(H.kred, nb, 2, ks, s);
-  k_kres<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, ks, v1, D::omd(L), b, y);
+  k_kres<T><<<grid_for(L.n, kB), kB, 0, s>>>(L.n, nb, H.kred.partial.p, ks, v1, D::omd(L), b, y);
   AN_CHECK_LAUNCH();
   vcycle<T>(H, c, b, y, x, s, sweeps);
   k_kprod<T, 3><<<nb, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, x, b,
