@@ -151,35 +151,49 @@ __global__ void k_fill_members(int n, const int* __restrict__ agg, int* __restri
   if (i < n) mem[atomicAdd(&cursor[agg[i]], 1)] = i;
 }
 
-// members of each aggregate into ascending order: up to 16 sorted in
-// registers (odd-even transposition network, padding INT_MAX), larger
-// aggregates by insertion in place
 // mpos[mem[k]] = k: each row's place in the member list
 __global__ void k_member_pos(int n, const int* __restrict__ mem, int* __restrict__ mpos) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) mpos[mem[k]] = k;
 }
 
+// N values in registers into ascending order (odd-even transposition network)
+template <int N>
+__device__ __forceinline__ void sort_regs(int (&v)[N]) {
+#pragma unroll
+  for (int round = 0; round < N; ++round) {
+#pragma unroll
+    for (int u = round & 1; u + 1 < N; u += 2) {
+      const int lo = min(v[u], v[u + 1]), hi = max(v[u], v[u + 1]);
+      v[u] = lo;
+      v[u + 1] = hi;
+    }
+  }
+}
+template <int N>
+__device__ __forceinline__ void sort_run(int* __restrict__ a, int len) {
+  int v[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] = u < len ? a[u] : 0x7fffffff;
+  sort_regs<N>(v);
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (u < len) a[u] = v[u];
+}
+
+// members of each aggregate into ascending order: up to 16 or 32 sorted in
+// registers (padding INT_MAX), larger aggregates by insertion in place
+// (composed pairwise aggregates hold about 16-24 members)
 __global__ void k_sort_members(int nc, const int* __restrict__ moff, int* __restrict__ mem) {
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nc) return;
   int b = moff[I], e = moff[I + 1];
   if (e - b <= 16) {
-    int v[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = (b + u < e) ? mem[b + u] : 0x7fffffff;
-#pragma unroll
-    for (int round = 0; round < 16; ++round) {
-#pragma unroll
-      for (int u = round & 1; u + 1 < 16; u += 2) {
-        const int lo = min(v[u], v[u + 1]), hi = max(v[u], v[u + 1]);
-        v[u] = lo;
-        v[u + 1] = hi;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (b + u < e) mem[b + u] = v[u];
+    sort_run<16>(mem + b, e - b);
+    return;
+  }
+  if (e - b <= 32) {
+    sort_run<32>(mem + b, e - b);
     return;
   }
   for (int a = b + 1; a < e; ++a) {
