@@ -1270,6 +1270,7 @@ void SimpleSolver::get_field(const std::string& name, double* out, bool device) 
   else if (name == "rAU") b = &rAU;
   else if (name == "div") b = &divCorr;
   else if (name == "phi") { b = &phi; n = (size_t)M.nF; }
+  else if (name == "pA") { b = &pvals; n = (size_t)Amom.nnz; }
   else throw Error(AN_EINVAL, "unknown field '" + name + "'");
   AN_HIP(hipMemcpy(out, b->p, sizeof(double) * n,
                    device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));

@@ -210,7 +210,17 @@ class Simple:
         return st, bool(conv.value)
 
     def field(self, name):
-        n = self.mesh.nFaces if name == "phi" else self.mesh.nCells
+        """A cell field, "phi" (one value per face) or "pA" (the last
+        pressure matrix's values in CSR order)."""
+        if name == "phi":
+            n = self.mesh.nFaces
+        elif name == "pA":
+            own, nei = self.mesh.owner[: self.mesh.nInternalFaces], self.mesh.neigh
+            pairs = np.unique(np.minimum(own, nei).astype(np.int64) * self.mesh.nCells
+                              + np.maximum(own, nei))
+            n = self.mesh.nCells + 2 * pairs.size
+        else:
+            n = self.mesh.nCells
         out = np.empty(n)
         check(lib().an_simple_get_field(self._h, name.encode(), ptr(out), L.AN_HOST))
         return out

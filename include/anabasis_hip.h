@@ -504,7 +504,9 @@ an_status an_simple_create(an_mesh_t mesh, const an_simple_params *par,
  * *converged when not NULL). */
 an_status an_simple_iterate(an_simple_t s, int n, an_simple_stats *last,
                             int *converged);
-/* name: "u","v","w","p","pCorr","rAU","phi"(nFaces),"div" */
+/* name: "u","v","w","p","pCorr","rAU","phi"(nFaces),"div", and "pA": the
+ * last pressure matrix's values in the CSR order of an_csr_from_faces
+ * (nCells + 2 x the number of distinct neighbour pairs) */
 an_status an_simple_get_field(an_simple_t s, const char *name, double *out,
                               an_memspace mem);
 an_status an_simple_set_field(an_simple_t s, const char *name,

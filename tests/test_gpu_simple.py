@@ -39,6 +39,9 @@ def test_simple_matches_oracle(case):
         assert rel(S.field(k), ref[k]) < 1e-8, k
     assert rel(S.field("phi"), ref["phi"]) < 1e-8
     assert rel(S.field("rAU"), ref["rAU"]) < 1e-10
+    # the last pressure matrix equals the oracle's assembly from the same
+    # rAU bit for bit (k_p_face + k_p_rows against GSV1 main.cu:1898-1950)
+    assert np.array_equal(S.field("pA"), O.pressure_assemble(om, S.field("rAU"), tP))
     rs = ref["stats"][-1]
     assert abs(st.massRes - rs.massRes) <= 1e-6 * max(rs.massRes, 1e-30) + 1e-12
     # forces on the cylinder (forceNormalSign -1, cases/cylinder.case:68-70)

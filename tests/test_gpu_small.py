@@ -177,6 +177,7 @@ def test_small_simple_matches_oracle(gpu, dims):
     for k in ("u", "v", "w", "p", "phi"):
         a, b = S.field(k), ref[k]
         assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), k
+    assert np.array_equal(S.field("pA"), O.pressure_assemble(om, S.field("rAU"), tP))
     k = m.patch_index("ymin")
     f = S.patch_forces("ymin", -1)
     fo = O.patch_forces(om, m.patchStartFace[k], m.patchStartFace[k] + m.patchNFaces[k],
