@@ -520,7 +520,7 @@ int run(int argc, char** argv) {
   parse_flags(expand_case(argc, argv), par);
   if (par.nPressureCorr > 0)
     throw Fail("nPressureCorr > 0 (extra pressure corrections after the non-orthogonal "
-               "passes) is not supported by this build; use nPressureCorr 0");
+               "passes) is not supported; use nPressureCorr 0");
 
   Mesh mesh;
   load_mesh(par.polyMeshDir, mesh);

@@ -686,8 +686,8 @@ def main():
                          "HBM traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single MI355X (how the path "
+              "would shard over several GPUs is documented, not implemented)", file=sys.stderr)
         sys.exit(2)
     from anabasis_amd import _lib as L
     L.check(L.lib().an_set_device(0))
