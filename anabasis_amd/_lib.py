@@ -55,7 +55,7 @@ class SolveInfo(C.Structure):
                 ("setupSeconds", C.c_double), ("solveSeconds", C.c_double),
                 ("refinementSteps", C.c_int), ("spmvCalls", C.c_int),
                 ("spmvAvgSeconds", C.c_double), ("vcycleCalls", C.c_int),
-                ("vcycleAvgSeconds", C.c_double)]
+                ("vcycleAvgSeconds", C.c_double), ("spmvIndexBytes", C.c_int)]
 
 
 class EllipticOptions(C.Structure):

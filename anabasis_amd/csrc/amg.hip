@@ -459,10 +459,12 @@ __device__ __forceinline__ KCoef kcyc_coef(const double* ks) {
 // finishes them in the kernel: the preconditioned CG's r.z and z.q.
 // KIND 0 with opos stores r_i at opos[i], its place in the aggregation's
 // member list, so the restriction sums contiguous runs.
-template <typename T, int KIND, int DOT = 0>
+// C16: columns from the 16-bit offsets c16 (ColOffsets) instead of ci.
+template <typename T, int KIND, int DOT, bool C16>
 __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restrict__ rbs,
                                                    const int* __restrict__ ro,
                                                    const int* __restrict__ ci,
+                                                   const int16_t* __restrict__ c16,
                                                    const T* __restrict__ va,
                                                    const T* __restrict__ omdinv,
                                                    const T* __restrict__ b,
@@ -500,7 +502,12 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
     if (KIND == 3) wi = (double)wv[i];
     if (DOT == 2) qi = (double)qv[i];
   }
-  if (q.stream && q.k1 > q.k0) stage_products(q, lane, ci, va, vj, prod[w]);
+  if (q.stream && q.k1 > q.k0) {
+    if constexpr (C16)
+      stage_products(q, lane, ColOffsets{c16, q.r0}, va, vj, prod[w]);
+    else
+      stage_products(q, lane, ci, va, vj, prod[w]);
+  }
   wave_sync();
   double d[2] = {0.0, 0.0};
   if (i < q.r1) {
@@ -508,7 +515,7 @@ __global__ __launch_bounds__(kB) void k_level_rows(int nrb, const int2* __restri
     if (q.stream) {
       for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     } else {
-      for (int k = kb; k < ke; ++k) s += (double)va[k] * vj(ci[k]);
+      for (int k = kb; k < ke; ++k) s += (double)va[k] * vj(C16 ? q.r0 + (int)c16[k] : ci[k]);
     }
     if (KIND == 0) {
       out[oi_pos] = (T)(bi - s);
@@ -1286,6 +1293,8 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
     L.A32.cols.borrow(const_cast<int*>(A.cols.p), (size_t)A.nnz);
     L.A32.rb.start.borrow(const_cast<int2*>(A.rb.start.p), (size_t)A.rb.count + 1);
     L.A32.rb.count = A.rb.count;
+    L.A32.has16 = A.has16;
+    if (A.has16) L.A32.c16.borrow(const_cast<int16_t*>(A.c16.p), (size_t)A.nnz);
     L.A32.vals.alloc((size_t)A.nnz);
     k_cast<float><<<grid_for(A.nnz, kB), kB, 0, s>>>(A.nnz, A.vals.p, L.A32.vals.p);
     L.omdinv32.alloc((size_t)L.n);
@@ -1313,6 +1322,22 @@ void AmgHierarchy::make_fp32(hipStream_t s) {
 // further Jacobi sweeps before the residual; after the coarse correction
 // every sweep is a Jacobi sweep.  dots (entry level only): the last up
 // kernel also finishes r.z (and z.q) of the result.
+// k_level_rows on a level's matrix, with 16-bit column offsets when it has
+// them
+template <typename T, int KIND, int DOT = 0>
+static void level_rows(const DevCSR<T>& A, int grid, hipStream_t s, const T* omd, const T* b,
+                       const T* v, T* out, const T* qv, RedArgs ra, const T* wv = nullptr,
+                       const double* cc = nullptr, const int* opos = nullptr) {
+  if (A.has16)
+    k_level_rows<T, KIND, DOT, true><<<grid, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
+                                                         A.cols.p, A.c16.p, A.vals.p, omd, b, v,
+                                                         out, qv, ra, wv, cc, opos);
+  else
+    k_level_rows<T, KIND, DOT, false><<<grid, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
+                                                          A.cols.p, A.c16.p, A.vals.p, omd, b, v,
+                                                          out, qv, ra, wv, cc, opos);
+}
+
 template <typename T>
 static void kcorrect(AmgHierarchy& H, int c, hipStream_t s, int sweeps);
 
@@ -1368,17 +1393,12 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   const int sp = spread_of(L.n);
   const RedArgs noRed{nullptr, nullptr};
   auto sweep = [&](const T* from, T* to, bool last) {  // to = from + omdinv (b - A from)
-    const int2* rbs = A.rb.start.p;
-    const int nrb = A.rb.count;
     if (last && dots && qv)
-      k_level_rows<T, 1, 2><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
-                                             D::omd(L), b, from, to, qv, dots->ra);
+      level_rows<T, 1, 2>(A, G * sp, s, D::omd(L), b, from, to, qv, dots->ra);
     else if (last && dots)
-      k_level_rows<T, 1, 1><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
-                                             D::omd(L), b, from, to, qv, dots->ra);
+      level_rows<T, 1, 1>(A, G * sp, s, D::omd(L), b, from, to, qv, dots->ra);
     else
-      k_level_rows<T, 1><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p, D::omd(L),
-                                          b, from, to, qv, noRed);
+      level_rows<T, 1>(A, G * sp, s, D::omd(L), b, from, to, qv, noRed);
     if (last && dots) {
       AN_CHECK_LAUNCH();
       finalize_parts(dots->ra, G, qv ? 2 : 1, s);
@@ -1391,9 +1411,8 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   const double* cc = L.cheb ? L.chebCoef.p : nullptr;
   const T* xs = y;
   if (L.cheb) {
-    k_level_rows<T, 2><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
-                                             A.vals.p, D::omd(L), b, y, r, (const T*)nullptr,
-                                             noRed, (const T*)nullptr, cc);
+    level_rows<T, 2>(A, G * sp, s, D::omd(L), b, y, r, (const T*)nullptr, noRed,
+                     (const T*)nullptr, cc);
     xs = r;
   } else {
     // pre-smoothing: x0 = y; extra sweeps ping-pong between x and r
@@ -1411,9 +1430,8 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   // 2.08 M rows), so r stays in row order there.
   const int* mpos = l > 0 ? L.mpos.p : nullptr;
   const int* mem = l > 0 ? nullptr : L.mem.p;
-  k_level_rows<T, 0><<<G * sp, kB, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p,
-                                      D::omd(L), b, xs, res, qv, noRed, (const T*)nullptr,
-                                      (const double*)nullptr, mpos);
+  level_rows<T, 0>(A, G * sp, s, D::omd(L), b, xs, res, qv, noRed, (const T*)nullptr,
+                   (const double*)nullptr, mpos);
   if (L.mrb.count > 0)
     k_restrict_rb<T><<<grid_for(L.mrb.count, kW) * spread_of(L.nc), kB, 0, s>>>(
         L.mrb.count, L.mrb.start.p, L.moff.p, mem, res, D::omd(C), D::b(C), D::y(C));
@@ -1439,17 +1457,12 @@ static void vcycle(AmgHierarchy& H, int l, const T* b, const T* y, T* x, hipStre
   if (L.cheb) {
     T* x1 = D::y(L);  // y's first-step value is used up
     sweep(e, x1, false);
-    const int2* rbs = A.rb.start.p;
-    const int nrb = A.rb.count;
     if (dots && qv)
-      k_level_rows<T, 3, 2><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
-                                                  D::omd(L), b, x1, x, qv, dots->ra, e, cc);
+      level_rows<T, 3, 2>(A, G * sp, s, D::omd(L), b, x1, x, qv, dots->ra, e, cc);
     else if (dots)
-      k_level_rows<T, 3, 1><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
-                                                  D::omd(L), b, x1, x, qv, dots->ra, e, cc);
+      level_rows<T, 3, 1>(A, G * sp, s, D::omd(L), b, x1, x, qv, dots->ra, e, cc);
     else
-      k_level_rows<T, 3><<<G * sp, kB, 0, s>>>(nrb, rbs, A.rowOff.p, A.cols.p, A.vals.p,
-                                               D::omd(L), b, x1, x, qv, noRed, e, cc);
+      level_rows<T, 3>(A, G * sp, s, D::omd(L), b, x1, x, qv, noRed, e, cc);
     if (dots) {
       AN_CHECK_LAUNCH();
       finalize_parts(dots->ra, G, qv ? 2 : 1, s);

@@ -112,10 +112,12 @@ __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A
 // 2: dot(Y, Y); 3: dot(DW, Y) and dot(Y, Y) into partial[6 blk + c] and
 // partial[6 blk + 3 + c] (per column, k_spmv's row blocks and trees).
 // Columns outside the live mask are neither written nor reduced.
-template <int MODE, int DOT>
+// C16: columns from the 16-bit offsets c16 (ColOffsets) instead of cols.
+template <int MODE, int DOT, bool C16>
 __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ rbs,
                                               const int* __restrict__ rowOff,
                                               const int* __restrict__ cols,
+                                              const int16_t* __restrict__ c16,
                                               const double* __restrict__ vals,
                                               const double* __restrict__ X,
                                               const double* __restrict__ B,
@@ -151,7 +153,7 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
 #pragma unroll
     for (int u = 0; u < kStreamPer; ++u) {
       const int k = min(q.k0 + lane + 64 * u, q.k1 - 1);
-      cidx[u] = cols[k];
+      cidx[u] = C16 ? q.r0 + (int)c16[k] : cols[k];
       v[u] = vals[k];
     }
 #pragma unroll
@@ -174,7 +176,8 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
       if (stream)
         for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
       else
-        for (int k = kb; k < ke; ++k) s += vals[k] * X[3 * cols[k] + c];
+        for (int k = kb; k < ke; ++k)
+          s += vals[k] * X[3 * (C16 ? q.r0 + (int)c16[k] : cols[k]) + c];
       const double yv = (MODE == 1) ? bw[c] - s : s;
       Y[3 * row + c] = yv;
       if (DOT == 1 || DOT == 3) dv[c] = bw[c] * yv;
@@ -195,6 +198,35 @@ __global__ __launch_bounds__(kB) void k3_spmv(int nrb, const int2* __restrict__ 
   } else if (DOT) {
     block_sum3(dv, sh);
     store3(dv, lb, partial);
+  }
+}
+
+// k3_spmv with 16-bit column offsets when the matrix has them; with
+// evStart/evStop the launch is timed (hipExtLaunchKernel)
+template <int MODE, int DOT>
+static void launch_spmv3(const DevCSR<double>& A, int nb, hipStream_t s, const double* X,
+                         const double* B, double* Y, const double* DW, const Bicg3State* st,
+                         double* partial, hipEvent_t evStart = nullptr,
+                         hipEvent_t evStop = nullptr) {
+  const int2* rbs = A.rb.start.p;
+  const int* ro = A.rowOff.p;
+  const int* ci = A.cols.p;
+  const int16_t* c16 = A.c16.p;
+  const double* va = A.vals.p;
+  if (A.has16) {
+    if (evStart)
+      hipExtLaunchKernelGGL(k3_spmv<MODE, DOT, true>, dim3(nb), dim3(kB), 0, s, evStart, evStop,
+                            0, A.rb.count, rbs, ro, ci, c16, va, X, B, Y, DW, st, partial);
+    else
+      k3_spmv<MODE, DOT, true><<<nb, kB, 0, s>>>(A.rb.count, rbs, ro, ci, c16, va, X, B, Y, DW,
+                                                 st, partial);
+  } else {
+    if (evStart)
+      hipExtLaunchKernelGGL(k3_spmv<MODE, DOT, false>, dim3(nb), dim3(kB), 0, s, evStart, evStop,
+                            0, A.rb.count, rbs, ro, ci, c16, va, X, B, Y, DW, st, partial);
+    else
+      k3_spmv<MODE, DOT, false><<<nb, kB, 0, s>>>(A.rb.count, rbs, ro, ci, c16, va, X, B, Y, DW,
+                                                  st, partial);
   }
 }
 
@@ -518,7 +550,6 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   const int G = std::min(grid_for(n, kB), 1024);
   const int Gd = std::min(grid_for(n, kB), 1024);  // dot(): red_blocks
   const int gN = grid_for(n, kB);
-  const int2* rbs = A.rb.start.p;
   double* sc = w.scal.p;
   double* part = w.red.partial.p;
   Bicg3State* st = reinterpret_cast<Bicg3State*>(w.state.p);
@@ -529,8 +560,7 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   // ||b||, r = b - A x, ||r||, the first stop test
   k3_dot<<<Gd, kB, 0, s>>>(n, w.b3.p, w.b3.p, nullptr, part);
   k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * S_BB, nullptr);
-  k3_spmv<1, 2><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.x.p,
-                                   w.b3.p, w.r.p, nullptr, nullptr, part);
+  launch_spmv3<1, 2>(A, nbS, s, w.x.p, w.b3.p, w.r.p, nullptr, nullptr, part);
   k3_fin<3><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_RR, nullptr);
   k3_init<<<1, 64, 0, s>>>(sc, st, activeIn, relTol, absTol);
   k3_zero_cols<<<gN, kB, 0, s>>>(n, w.x.p, st);
@@ -547,20 +577,12 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
     const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;
     k3_bicg_p<<<gN, kB, 0, s>>>(n, w.r.p, w.p.p, w.v.p, dinv, w.ph.p, sc, rhoNew, rhoOld,
                                 it == 1 ? 1 : 0, st);
-    if (evStart && it == 1)
-      hipExtLaunchKernelGGL(k3_spmv<0, 1>, dim3(nbS), dim3(kB), 0, s, evStart, evStop, 0,
-                            A.rb.count, rbs, (const int*)A.rowOff.p, (const int*)A.cols.p,
-                            (const double*)A.vals.p, (const double*)w.ph.p,
-                            (const double*)nullptr, w.v.p, (const double*)w.r0.p,
-                            (const Bicg3State*)st, part);
-    else
-      k3_spmv<0, 1><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.ph.p,
-                                       nullptr, w.v.p, w.r0.p, st, part);
+    launch_spmv3<0, 1>(A, nbS, s, w.ph.p, nullptr, w.v.p, w.r0.p, st, part,
+                       it == 1 ? evStart : nullptr, evStop);
     k3_fin<3><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_R0V, st);
     k3_bicg_s<<<G, kB, 0, s>>>(n, w.r.p, w.v.p, dinv, w.s_.p, w.sh.p, sc, rhoNew, st, part);
     k3_fin<3><<<1, 1024, 0, s>>>(G, part, sc + 3 * S_SS, st);
-    k3_spmv<0, 3><<<nbS, kB, 0, s>>>(A.rb.count, rbs, A.rowOff.p, A.cols.p, A.vals.p, w.sh.p,
-                                     nullptr, w.t.p, w.s_.p, st, part);
+    launch_spmv3<0, 3>(A, nbS, s, w.sh.p, nullptr, w.t.p, w.s_.p, st, part);
     k3_fin<6><<<1, 1024, 0, s>>>(nbS, part, sc + 3 * S_TS, st);  // S_TS[0..2], S_TT[0..2]
     k3_bicg_x<<<G, kB, 0, s>>>(n, w.x.p, w.r.p, w.p.p, w.s_.p, w.t.p, dinv, w.r0.p, sc, st,
                                part);
@@ -599,6 +621,7 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
     inf[c].finalResNorm = zero ? 0.0 : h.rn[c];
     inf[c].finalRelResNorm = zero ? 0.0 : h.rn[c] / h.bnorm[c];
     inf[c].solveSeconds = secs;
+    inf[c].spmvIndexBytes = A.has16 ? 2 : 4;
     if (!h.conv[c]) status = AN_ENOCONV;
   }
   if (info)

@@ -66,6 +66,37 @@ void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s) {
   AN_HIP(hipStreamSynchronize(s));
 }
 
+// one thread per row block: its nonzeros' columns relative to its first row
+__global__ void k_col_offsets(int nrb, const int2* __restrict__ rbs, const int* __restrict__ cols,
+                              int16_t* __restrict__ c16, int* __restrict__ overflow) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nrb) return;
+  const int2 a = rbs[b], e = rbs[b + 1];
+  bool over = false;
+  for (int k = a.y; k < e.y; ++k) {
+    const int d = cols[k] - a.x;
+    over |= d < -32768 || d > 32767;
+    c16[k] = (int16_t)d;  // not used when any offset overflows
+  }
+  if (over) atomicOr(overflow, 1);
+}
+
+bool build_col_offsets(long long nnz, const int* cols, const RowBlocks& rb, DevBuf<int16_t>& c16,
+                       hipStream_t s) {
+  c16.release();
+  if (nnz <= 0 || rb.count == 0) return false;
+  c16.alloc((size_t)nnz);
+  DevBuf<int> over(1);
+  over.zero(s);
+  k_col_offsets<<<grid_for(rb.count, 256), 256, 0, s>>>(rb.count, rb.start.p, cols, c16.p,
+                                                        over.p);
+  AN_CHECK_LAUNCH();
+  int h = 0;
+  AN_HIP(hipMemcpyAsync(&h, over.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  AN_HIP(hipStreamSynchronize(s));
+  if (h) c16.release();
+  return h == 0;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -153,10 +184,12 @@ void Reducer::init(int maxBlocks, hipStream_t s) {
 // MODE 0: y = A x ; MODE 1: y = b - A x.  DOT 0: none; 1: dot(dw, y);
 // 2: dot(y, y); 3 (MODE 1): dot(y, y) and dot(b, b); 4: dot(dw, y) and
 // dot(y, y) (two partials per block for 3 and 4).
-template <typename VT, typename XT, typename YT, int MODE, int DOT>
+// C16: columns from the 16-bit offsets c16 (ColOffsets) instead of cols.
+template <typename VT, typename XT, typename YT, int MODE, int DOT, bool C16>
 __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict__ rbs,
                                                  const int* __restrict__ rowOff,
                                                  const int* __restrict__ cols,
+                                                 const int16_t* __restrict__ c16,
                                                  const VT* __restrict__ vals,
                                                  const XT* __restrict__ x,
                                                  const double* __restrict__ b, YT* __restrict__ y,
@@ -177,8 +210,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
     if (MODE == 1) bi = b[row];
     if (DOT == 1 || DOT == 4) wi = (double)dw[row];
   }
-  if (q.stream && q.k1 > q.k0)
-    stage_products(q, lane, cols, vals, [&](int j) { return (double)x[j]; }, prod[w]);
+  auto xv = [&](int j) { return (double)x[j]; };
+  if (q.stream && q.k1 > q.k0) {
+    if constexpr (C16)
+      stage_products(q, lane, ColOffsets{c16, q.r0}, vals, xv, prod[w]);
+    else
+      stage_products(q, lane, cols, vals, xv, prod[w]);
+  }
   wave_sync();
   double dv = 0.0, db = 0.0;
   if (row < q.r1) {
@@ -186,7 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int nrb, const int2* __restrict
     if (q.stream)
       for (int k = kb; k < ke; ++k) s += prod[w][k - q.k0];
     else
-      for (int k = kb; k < ke; ++k) s += (double)vals[k] * (double)x[cols[k]];
+      for (int k = kb; k < ke; ++k)
+        s += (double)vals[k] * (double)x[C16 ? q.r0 + (int)c16[k] : cols[k]];
     YT v;
     if (MODE == 1)
       v = (YT)(bi - s);
@@ -214,37 +253,50 @@ static int spmv_grid(const DevCSR<VT>& A) {
   return grid_for(A.rb.count, kWaves);
 }
 
+// k_spmv with 16-bit column offsets when the matrix has them; with
+// evStart/evStop the launch is timed (hipExtLaunchKernel)
+template <typename VT, typename XT, typename YT, int MODE, int DOT>
+static void launch_spmv(const DevCSR<VT>& A, int nb, hipStream_t s, const XT* x, const double* b,
+                        YT* y, const YT* dw, RedArgs ra, hipEvent_t evStart = nullptr,
+                        hipEvent_t evStop = nullptr) {
+  const int2* rbs = A.rb.start.p;
+  const int* ro = A.rowOff.p;
+  const int* ci = A.cols.p;
+  const int16_t* c16 = A.c16.p;
+  const VT* va = A.vals.p;
+  if (A.has16) {
+    if (evStart)
+      hipExtLaunchKernelGGL(k_spmv<VT, XT, YT, MODE, DOT, true>, dim3(nb), dim3(kBlock), 0, s,
+                            evStart, evStop, 0, A.rb.count, rbs, ro, ci, c16, va, x, b, y, dw, ra);
+    else
+      k_spmv<VT, XT, YT, MODE, DOT, true><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, c16, va,
+                                                                x, b, y, dw, ra);
+  } else {
+    if (evStart)
+      hipExtLaunchKernelGGL(k_spmv<VT, XT, YT, MODE, DOT, false>, dim3(nb), dim3(kBlock), 0, s,
+                            evStart, evStop, 0, A.rb.count, rbs, ro, ci, c16, va, x, b, y, dw, ra);
+    else
+      k_spmv<VT, XT, YT, MODE, DOT, false><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, c16,
+                                                                 va, x, b, y, dw, ra);
+  }
+  AN_CHECK_LAUNCH();
+}
+
 template <typename VT, typename XT>
 void spmv(const DevCSR<VT>& A, const XT* x, XT* y, const XT* dotWith, double* scal, int slot,
           Reducer& red, hipStream_t s, hipEvent_t evStart, hipEvent_t evStop) {
   const int nb = spmv_grid(A);
-  const int2* rbs = A.rb.start.p;
-  const int* ro = A.rowOff.p;
-  const int* ci = A.cols.p;
-  const VT* va = A.vals.p;
   const double* nob = nullptr;
   if (dotWith) {
     if (nb > red.cap) throw Error(7, "reducer too small");
     const RedArgs ra = red.args(scal + slot);
-    if (evStart)
-      hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 1>, dim3(nb), dim3(kBlock), 0, s, evStart,
-                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, dotWith, ra);
-    else
-      k_spmv<VT, XT, XT, 0, 1><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y,
-                                                     dotWith, ra);
-    AN_CHECK_LAUNCH();
+    launch_spmv<VT, XT, XT, 0, 1>(A, nb, s, x, nob, y, dotWith, ra, evStart, evStop);
     finalize_parts(ra, nb, 1, s);
   } else {
     const XT* nodw = nullptr;
-    const RedArgs none{nullptr, nullptr};
-    if (evStart)
-      hipExtLaunchKernelGGL(k_spmv<VT, XT, XT, 0, 0>, dim3(nb), dim3(kBlock), 0, s, evStart,
-                            evStop, 0, A.rb.count, rbs, ro, ci, va, x, nob, y, nodw, none);
-    else
-      k_spmv<VT, XT, XT, 0, 0><<<nb, kBlock, 0, s>>>(A.rb.count, rbs, ro, ci, va, x, nob, y, nodw,
-                                                     none);
+    launch_spmv<VT, XT, XT, 0, 0>(A, nb, s, x, nob, y, nodw, RedArgs{nullptr, nullptr}, evStart,
+                                  evStop);
   }
-  AN_CHECK_LAUNCH();
 }
 
 template <typename VT, typename RT>
@@ -252,10 +304,7 @@ void residual(const DevCSR<VT>& A, const double* b, const double* x, RT* r, doub
               Reducer& red, hipStream_t s) {
   const int nb = spmv_grid(A);
   if (nb > red.cap) throw Error(7, "reducer too small");
-  k_spmv<VT, double, RT, 1, 2><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
-                                                     A.cols.p, A.vals.p, x, b, r, nullptr,
-                                                     red.args(scal + S_RR));
-  AN_CHECK_LAUNCH();
+  launch_spmv<VT, double, RT, 1, 2>(A, nb, s, x, b, r, (const RT*)nullptr, red.args(scal + S_RR));
   finalize(red, nb, 1, scal + S_RR, s);
 }
 
@@ -264,10 +313,7 @@ int residual_bb(const DevCSR<VT>& A, const double* b, const double* x, RT* r, Re
                 hipStream_t s) {
   const int nb = spmv_grid(A);
   if (2 * nb > 4 * red.cap) throw Error(7, "reducer too small");
-  k_spmv<VT, double, RT, 1, 3><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p,
-                                                     A.cols.p, A.vals.p, x, b, r, nullptr,
-                                                     red.args(nullptr));
-  AN_CHECK_LAUNCH();
+  launch_spmv<VT, double, RT, 1, 3>(A, nb, s, x, b, r, (const RT*)nullptr, red.args(nullptr));
   return nb;
 }
 
@@ -275,10 +321,8 @@ int spmv_dot_self(const DevCSR<double>& A, const double* x, double* y, const dou
                   Reducer& red, hipStream_t s) {
   const int nb = spmv_grid(A);
   if (2 * nb > 4 * red.cap) throw Error(7, "reducer too small");
-  k_spmv<double, double, double, 0, 4><<<nb, kBlock, 0, s>>>(
-      A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p, A.vals.p, x, nullptr, y, dw,
-      red.args(nullptr));
-  AN_CHECK_LAUNCH();
+  launch_spmv<double, double, double, 0, 4>(A, nb, s, x, (const double*)nullptr, y, dw,
+                                            red.args(nullptr));
   return nb;
 }
 
@@ -286,10 +330,8 @@ template <typename VT, typename XT>
 int spmv_self_dot(const DevCSR<VT>& A, const XT* x, XT* y, Reducer& red, hipStream_t s) {
   const int nb = spmv_grid(A);
   if (nb > red.cap) throw Error(7, "reducer too small");
-  k_spmv<VT, XT, XT, 0, 2><<<nb, kBlock, 0, s>>>(A.rb.count, A.rb.start.p, A.rowOff.p, A.cols.p,
-                                                 A.vals.p, x, nullptr, y, nullptr,
-                                                 red.args(nullptr));
-  AN_CHECK_LAUNCH();
+  launch_spmv<VT, XT, XT, 0, 2>(A, nb, s, x, (const double*)nullptr, y, (const XT*)nullptr,
+                                red.args(nullptr));
   return nb;
 }
 

@@ -23,7 +23,17 @@ struct RowBlocks {
 // Greedy partition of rowOff[0..n] (synchronises once to read the count).
 void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s);
 
-// Device CSR matrix (values in T).
+// Column offsets of the CSR-stream kernels: c16[k] = cols[k] - r0, r0 the
+// first row of the row block holding nonzero k, as 16-bit integers, when
+// every offset of the matrix fits (|col - row| below about 32 700: true of
+// banded mesh orderings); returns false, leaving c16 empty, otherwise.  The
+// streaming kernels then read 2 bytes of column index per nonzero instead
+// of 4.  Synchronises once.
+bool build_col_offsets(long long nnz, const int* cols, const RowBlocks& rb, DevBuf<int16_t>& c16,
+                       hipStream_t s);
+
+// Device CSR matrix (values in T).  c16 (with has16) holds the column
+// offsets above; the int32 cols stay for every other use.
 template <typename T>
 struct DevCSR {
   int n = 0;
@@ -31,7 +41,22 @@ struct DevCSR {
   DevBuf<int> rowOff, cols, diag;
   DevBuf<T> vals;
   RowBlocks rb;
-  void finish_pattern(hipStream_t s) { build_row_blocks(n, rowOff.p, rb, s); }
+  DevBuf<int16_t> c16;
+  bool has16 = false;
+  // offsets16 false: no column offsets (callers that size their memory by
+  // rows only)
+  void finish_pattern(hipStream_t s, bool offsets16 = true) {
+    build_row_blocks(n, rowOff.p, rb, s);
+    has16 = offsets16 && build_col_offsets(nnz, cols.p, rb, c16, s);
+  }
+};
+
+// Column of nonzero k from the 16-bit offsets of its row block (first row
+// r0); indexes like the int32 column array it stands for.
+struct ColOffsets {
+  const int16_t* __restrict__ p;
+  int r0;
+  __device__ __forceinline__ int operator[](int k) const { return r0 + (int)p[k]; }
 };
 
 // Waves synchronise their own LDS slice without a block barrier.
@@ -66,9 +91,9 @@ __device__ __forceinline__ WaveRows wave_rows(int nrb, const int2* __restrict__ 
 // rather than one per nonzero.  Slots past the block's end re-read its last
 // nonzero (no divergent branches between the loads, so the compiler keeps
 // them all in flight) and are not stored.  Requires k1 > k0.
-template <typename VT, typename XF>
-__device__ __forceinline__ void stage_products(const WaveRows& q, int lane,
-                                               const int* __restrict__ cols,
+// cols: the int32 column array or the block's ColOffsets.
+template <typename VT, typename CI, typename XF>
+__device__ __forceinline__ void stage_products(const WaveRows& q, int lane, CI cols,
                                                const VT* __restrict__ vals, XF xval,
                                                double* __restrict__ prod) {
   int c[kStreamPer];

@@ -111,7 +111,7 @@ extern "C" an_status an_bicgstab_solve_noalloc(int n, const int* rowOffsets, con
   A.vals.p = const_cast<double*>(values);
   A.vals.n = (size_t)nnz;
   A.vals.pool = &arena;
-  A.finish_pattern(0);
+  A.finish_pattern(0, false);  // no 16-bit column offsets: the scratch is sized by rows
   if (scratch_need(n, A.rb.count) > kScratchBytes)
     throw Error(AN_EINVAL, "an_bicgstab_solve_noalloc: the matrix's " +
                                std::to_string(A.rb.count) + " row blocks need more than the " +

@@ -101,6 +101,7 @@ PcgSolver::EvPair* PcgSolver::next_ev(std::vector<EvPair>& pool, size_t& used) {
 }
 
 void PcgSolver::collect_profile(an_solve_info* inf) {
+  inf->spmvIndexBytes = A.has16 ? 2 : 4;
   if (!opt.profile) return;
   AN_HIP(hipStreamSynchronize(s));
   double t = 0.0;
@@ -930,6 +931,7 @@ an_status bicgstab_device(const DevCSR<double>& A, const double* b, double* x, i
   inf.finalResNorm = rn;
   inf.finalRelResNorm = rn / bnorm;
   inf.solveSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  inf.spmvIndexBytes = A.has16 ? 2 : 4;
   if (!conv) st = AN_ENOCONV;
   if (info) *info = inf;
   return st;

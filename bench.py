@@ -194,7 +194,7 @@ def run_momentum3(n):
     mu = 1e-3, Dirichlet inflow faces), solved as three separate BiCGSTAB
     solves and as one fused 3-RHS solve; reports both times, whether the
     results are bitwise equal, and the fused SpMV's bandwidth
-    (12 nnz + 4(nC+1) + 48 nC algorithmic bytes)."""
+    ((8 + index bytes) nnz + 4(nC+1) + 48 nC algorithmic bytes)."""
     from anabasis_amd import libpoisson as lp
     t0 = time.time()
     m = lp.Mesh.hexbox(n, n, n, 1.0, 1.0, 1.0, jitter=0.2, seed=SEED + 1)
@@ -226,7 +226,8 @@ def run_momentum3(n):
     xs, infos = lp.bicgstab3(pat.rowOffsets, pat.cols, vals, bs, **kw)
     t_fused = time.time() - t0
     same = all(np.array_equal(xs[c], seps[c][0]) for c in range(3))
-    alg = 12 * pat.nnz + 4 * (m.nCells + 1) + 48 * m.nCells
+    idx = int(infos[0].spmvIndexBytes or 4)  # column index bytes per nonzero read
+    alg = (8 + idx) * pat.nnz + 4 * (m.nCells + 1) + 48 * m.nCells
     ts = infos[0].spmvAvgSeconds
     conv = [bool(i.converged) for i in infos] + [bool(s[1].converged) for s in seps]
     dev = momentum3_device(pat, vals, bs, kw["maxIter"], kw["absTol"], xs)
@@ -624,7 +625,8 @@ def run_psolve(n, reps):
                          refinement_steps=infos[-1].refinementSteps, levels=infos[-1].levels,
                          spmv_avg_s=float(np.mean([i.spmvAvgSeconds for i in infos])),
                          spmv_calls=int(sum(i.spmvCalls for i in infos)),
-                         vcycle_avg_s=float(np.mean([i.vcycleAvgSeconds for i in infos])), x=x)
+                         vcycle_avg_s=float(np.mean([i.vcycleAvgSeconds for i in infos])),
+                         index_bytes=int(infos[-1].spmvIndexBytes or 4), x=x)
         r = res[name]
         log(f"psolve {name}: setup {setup:.3f}s solve {r['solve_s']:.4f}s its {r['iterations']} "
             f"relres {relres:.2e} spmv {r['spmv_avg_s'] * 1e6:.1f}us vcycle "
@@ -730,7 +732,10 @@ def main():
     if not args.no_psolve:
         pm, pat, ps = run_psolve(args.pn, args.psolve_reps)
         nC, nnz = pm.nCells, pat.nnz
-        alg = 12 * nnz + 4 * (nC + 1) + 16 * nC
+        # 8 B value + the column index the kernel reads (2 B: 16-bit offsets
+        # from the row block's first row, when the matrix's bandwidth allows)
+        idx = ps["fp64"]["index_bytes"]
+        alg = (8 + idx) * nnz + 4 * (nC + 1) + 16 * nC
         t = ps["fp64"]["spmv_avg_s"]
         achieved = alg / t / 1e9
         traffic = spmv_traffic(alg)
@@ -739,7 +744,8 @@ def main():
                 "kernel": "k_spmv<double,double,double,0,1> (PCG SpMV + fused p.q), "
                           f"{args.pn}^3 pressure matrix",
                 "alg_bytes_per_launch": alg, "avg_launch_s": t,
-                "launches_timed": ps["fp64"]["spmv_calls"]}
+                "launches_timed": ps["fp64"]["spmv_calls"], "index_bytes": idx,
+                "alg_bytes_int32_index": 12 * nnz + 4 * (nC + 1) + 16 * nC}
     mom3 = run_momentum3(args.mom3_n) if args.mom3_n > 0 else None
     mms = run_mms_tet(args.mms_n) if args.mms_n > 0 else None
     case2 = run_cylinder_case(args.case_cells, 500) if args.case_cells > 0 else None

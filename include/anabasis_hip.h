@@ -209,6 +209,10 @@ typedef struct {
   double spmvAvgSeconds;
   int vcycleCalls;
   double vcycleAvgSeconds;
+  /* bytes of column index the Krylov SpMV reads per nonzero: 2 when every
+   * column of the matrix fits a 16-bit offset from its CSR row block's
+   * first row (banded mesh orderings), else 4 */
+  int spmvIndexBytes;
 } an_solve_info;
 
 typedef struct an_solver_s *an_solver_t;
