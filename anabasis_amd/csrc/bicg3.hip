@@ -248,13 +248,22 @@ __global__ __launch_bounds__(kB) void k3_bicg_p(int n, const double* __restrict_
                     : (sc[3 * rhoNew + c] / sc[3 * rhoOld + c]) * (sc[3 * S_AL + c] / sc[3 * S_OM + c]);
     om[c] = first ? 0.0 : sc[3 * S_OM + c];
   }
+  // every column's operands are loaded together (a frozen column's are
+  // read and dropped): loads behind a per-column branch waited one by one
   for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
     const double di = dinv[i];
+    double rv[3], pv[3], vv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rv[c] = R[3 * i + c];
+      pv[c] = first ? 0.0 : P[3 * i + c];
+      vv[c] = first ? 0.0 : V[3 * i + c];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (!((mask >> c) & 1)) continue;
       const int e = 3 * i + c;
-      const double pi = first ? R[e] : R[e] + beta[c] * (P[e] - om[c] * V[e]);
+      const double pi = first ? rv[c] : rv[c] + beta[c] * (pv[c] - om[c] * vv[c]);
       P[e] = pi;
       PH[e] = di * pi;
     }

@@ -426,23 +426,43 @@ int PcgSolver::pcg32(const float* b, float* x, double tol, int maxIter, double r
 // start is never worse than x0 in the norm the stop test uses.
 static constexpr int kProjMax = 16;
 
-// out[i] = V_i . y, i < m (block partials, m per block)
-__global__ __launch_bounds__(kB) void k_mdot(int n, int m, const double* __restrict__ V,
+// out[i] = V_i . y, i < M (block partials, M per block).  M is a template
+// argument so that a thread issues its M loads of a row together (a
+// run-time count put each load behind its own branch and wait).
+template <int M>
+__global__ __launch_bounds__(kB) void k_mdot(int n, const double* __restrict__ V,
                                              const double* __restrict__ y,
                                              double* __restrict__ partial) {
-  __shared__ double sh[kProjMax * (kB / 64)];
-  double d[kProjMax];
+  __shared__ double sh[M * (kB / 64)];
+  double d[M];
 #pragma unroll
-  for (int i = 0; i < kProjMax; ++i) d[i] = 0.0;
+  for (int i = 0; i < M; ++i) d[i] = 0.0;
   for (int j = blockIdx.x * kB + threadIdx.x; j < n; j += gridDim.x * kB) {
     const double yj = y[j];
+    double v[M];
 #pragma unroll
-    for (int i = 0; i < kProjMax; ++i)
-      if (i < m) d[i] += V[(size_t)i * n + j] * yj;
+    for (int i = 0; i < M; ++i) v[i] = V[(size_t)i * n + j];
+#pragma unroll
+    for (int i = 0; i < M; ++i) d[i] += v[i] * yj;
   }
-  block_sums<kProjMax>(d, sh);
+  block_sums<M>(d, sh);
   if (threadIdx.x == 0)
-    for (int i = 0; i < m; ++i) partial[(size_t)blockIdx.x * m + i] = d[i];
+#pragma unroll
+    for (int i = 0; i < M; ++i) partial[(size_t)blockIdx.x * M + i] = d[i];
+}
+
+template <int M>
+static void launch_mdot(int m, int nb, hipStream_t s, int n, const double* V, const double* y,
+                        double* partial) {
+  if constexpr (M <= kProjMax) {
+    if (m == M) {
+      k_mdot<M><<<nb, kB, 0, s>>>(n, V, y, partial);
+      return;
+    }
+    launch_mdot<M + 1>(m, nb, s, n, V, y, partial);
+  } else {
+    throw Error(7, "projection count out of range");
+  }
 }
 
 // x += sx * X c and y += sy * B c in one pass (sums in index order); the
@@ -545,7 +565,7 @@ void PcgSolver::proj_coef(const double* rr) {
   // partial buffer (4 cap doubles) holds for projM components
   const int nb =
       std::min({grid_for(n, kB), 1024, 4 * red.cap / std::max(projM, 1)});
-  k_mdot<<<nb, kB, 0, s>>>(n, projM, projB.p, rr, red.partial.p);
+  launch_mdot<1>(projM, nb, s, n, projB.p, rr, red.partial.p);
   AN_CHECK_LAUNCH();
   finalize(red, nb, projM, scal.p + S_PROJ, s);
 }
