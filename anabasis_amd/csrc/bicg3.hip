@@ -80,8 +80,11 @@ __device__ __forceinline__ int live_mask(const Bicg3State* st) { return st ? st-
 
 // partial[3 blk + c] = block's share of sum_i A[3i+c] B[3i+c] (k_dot's map);
 // nothing once no column is live
+// COPY: B is not read but written, B = A, and the sums are A.A (the bits of
+// a separate copy and dot of the two).
+template <bool COPY = false>
 __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A,
-                                             const double* __restrict__ B,
+                                             double* __restrict__ B,
                                              const Bicg3State* st,
                                              double* __restrict__ partial) {
   __shared__ double sh[3][kW];
@@ -94,9 +97,16 @@ __global__ __launch_bounds__(kB) void k3_dot(int n, const double* __restrict__ A
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       a[0][c] = A[3 * i + c];
-      b[0][c] = B[3 * i + c];
       a[1][c] = A[3 * j + c];
-      b[1][c] = B[3 * j + c];
+      if (COPY) {
+        b[0][c] = a[0][c];
+        b[1][c] = a[1][c];
+        B[3 * i + c] = a[0][c];
+        B[3 * j + c] = a[1][c];
+      } else {
+        b[0][c] = B[3 * i + c];
+        b[1][c] = B[3 * j + c];
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] += a[0][c] * b[0][c];
@@ -550,7 +560,8 @@ static inline bool speculate3(double q, double qPrev) {
 an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], double* const x[3],
                            int maxIter, double relTol, double absTol, Bicg3Work& w,
                            hipStream_t s, an_solve_info info[3], int activeIn,
-                           hipEvent_t evStart, hipEvent_t evStop) {
+                           hipEvent_t evStart, hipEvent_t evStop, const double* const* x0,
+                           const double* dinvIn) {
   const int n = A.n;
   auto t0 = std::chrono::steady_clock::now();
   const int nbS = grid_for(A.rb.count, kW);
@@ -562,9 +573,13 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   double* sc = w.scal.p;
   double* part = w.red.partial.p;
   Bicg3State* st = reinterpret_cast<Bicg3State*>(w.state.p);
-  k3_dinv<<<gN, kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
+  // the Jacobi scaling: the caller's (formed where the diagonal was
+  // assembled) or 1 / the diagonal entry of each row
+  if (!dinvIn) k3_dinv<<<gN, kB, 0, s>>>(n, A.rowOff.p, A.cols.p, A.vals.p, w.dinv.p);
   k3_pack<<<gN, kB, 0, s>>>(n, b[0], b[1], b[2], w.b3.p);
-  k3_pack<<<gN, kB, 0, s>>>(n, x[0], x[1], x[2], w.x.p);
+  // the start: x0 when given (x is then written only at the end)
+  const double* const* xs = x0 ? x0 : x;
+  k3_pack<<<gN, kB, 0, s>>>(n, xs[0], xs[1], xs[2], w.x.p);
   AN_CHECK_LAUNCH();
   // ||b||, r = b - A x, ||r||, the first stop test
   k3_dot<<<Gd, kB, 0, s>>>(n, w.b3.p, w.b3.p, nullptr, part);
@@ -574,13 +589,13 @@ an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], do
   k3_init<<<1, 64, 0, s>>>(sc, st, activeIn, relTol, absTol);
   k3_zero_cols<<<gN, kB, 0, s>>>(n, w.x.p, st);
   AN_CHECK_LAUNCH();
-  AN_HIP(hipMemcpyAsync(w.r0.p, w.r.p, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, s));
-  // rho of the first iteration; each later one comes out of k3_bicg_x
-  k3_dot<<<Gd, kB, 0, s>>>(n, w.r0.p, w.r.p, st, part);
+  // r0 = r and rho of the first iteration = r0.r in one pass; each later
+  // rho comes out of k3_bicg_x
+  k3_dot<true><<<Gd, kB, 0, s>>>(n, w.r.p, w.r0.p, st, part);
   k3_fin<3><<<1, 1024, 0, s>>>(Gd, part, sc + 3 * S_RHO1, st);
   AN_CHECK_LAUNCH();
   const unsigned base = w.stamp;
-  const double* dinv = w.dinv.p;
+  const double* dinv = dinvIn ? dinvIn : w.dinv.p;
   auto queue_iteration = [&](int it) {
     const int rhoNew = (it & 1) ? S_RHO1 : S_RHO0;
     const int rhoOld = (it & 1) ? S_RHO0 : S_RHO1;

@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
                                                   const double* __restrict__ coef,
                                                   double* __restrict__ vals,
                                                   double* __restrict__ rAU, int nrb,
-                                                  const int2* __restrict__ rbs) {
+                                                  const int2* __restrict__ rbs,
+                                                  double* __restrict__ dinv) {
   __shared__ double sv[FULL ? kMomW : 1][FULL ? kStreamCap : 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   WaveRows wr{0, 0, 0, 0, false};
@@ -320,6 +321,9 @@ __global__ __launch_bounds__(kB) void k_mom_cells(MeshView m, Mom3Args a,
     }
     *slotp(dp) = aR;
     rAU[c] = (fabs(aR) > 1e-30) ? V / aR : 0.0;
+    // the Jacobi scaling of the momentum solves (the expression of their
+    // own set-up from the diagonal entry)
+    dinv[c] = (aR != 0.0) ? 1.0 / aR : 1.0;
     aPold = aR * a.uRelax;
   } else {
     aPold = vals[dp] * a.uRelax;
@@ -701,19 +705,34 @@ __device__ __forceinline__ double wmax(double v) {
 }
 
 // per-block partials of sum (a-b)^2, sum a^2, max |a| (b may be null)
+// Each thread takes its grid-stride elements four at a time, all loads of
+// the four issued together and the sums taken in element order (the bits of
+// one element at a time).
+template <bool HAS_B>
 __global__ __launch_bounds__(kB) void k_stats(int n, const double* __restrict__ a,
                                               const double* __restrict__ b,
                                               double* __restrict__ partial) {
   __shared__ double sh[3][kB / 64];
   double s0 = 0.0, s1 = 0.0, mx = 0.0;
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
-    const double ai = a[i];
-    if (b) {
-      const double d = ai - b[i];
-      s0 += d * d;
+  const int S = gridDim.x * kB;
+  for (int i0 = blockIdx.x * kB + threadIdx.x; i0 < n; i0 += 4 * S) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * S;
+      av[u] = i < n ? a[i] : 0.0;
+      bv[u] = (HAS_B && i < n) ? b[i] : 0.0;
     }
-    s1 += ai * ai;
-    mx = fmax(mx, fabs(ai));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (i0 + u * S >= n) break;
+      if (HAS_B) {
+        const double d = av[u] - bv[u];
+        s0 += d * d;
+      }
+      s1 += av[u] * av[u];
+      mx = fmax(mx, fabs(av[u]));
+    }
   }
   s0 = wsum(s0);
   s1 = wsum(s1);
@@ -906,6 +925,7 @@ SimpleSolver::SimpleSolver(DevMesh& mesh, const an_simple_params& p_, const int*
     }
     momCorr.alloc(3 * (size_t)M.nIF);
     momCoef.alloc(4 * (size_t)M.nIF);
+    momDinv.alloc(nC);
   }
   p.alloc(nC);
   p.zero(s);
@@ -965,7 +985,10 @@ void SimpleSolver::grad(int bcKind, const double* f, DevBuf<double>* out) {
 
 double SimpleSolver::relchg(const double* a, const double* b) {
   const int G = std::min(grid_for(M.nC, kB), 340);
-  k_stats<<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  if (b)
+    k_stats<true><<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  else
+    k_stats<false><<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
   k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p);
   AN_CHECK_LAUNCH();
   AN_HIP(hipMemcpyAsync(hs.h, scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -976,7 +999,10 @@ double SimpleSolver::relchg(const double* a, const double* b) {
 void SimpleSolver::queue_stats(const double* a, const double* b, int k) {
   if (k < 0 || k >= kStatSlots) throw Error(7, "statistics slot out of range");
   const int G = std::min(grid_for(M.nC, kB), 340);
-  k_stats<<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  if (b)
+    k_stats<true><<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
+  else
+    k_stats<false><<<G, kB, 0, s>>>(M.nC, a, b, red.partial.p);
   k_stats_fin<<<1, 256, 0, s>>>(G, red.partial.p, scal.p + 3 * k);
   AN_CHECK_LAUNCH();
 }
@@ -1057,8 +1083,8 @@ void SimpleSolver::momentum_component(int comp, an_simple_stats* st) {
 void SimpleSolver::momentum_fused(an_simple_stats* st) {
   const int nC = M.nC;
   const int nVel = std::max(par.nVelNonOrthCorr, 0) + 1;
-  for (int c = 0; c < 3; ++c)
-    AN_HIP(hipMemcpyAsync(q[c].p, qOld[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
+  // the iterate starts at the old velocity (qIter = qOld): iterate() has
+  // just made q[c] equal to qOld[c] and nothing has written either since
   int active = 7;
   for (int it = 1; it <= nVel && active; ++it) {
     mark(0);
@@ -1105,22 +1131,22 @@ void SimpleSolver::momentum_fused(an_simple_stats* st) {
     if (it == 1) {  // active == 7 here: the x component builds the matrix
       k_mom_faces<true><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
       k_mom_cells<true><<<grid_for(Amom.rb.count, kMomW), kB, 0, s>>>(
-          mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p, Amom.rb.count, Amom.rb.start.p);
+          mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p, Amom.rb.count, Amom.rb.start.p,
+          momDinv.p);
     } else {
       k_mom_faces<false><<<Gf, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p);
       k_mom_cells<false><<<Gc, kB, 0, s>>>(mv, a3, momCorr.p, momCoef.p, Amom.vals.p, rAU.p, 0,
-                                           nullptr);
+                                           nullptr, nullptr);
     }
     AN_CHECK_LAUNCH();
-    for (int c = 0; c < 3; ++c)
-      if ((active >> c) & 1)
-        AN_HIP(hipMemcpyAsync(qNew3[c].p, q[c].p, sizeof(double) * nC, hipMemcpyDeviceToDevice, s));
     mark(1);
+    // the solves start from the iterate q and leave the new one in qNew3
     const double* b3[3] = {rhs3[0].p, rhs3[1].p, rhs3[2].p};
     double* x3[3] = {qNew3[0].p, qNew3[1].p, qNew3[2].p};
+    const double* x03[3] = {q[0].p, q[1].p, q[2].p};
     an_solve_info info[3] = {};
     (void)bicgstab3_device(Amom, b3, x3, par.velMaxit, par.velRelTol, par.velTol, bw3, s, info,
-                           active);
+                           active, nullptr, nullptr, x03, momDinv.p);
     mark(2);
     AN_HIP(hipStreamSynchronize(s));
     tAcc[0] += since(0, 1);

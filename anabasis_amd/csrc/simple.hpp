@@ -59,6 +59,7 @@ class SimpleSolver {
   Bicg3Work bw3;
   DevBuf<double> g3[3][3], rhs3[3], qNew3[3];
   DevBuf<double> momCorr, momCoef;  // per internal face: 3 corrections, 4 matrix terms
+  DevBuf<double> momDinv;           // 1 / diagonal of the momentum matrix (Jacobi)
   Reducer red;
   DevBuf<double> scal;
   PinnedScalars hs;

@@ -211,7 +211,8 @@ struct Bicg3Work {
 an_status bicgstab3_device(const DevCSR<double>& A, const double* const b[3], double* const x[3],
                            int maxIter, double relTol, double absTol, Bicg3Work& w,
                            hipStream_t s, an_solve_info info[3], int activeMask = 7,
-                           hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
+                           hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr,
+                           const double* const* x0 = nullptr, const double* dinv = nullptr);
 
 void default_pcg_options(an_pcg_options* o);
 
