@@ -67,13 +67,16 @@ void build_row_blocks(int n, const int* rowOff, RowBlocks& rb, hipStream_t s) {
 }
 
 // one thread per row block: its nonzeros' columns relative to its first row
-__global__ void k_col_offsets(int nrb, const int2* __restrict__ rbs, const int* __restrict__ cols,
-                              int16_t* __restrict__ c16, int* __restrict__ overflow) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nrb) return;
+// one wave per row block, its lanes across the block's nonzeros
+__global__ __launch_bounds__(256) void k_col_offsets(int nrb, const int2* __restrict__ rbs,
+                                                     const int* __restrict__ cols,
+                                                     int16_t* __restrict__ c16,
+                                                     int* __restrict__ overflow) {
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (b >= nrb) return;  // wave-uniform
   const int2 a = rbs[b], e = rbs[b + 1];
   bool over = false;
-  for (int k = a.y; k < e.y; ++k) {
+  for (int k = a.y + lane; k < e.y; k += 64) {
     const int d = cols[k] - a.x;
     over |= d < -32768 || d > 32767;
     c16[k] = (int16_t)d;  // not used when any offset overflows
@@ -88,8 +91,7 @@ bool build_col_offsets(long long nnz, const int* cols, const RowBlocks& rb, DevB
   c16.alloc((size_t)nnz);
   DevBuf<int> over(1);
   over.zero(s);
-  k_col_offsets<<<grid_for(rb.count, 256), 256, 0, s>>>(rb.count, rb.start.p, cols, c16.p,
-                                                        over.p);
+  k_col_offsets<<<grid_for(rb.count, 4), 256, 0, s>>>(rb.count, rb.start.p, cols, c16.p, over.p);
   AN_CHECK_LAUNCH();
   int h = 0;
   AN_HIP(hipMemcpyAsync(&h, over.p, sizeof(int), hipMemcpyDeviceToHost, s));
