@@ -294,13 +294,16 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
   double alpha[3], acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < 3; ++c) alpha[c] = sc[3 * rhoNew + c] / sc[3 * S_R0V + c];
+  // a thread's grid-stride elements four at a time, all loads first, the
+  // sums in element order (the bits of one element at a time)
+  constexpr int U = 4;
   const int St = gridDim.x * kB;
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
-    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
-    const int el[2] = {i, j};
-    double di[2], r[2][3], v[2][3];
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += U * St) {
+    int el[U];
+    double di[U], r[U][3], v[U][3];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
+      el[u] = min(i + u * St, n - 1);
       di[u] = dinv[el[u]];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -309,8 +312,8 @@ __global__ __launch_bounds__(kB) void k3_bicg_s(int n, const double* __restrict_
       }
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (u == 1 && j == i) break;
+    for (int u = 0; u < U; ++u) {
+      if (i + u * St >= n) break;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (!((mask >> c) & 1)) continue;
@@ -358,13 +361,16 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
     omega[c] = (tt > 0.0) ? sc[3 * S_TS + c] / tt : 0.0;
     convS[c] = sqrt(sc[3 * S_SS + c]) <= st->tol[c];
   }
+  // two grid-stride elements per step, loads first (three or four take
+  // 132 or 169 VGPRs: fewer than the grid's four waves per SIMD fit)
+  constexpr int U = 2;
   const int St = gridDim.x * kB;
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += 2 * St) {
-    const int j = (i + St < n) ? i + St : i;  // second element of the pair (or i again)
-    const int el[2] = {i, j};
-    double di[2], xv[2][3], pv[2][3], sv[2][3], tv[2][3], r0[2][3];
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += U * St) {
+    int el[U];
+    double di[U], xv[U][3], pv[U][3], sv[U][3], tv[U][3], r0[U][3];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
+      el[u] = min(i + u * St, n - 1);
       di[u] = dinv[el[u]];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -377,8 +383,8 @@ __global__ __launch_bounds__(kB) void k3_bicg_x(int n, double* __restrict__ X,
       }
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (u == 1 && j == i) break;
+    for (int u = 0; u < U; ++u) {
+      if (i + u * St >= n) break;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (!((mask >> c) & 1)) continue;

@@ -398,12 +398,32 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, T* __restrict__ x,
   const double d = scal[den], a = scal[num] / d;
   const double alpha = (d > 0.0 && isfinite(a)) ? a : 0.0;
   double v = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-    const T ri = (T)((double)r[i] - alpha * (double)q[i]);
-    r[i] = ri;
-    if (y) y[i] = (T)((double)omd[i] * (double)ri);  // the preconditioner's first sweep
-    v += (double)ri * (double)ri;
+  // a thread's grid-stride elements four at a time: all loads of the four
+  // first, then the updates and r.r in element order (the bits of one
+  // element at a time)
+  constexpr int U = 4;
+  const int S = gridDim.x * kBlock;
+  for (int i0 = blockIdx.x * kBlock + threadIdx.x; i0 < n; i0 += U * S) {
+    T xv[U], pv[U], rv[U], qv[U], ov[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = min(i0 + u * S, n - 1);
+      xv[u] = x[i];
+      pv[u] = p[i];
+      rv[u] = r[i];
+      qv[u] = q[i];
+      ov[u] = y ? omd[i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * S;
+      if (i >= n) break;
+      x[i] = (T)((double)xv[u] + alpha * (double)pv[u]);
+      const T ri = (T)((double)rv[u] - alpha * (double)qv[u]);
+      r[i] = ri;
+      if (y) y[i] = (T)((double)ov[u] * (double)ri);  // the preconditioner's first sweep
+      v += (double)ri * (double)ri;
+    }
   }
   const double bs = block_sum(v, sh);
   store_partials<1>(&bs, blockIdx.x, ra);
@@ -448,14 +468,33 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_p2(int n, double* __restr
   const double d = scal[den], a = scal[num] / d;
   const double alpha = (d > 0.0 && isfinite(a)) ? a : 0.0;
   double v = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x[i] = x[i] + alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const float rf = (float)ri;
-    r32[i] = rf;
-    y32[i] = (float)((double)omd32[i] * (double)rf);
-    v += ri * ri;
+  // four grid-stride elements per step, loads first (k_pcg_update)
+  constexpr int U = 4;
+  const int S = gridDim.x * kBlock;
+  for (int i0 = blockIdx.x * kBlock + threadIdx.x; i0 < n; i0 += U * S) {
+    double xv[U], pv[U], rv[U], qv[U];
+    float ov[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = min(i0 + u * S, n - 1);
+      xv[u] = x[i];
+      pv[u] = p[i];
+      rv[u] = r[i];
+      qv[u] = q[i];
+      ov[u] = omd32[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * S;
+      if (i >= n) break;
+      x[i] = xv[u] + alpha * pv[u];
+      const double ri = rv[u] - alpha * qv[u];
+      r[i] = ri;
+      const float rf = (float)ri;
+      r32[i] = rf;
+      y32[i] = (float)((double)ov[u] * (double)rf);
+      v += ri * ri;
+    }
   }
   const double bs = block_sum(v, sh);
   store_partials<1>(&bs, blockIdx.x, ra);
