@@ -30,7 +30,8 @@ static constexpr int kCap = kStreamCap;
 // cycle's small kernels read what the previous one wrote from one L2.
 // Measured on the 2.08 M-cell pressure hierarchy: V-cycle 422 -> 417 us;
 // spreading the 24 k-row level as well was slower (467 us: its 229 blocks
-// then share one XCD's 32 CUs).
+// then share one XCD's 32 CUs); with four matchings per level, taking the
+// 5.3 k-row level in as well (a limit of 8192) made no difference.
 static constexpr int kSpreadRows = 4096;
 static inline int spread_of(int n) { return n <= kSpreadRows ? 8 : 1; }
 
